@@ -1,0 +1,131 @@
+/*
+ * mdft_gpu.h -- C-ABI of the MI355X MDFT functional engine.
+ *
+ * The drop-in boundary is the reference call
+ *     subroutine energy_and_gradient(f, df)
+ * (src/module_energy_and_gradient.f90:36,63-64), which sums the ideal+external
+ * term (src/module_energy_ideal_and_external.f90:14-74) and the excess term
+ * bound to the procedure pointer excess_energy (src/module_proc_ptrs.f90:3-21,
+ * bound to energy_cproj_mrso at src/module_init_simu.f90:60-73).  Fortran
+ * hosts bind these entry points with ISO_C_BINDING (module mdft_gpu in
+ * mdft-dev_general_amd/fortran/mdft_gpu.f90).
+ *
+ * All entry points return 0 on success and a negative MDFT_E* code on error;
+ * mdft_last_error() then describes the failure (the reference stops the
+ * program with `error stop` in the same situations).  A context belongs to one
+ * host thread and runs all its work on one HIP stream of its own.
+ *
+ * Array layouts follow the reference (Fortran column-major, leftmost fastest):
+ *   xi, vext, df : double [nz][ny][nx][no]   (reference xi(no,nx,ny,nz))
+ *   orientation index io = ipsi + npsi*(iphi + nphi*itheta)
+ */
+#ifndef MDFT_GPU_H
+#define MDFT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdft_ctx mdft_ctx;
+
+enum {
+    MDFT_OK = 0,
+    MDFT_EINVAL = -1,   /* invalid argument or inconsistent sizes */
+    MDFT_ENOMEM = -2,   /* device allocation failed */
+    MDFT_EHIP = -3,     /* HIP runtime error */
+    MDFT_ESTATE = -4,   /* call order violated (e.g. evaluation before set_ck) */
+    MDFT_ERANGE = -5,   /* c(k) table does not cover the grid's |q| range */
+    MDFT_ENODEV = -6    /* no usable GPU */
+};
+
+/* Precision of the bulk arithmetic (transforms, FFTs, OZ contraction).  The
+ * free-energy reductions, the ideal term and the gradient stay FP64. */
+enum { MDFT_PREC_FP64 = 0, MDFT_PREC_FP32 = 1, MDFT_PREC_BF16 = 2 };
+
+/* flags for evaluation calls */
+enum {
+    MDFT_DEVICE_PTRS = 1, /* xi / df / vext arguments are device pointers */
+    MDFT_TIMERS = 2       /* record per-phase HIP event timings (mdft_phase_times) */
+};
+
+typedef struct mdft_desc {
+    int nx, ny, nz;        /* grid nodes, reference `boxnod` (module_grid.f90:74) */
+    double lx, ly, lz;     /* box lengths in Angstrom, `boxlen` (module_grid.f90:63) */
+    int mmax;              /* angular order, `mmax` (module_grid.f90:95), 0..6 */
+    int mrso;              /* molecular rotational symmetry order, `molrotsymorder` */
+    double kT;             /* kJ/mol (module_thermo.f90:26) */
+    double n0;             /* bulk number density, 1/A^3 (module_solvent.f90:221) */
+    int precision;         /* MDFT_PREC_* */
+    int device;            /* HIP device ordinal */
+} mdft_desc;
+
+/* Creates the device state: quadrature, Wigner-d and k tables, the q/-q pair
+ * schedule and all field buffers.  Replaces the one-time set-up done inside
+ * energy_cproj_mrso (src/module_energy_cproj_mrso.f90:118-267) and grid%init
+ * (src/module_grid.f90:51-212). */
+int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out);
+void mdft_ctx_destroy(mdft_ctx *ctx);
+
+/* Last error text for ctx (or the last failed create when ctx == NULL). */
+const char *mdft_last_error(const mdft_ctx *ctx);
+
+/* info[0..9] = ntheta, nphi, npsi, no, np, npc, nq (0 before set_ck),
+ *              nx*ny*nz (low 31 bits), precision, mrso */
+int mdft_get_info(const mdft_ctx *ctx, int *info);
+/* Angular quadrature: theta[ntheta] (radians), wtheta[ntheta], w[no]
+ * (module_grid.f90:116-139). */
+int mdft_get_quadrature(const mdft_ctx *ctx, double *theta, double *wtheta, double *w);
+
+/* External potential vext(no,nx,ny,nz) in kJ/mol, already clipped by the
+ * caller (module_vext.f90:638-640).  Replaces solvent(1)%vext. */
+int mdft_set_vext(mdft_ctx *ctx, const double *vext, int flags);
+
+/* Direct correlation function c^{mn}_{mu nu,khi}(|q|) exactly as read by
+ * read_c_luc (src/module_read_c_luc.f90:6-124): np_file index tuples and
+ * nq_file rows of complex values c[iq][i] (interleaved re, im).  The library
+ * keeps the first nq = int(qmax/dq + 0.5) + 2 rows (read_c_luc.f90:101) and
+ * applies the relayout, conjugation and sign folding of
+ * src/module_energy_cproj_mrso.f90:381-429. */
+int mdft_set_ck(mdft_ctx *ctx, int np_file, const int *m, const int *n, const int *mu,
+                const int *nu, const int *khi, int nq_file, double dq, const double *c);
+
+/* energy_and_gradient(f, df): f_parts[0..2] = F_id, F_ext, F_exc (kJ/mol).
+ * df (nullable: F only, as in module_pressure_correction.f90:124) receives
+ * the full gradient with the reference convention df = (1/dv) dF/dxi; it is
+ * overwritten, not accumulated. */
+int mdft_energy_and_gradient(mdft_ctx *ctx, const double *xi, double *f_parts, double *df, int flags);
+
+/* excess_energy_t plug-in (src/module_proc_ptrs.f90:3-21): F_exc only;
+ * df (nullable) is accumulated into, as energy_cproj_mrso does (:739). */
+int mdft_excess_energy(mdft_ctx *ctx, const double *xi, double *fexc, double *df, int flags);
+
+/* energy_ideal_and_external(fid, fext, df) (module_energy_ideal_and_external.f90:14);
+ * df (nullable) is accumulated into. */
+int mdft_ideal_external(mdft_ctx *ctx, const double *xi, double *fid, double *fext, double *df, int flags);
+
+/* Intermediate projection field of the excess term for parity checks:
+ * stage 1 after angl2proj, 2 after the forward FFTs, 3 after rotate/OZ/rotate
+ * back, 4 after the inverse FFTs.  out: complex double [nz][ny][nx][np]. */
+int mdft_excess_stage(mdft_ctx *ctx, const double *xi, int stage, double *out, int flags);
+
+/* HIP-event timings (ms) of the phases of the last evaluation run with
+ * MDFT_TIMERS: [0] angl2proj, [1] forward FFT, [2] OZ, [3] inverse FFT,
+ * [4] proj2angl+ideal+gradient, [5] reductions, [6] total; returns the
+ * number of entries written (<= n). */
+int mdft_phase_times(const mdft_ctx *ctx, double *ms, int n);
+
+/* Device memory owned by the caller, for device-resident fields
+ * (MDFT_DEVICE_PTRS) and the device L-BFGS vectors. */
+int mdft_device_alloc(mdft_ctx *ctx, size_t bytes, void **dptr);
+int mdft_device_free(mdft_ctx *ctx, void *dptr);
+int mdft_memcpy_h2d(mdft_ctx *ctx, void *dst, const void *src, size_t bytes);
+int mdft_memcpy_d2h(mdft_ctx *ctx, void *dst, const void *src, size_t bytes);
+int mdft_synchronize(mdft_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

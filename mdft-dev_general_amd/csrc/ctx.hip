@@ -1,0 +1,696 @@
+// ctx.hip -- context set-up and the C-ABI entry points of libmdft_gpu.so.
+//
+// Host-side tables restate the reference's one-time set-up:
+//   angular quadrature        src/module_grid.f90:95-139, 262-301
+//   k vectors and -q map      src/module_grid.f90:319-331, 384-418
+//   projection index tables   src/module_energy_cproj_mrso.f90:151-175
+//   Wigner-d table            src/module_energy_cproj_mrso.f90:224-235, src/module_wigner_d.f90:6-60
+//   c(k) relayout             src/module_read_c_luc.f90:101, src/module_energy_cproj_mrso.f90:366-429
+// and the evaluation order of energy_and_gradient
+// (src/module_energy_and_gradient.f90:36-240).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "mdft_internal.h"
+
+static std::string g_create_error;
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+            return e_ == hipErrorOutOfMemory ? MDFT_ENOMEM : MDFT_EHIP;                    \
+        }                                                                                  \
+    } while (0)
+
+namespace {
+
+int fail(mdft_ctx *c, int code, const std::string &msg) {
+    c->err = msg;
+    return code;
+}
+
+// module_grid.f90:262-301
+void gauss_legendre(int n, double *x, double *w) {
+    const double pi = acos(-1.0);
+    const int m = (n + 1) / 2;
+    for (int i = 1; i <= m; i++) {
+        double xi = cos(pi * (i - 0.25) / (n + 0.5));
+        double deltaxi = 1.0, pp = 1.0;
+        int guard = 0;
+        while (fabs(deltaxi) > DBL_EPSILON && guard++ < 200) {
+            double p1 = 1.0, p2 = 0.0, p3;
+            for (int j = 1; j <= n; j++) {
+                p3 = p2;
+                p2 = p1;
+                p1 = ((double)(2 * j - 1) * xi * p2 - (double)(j - 1) * p3) / (double)j;
+            }
+            pp = n * (xi * p1 - p2) / (xi * xi - 1.0);
+            deltaxi = -p1 / pp;
+            xi += deltaxi;
+        }
+        x[i - 1] = xi;
+        w[i - 1] = 1.0 / ((1.0 - xi * xi) * pp * pp);
+        x[n - i] = -xi;
+        w[n - i] = w[i - 1];
+    }
+}
+
+double factorial(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; i++) f *= i;
+    return f;
+}
+
+// module_wigner_d.f90:6-60
+double wigner_small_d(int m, int mu, int mup, double theta) {
+    if (std::abs(mu) > m || std::abs(mup) > m) return 0.0;
+    if (m == 0) return 1.0;
+    if (mu == 0 || mup == 0) {
+        int mu0 = mu;
+        double th = theta;
+        if (mu == 0) {
+            mu0 = mup;
+            th = -theta;
+        }
+        double x = 1.0;
+        if (mu0 < 0) {
+            x = ((-mu0) % 2) ? -1.0 : 1.0;
+            mu0 = -mu0;
+        }
+        const double cc = cos(th);
+        double pm1 = 0.0, pm = 1.0, pm2;
+        for (int l = mu0 + 1; l <= m; l++) {
+            pm2 = pm1;
+            pm1 = pm;
+            pm = (cc * (double)(2 * l - 1) * pm1 - (double)(l + mu0 - 1) * pm2) / (double)(l - mu0);
+        }
+        const double sgn = (mu0 % 2) ? -1.0 : 1.0;
+        return x * sgn * sqrt(factorial(m - mu0) / factorial(m + mu0)) * factorial(2 * mu0) /
+               (pow(2.0, mu0) * factorial(mu0)) * pow(sin(th), mu0) * pm;
+    }
+    double s = 0.0;
+    const double cc = cos(0.5 * theta), ss = sin(0.5 * theta);
+    const int itmin = std::max(0, mu - mup), itmax = std::min(m + mu, m - mup);
+    for (int it = itmin; it <= itmax; it++)
+        s += ((it % 2) ? -1.0 : 1.0) /
+             (factorial(m + mu - it) * factorial(m - mup - it) * factorial(it) * factorial(it - mu + mup)) *
+             pow(cc, 2 * m + mu - mup - 2 * it) * pow(ss, 2 * it - mu + mup);
+    return sqrt(factorial(m + mu) * factorial(m - mu) * factorial(m + mup) * factorial(m - mup)) * s;
+}
+
+// module_grid.f90:319-331
+double kproj(int l0, int n, double len) {
+    const double twopi = 2.0 * acos(-1.0);
+    const int l = l0 + 1;
+    if (l <= n / 2 + 1) return (double)(l - 1) * twopi / len;
+    return (double)(l - 1 - n) * twopi / len;
+}
+
+int mq(int i, int n) { return i == 0 ? 0 : n - i; }  // module_grid.f90:384-418
+
+template <typename T>
+hipError_t upload(mdft_ctx *c, T **dst, const T *src, size_t n) {
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) return e;
+    if (n) e = hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice);
+    return e;
+}
+
+void free_pairs(PairSchedule &p) {
+    (void)hipFree(p.pq);
+    (void)hipFree(p.pmq);
+    (void)hipFree(p.pflag);
+    (void)hipFree(p.cstart);
+    (void)hipFree(p.ccount);
+    (void)hipFree(p.ciq);
+    p = PairSchedule{};
+}
+
+// (q, -q) pairs in the order the reference visits them (cproj_mrso.f90:462-476):
+// planes iz = 0..nz/2; in the planes that are their own -q partner the pair is
+// handled from its first member in (iy, ix) order.  Pairs are then grouped by
+// interpolation row iq (stable) and cut into chunks.
+int build_pairs(mdft_ctx *c) {
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    std::vector<int> pq, pmq, pflag, piq;
+    pq.reserve(c->N / 2 + 8);
+    for (int iz = 0; iz <= nz / 2; iz++) {
+        const int izm = mq(iz, nz);
+        for (int iy = 0; iy < ny; iy++) {
+            const int iym = mq(iy, ny);
+            for (int ix = 0; ix < nx; ix++) {
+                const int ixm = mq(ix, nx);
+                if (izm == iz) {
+                    const long self = (long)iy * nx + ix, part = (long)iym * nx + ixm;
+                    if (part < self) continue;
+                }
+                const double q0 = c->kx[ix], q1 = c->ky[iy], q2 = c->kz[iz];
+                const double eff = sqrt(q0 * q0 + q1 * q1 + q2 * q2) / c->dq + 1.0;
+                const int iq = (int)eff;
+                if (iq < 1 || iq + 1 > c->nq) return fail(c, MDFT_ERANGE, "c(k) table too short for |q| of the grid");
+                const int g = (iz * ny + iy) * nx + ix, gm = (izm * ny + iym) * nx + ixm;
+                int flag = (g == gm) ? 1 : 0;
+                if (ix == nx / 2 || iy == ny / 2 || iz == nz / 2) flag |= 2;
+                pq.push_back(g);
+                pmq.push_back(gm);
+                pflag.push_back(flag);
+                piq.push_back(iq);
+            }
+        }
+    }
+    const int npairs = (int)pq.size();
+    std::vector<int> order(npairs);
+    for (int i = 0; i < npairs; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return piq[a] < piq[b]; });
+    std::vector<int> spq(npairs), spmq(npairs), sflag(npairs), cstart, ccount, ciq;
+    const int CH = 256;
+    for (int i = 0; i < npairs; i++) {
+        const int k = order[i];
+        spq[i] = pq[k];
+        spmq[i] = pmq[k];
+        sflag[i] = pflag[k];
+        if (cstart.empty() || ciq.back() != piq[k] || ccount.back() == CH) {
+            cstart.push_back(i);
+            ccount.push_back(0);
+            ciq.push_back(piq[k]);
+        }
+        ccount.back()++;
+    }
+    free_pairs(c->pairs);
+    PairSchedule &p = c->pairs;
+    p.npairs = npairs;
+    p.nchunks = (int)cstart.size();
+    HIPCHK(c, upload(c, &p.pq, spq.data(), spq.size()));
+    HIPCHK(c, upload(c, &p.pmq, spmq.data(), spmq.size()));
+    HIPCHK(c, upload(c, &p.pflag, sflag.data(), sflag.size()));
+    HIPCHK(c, upload(c, &p.cstart, cstart.data(), cstart.size()));
+    HIPCHK(c, upload(c, &p.ccount, ccount.data(), ccount.size()));
+    HIPCHK(c, upload(c, &p.ciq, ciq.data(), ciq.size()));
+    return MDFT_OK;
+}
+
+int ensure_events(mdft_ctx *c) {
+    if (c->timers_ready) return MDFT_OK;
+    for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
+    c->timers_ready = true;
+    return MDFT_OK;
+}
+
+int ready(mdft_ctx *c) {
+    if (!c) return MDFT_EINVAL;
+    if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set (call mdft_set_vext first)");
+    if (!c->have_ck) return fail(c, MDFT_ESTATE, "c(k) not set (call mdft_set_ck first)");
+    return MDFT_OK;
+}
+
+hipError_t ensure_stage(mdft_ctx *c, double **buf) {
+    if (*buf) return hipSuccess;
+    return hipMalloc((void **)buf, sizeof(double) * c->N * c->no);
+}
+
+// Runs the excess-term projection pipeline up to `stage` (1..4) on device xi.
+int run_excess_pipeline(mdft_ctx *c, const double *xi, int stage, bool timers) {
+    if (timers) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(c, launch_angl2proj_f64(c, xi, c->d_proj));
+    if (timers) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    if (stage >= 2) HIPCHK(c, launch_fft3d_f64(c, c->d_proj, +1));
+    if (timers) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    if (stage >= 3) HIPCHK(c, launch_oz_f64(c, c->d_proj));
+    if (timers) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    if (stage >= 4) HIPCHK(c, launch_fft3d_f64(c, c->d_proj, -1));
+    if (timers) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    return MDFT_OK;
+}
+
+int read_sums(mdft_ctx *c, int nblocks, double *out3) {
+    HIPCHK(c, launch_reduce3(c, nblocks));
+    HIPCHK(c, hipMemcpyAsync(c->h_sums, c->d_sums, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out3[0] = c->h_sums[0];
+    out3[1] = c->h_sums[1];
+    out3[2] = c->h_sums[2];
+    return MDFT_OK;
+}
+
+int finish_timers(mdft_ctx *c) {
+    float t;
+    const int pairs[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
+    for (int i = 0; i < 6; i++) {
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev[pairs[i][0]], c->ev[pairs[i][1]]));
+        c->phase_ms[i] = t;
+    }
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[6]));
+    c->phase_ms[6] = t;
+    c->nphase = 7;
+    return MDFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
+    if (!desc || !out) {
+        g_create_error = "null argument";
+        return MDFT_EINVAL;
+    }
+    *out = nullptr;
+    const mdft_desc &d = *desc;
+    if (d.nx < 1 || d.ny < 1 || d.nz < 1 || !(d.lx > 0) || !(d.ly > 0) || !(d.lz > 0) || d.mmax < 0 ||
+        d.mmax > MDFT_MAX_MMAX || d.mrso < 1 || !(d.kT > 0) || !(d.n0 > 0)) {
+        g_create_error = "invalid grid/solvent description";
+        return MDFT_EINVAL;
+    }
+    if (d.precision != MDFT_PREC_FP64) {
+        g_create_error = "only MDFT_PREC_FP64 is available in this build";
+        return MDFT_EINVAL;
+    }
+    if ((long long)d.nx * d.ny * d.nz >= (1LL << 31)) {
+        g_create_error = "grid too large (nx*ny*nz must fit in 31 bits)";
+        return MDFT_EINVAL;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device";
+        return MDFT_ENODEV;
+    }
+    if (d.device < 0 || d.device >= ndev) {
+        g_create_error = "device ordinal out of range";
+        return MDFT_EINVAL;
+    }
+    if (hipSetDevice(d.device) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return MDFT_EHIP;
+    }
+    mdft_ctx *c = new mdft_ctx();
+    c->d = d;
+    c->L = d.mmax;
+    c->H = d.mmax / d.mrso;
+    c->ntheta = d.mmax + 1;
+    c->nphi = 2 * d.mmax + 1;
+    c->npsi = 2 * (d.mmax / d.mrso) + 1;
+    c->no = c->ntheta * c->nphi * c->npsi;
+    c->nmu2 = c->H + 1;
+    c->N = (size_t)d.nx * d.ny * d.nz;
+    c->dv = (d.lx / d.nx) * (d.ly / d.ny) * (d.lz / d.nz);
+    const double pi = acos(-1.0);
+    c->rho0 = d.n0 / (8.0 * pi * pi / d.mrso);
+    int np = 0;
+    for (int m = 0; m <= d.mmax; m++) np += (2 * m + 1) * (m / d.mrso + 1);
+    c->np = np;
+    // quadrature
+    c->theta.resize(c->ntheta);
+    c->wtheta.resize(c->ntheta);
+    gauss_legendre(c->ntheta, c->theta.data(), c->wtheta.data());
+    for (auto &t : c->theta) t = acos(t);
+    c->w.resize(c->no);
+    const double eightpisq = 8.0 * pi * pi;
+    for (int it = 0, io = 0; it < c->ntheta; it++)
+        for (int iphi = 0; iphi < c->nphi; iphi++)
+            for (int ipsi = 0; ipsi < c->npsi; ipsi++, io++)
+                c->w[io] = c->wtheta[it] * (1.0 / c->nphi) * (1.0 / (double)(c->npsi * d.mrso)) * eightpisq;
+    // projection tables
+    const int W = 2 * c->L + 1;
+    c->pidx.assign((size_t)(c->L + 1) * W * c->nmu2, -1);
+    for (int m = 0, ip = 0; m <= c->L; m++)
+        for (int mup = -m; mup <= m; mup++)
+            for (int mu = 0; mu <= m; mu += d.mrso, ip++) {
+                c->pidx[(m * W + mup + c->L) * c->nmu2 + mu / d.mrso] = ip;
+                c->pm.push_back(m);
+                c->pmup.push_back(mup);
+                c->pmu.push_back(mu);
+            }
+    c->wig.resize((size_t)np * c->ntheta);
+    std::vector<double> a2p(np * c->ntheta), p2a(np * c->ntheta), fmv(np);
+    for (int ip = 0; ip < np; ip++) {
+        const int m = c->pm[ip];
+        const double fm = (m == 7) ? sqrt(14.0) : sqrt((double)(2 * m + 1));
+        fmv[ip] = fm;
+        for (int it = 0; it < c->ntheta; it++) {
+            const double dd = wigner_small_d(m, c->pmup[ip], c->pmu[ip], c->theta[it]);
+            c->wig[ip * c->ntheta + it] = dd;
+            a2p[ip * c->ntheta + it] = dd * c->wtheta[it];
+            p2a[ip * c->ntheta + it] = dd * fm;
+        }
+    }
+    std::vector<double> cph(c->nphi), sph(c->nphi), cps(c->npsi), sps(c->npsi);
+    for (int k = 0; k < c->nphi; k++) {
+        cph[k] = cos(2.0 * pi * k / c->nphi);
+        sph[k] = sin(2.0 * pi * k / c->nphi);
+    }
+    for (int k = 0; k < c->npsi; k++) {
+        cps[k] = cos(2.0 * pi * k / c->npsi);
+        sps[k] = sin(2.0 * pi * k / c->npsi);
+    }
+    // OZ coefficient offsets per projection row (cproj_mrso.f90:381 ordering)
+    std::vector<int> ia0(np);
+    for (int ip = 0, ia = 0; ip < np; ip++) {
+        ia0[ip] = ia;
+        for (int n = std::abs(c->pmup[ip]); n <= c->L; n++) ia += 2 * (n / d.mrso) + 1;
+    }
+    c->kx.resize(d.nx);
+    c->ky.resize(d.ny);
+    c->kz.resize(d.nz);
+    for (int l = 0; l < d.nx; l++) c->kx[l] = kproj(l, d.nx, d.lx);
+    for (int l = 0; l < d.ny; l++) c->ky[l] = kproj(l, d.ny, d.ly);
+    for (int l = 0; l < d.nz; l++) c->kz[l] = kproj(l, d.nz, d.lz);
+
+    auto bad = [&](int code, const std::string &msg) {
+        g_create_error = msg;
+        mdft_ctx_destroy(c);
+        return code;
+    };
+    const int ns[3] = {d.nx, d.ny, d.nz};
+    for (int ax = 0; ax < 3; ax++) {
+        FftAxisPlan &p = c->fft[ax];
+        p.n = ns[ax];
+        p.nst = fft_make_plan(ns[ax], p.radix);
+        if (p.nst < 0) return bad(MDFT_EINVAL, "grid sizes must factor into 2, 3 and 5");
+        std::vector<double2> tw(ns[ax]);
+        for (int k = 0; k < ns[ax]; k++) tw[k] = make_double2(cos(2.0 * pi * k / ns[ax]), sin(2.0 * pi * k / ns[ax]));
+        if (upload(c, &p.tw, tw.data(), tw.size()) != hipSuccess) return bad(MDFT_ENOMEM, "device allocation failed");
+    }
+    {
+        int lim = 0;
+        if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, d.device) == hipSuccess && lim > 0)
+            c->lds_limit = (size_t)lim;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(MDFT_EHIP, "hipStreamCreate failed");
+    bool ok = upload(c, &c->d_w, c->w.data(), c->w.size()) == hipSuccess &&
+              upload(c, &c->d_a2p, a2p.data(), a2p.size()) == hipSuccess &&
+              upload(c, &c->d_p2a, p2a.data(), p2a.size()) == hipSuccess &&
+              upload(c, &c->d_fm, fmv.data(), fmv.size()) == hipSuccess &&
+              upload(c, &c->d_cph, cph.data(), cph.size()) == hipSuccess &&
+              upload(c, &c->d_sph, sph.data(), sph.size()) == hipSuccess &&
+              upload(c, &c->d_cps, cps.data(), cps.size()) == hipSuccess &&
+              upload(c, &c->d_sps, sps.data(), sps.size()) == hipSuccess &&
+              upload(c, &c->d_pidx, c->pidx.data(), c->pidx.size()) == hipSuccess &&
+              upload(c, &c->d_pm, c->pm.data(), c->pm.size()) == hipSuccess &&
+              upload(c, &c->d_pmup, c->pmup.data(), c->pmup.size()) == hipSuccess &&
+              upload(c, &c->d_pmu, c->pmu.data(), c->pmu.size()) == hipSuccess &&
+              upload(c, &c->d_ia0, ia0.data(), ia0.size()) == hipSuccess &&
+              upload(c, &c->d_kx, c->kx.data(), c->kx.size()) == hipSuccess &&
+              upload(c, &c->d_ky, c->ky.data(), c->ky.size()) == hipSuccess &&
+              upload(c, &c->d_kz, c->kz.data(), c->kz.size()) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_proj, sizeof(double2) * c->N * np) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_vext, sizeof(double) * c->N * c->no) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_sums, sizeof(double) * 8) == hipSuccess;
+    ok = ok && hipHostMalloc(&c->h_sums, sizeof(double) * 8, hipHostMallocDefault) == hipSuccess;
+    if (!ok) return bad(MDFT_ENOMEM, "device allocation failed");
+    AngTables &t = c->ang;
+    t.ntheta = c->ntheta;
+    t.nphi = c->nphi;
+    t.npsi = c->npsi;
+    t.no = c->no;
+    t.np = np;
+    t.L = c->L;
+    t.H = c->H;
+    t.mrso = d.mrso;
+    t.w = c->d_w;
+    t.a2p = c->d_a2p;
+    t.p2a = c->d_p2a;
+    t.fm = c->d_fm;
+    t.cph = c->d_cph;
+    t.sph = c->d_sph;
+    t.cps = c->d_cps;
+    t.sps = c->d_sps;
+    t.pidx = c->d_pidx;
+    t.pm = c->d_pm;
+    t.pmup = c->d_pmup;
+    t.pmu = c->d_pmu;
+    *out = c;
+    return MDFT_OK;
+}
+
+void mdft_ctx_destroy(mdft_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &f : c->fft) (void)hipFree(f.tw);
+    void *bufs[] = {c->d_w,   c->d_a2p,  c->d_p2a,  c->d_fm,   c->d_cph,      c->d_sph,        c->d_cps,
+                    c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
+                    c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
+                    c->d_partials, c->d_sums};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    free_pairs(c->pairs);
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
+    if (c->timers_ready)
+        for (auto &e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *mdft_last_error(const mdft_ctx *c) {
+    if (!c) return g_create_error.c_str();
+    return c->err.c_str();
+}
+
+int mdft_get_info(const mdft_ctx *c, int *info) {
+    if (!c || !info) return MDFT_EINVAL;
+    info[0] = c->ntheta;
+    info[1] = c->nphi;
+    info[2] = c->npsi;
+    info[3] = c->no;
+    info[4] = c->np;
+    info[5] = c->npc;
+    info[6] = c->nq;
+    info[7] = (int)(c->N & 0x7fffffff);
+    info[8] = c->d.precision;
+    info[9] = c->d.mrso;
+    return MDFT_OK;
+}
+
+int mdft_get_quadrature(const mdft_ctx *c, double *theta, double *wtheta, double *w) {
+    if (!c) return MDFT_EINVAL;
+    if (theta) memcpy(theta, c->theta.data(), sizeof(double) * c->ntheta);
+    if (wtheta) memcpy(wtheta, c->wtheta.data(), sizeof(double) * c->ntheta);
+    if (w) memcpy(w, c->w.data(), sizeof(double) * c->no);
+    return MDFT_OK;
+}
+
+int mdft_set_vext(mdft_ctx *c, const double *vext, int flags) {
+    if (!c || !vext) return MDFT_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(c->d_vext, vext, sizeof(double) * c->N * c->no,
+                             (flags & MDFT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_vext = true;
+    return MDFT_OK;
+}
+
+int mdft_set_ck(mdft_ctx *c, int np_file, const int *m, const int *n, const int *mu, const int *nu, const int *khi,
+                int nq_file, double dq, const double *cf) {
+    if (!c || np_file < 1 || !m || !n || !mu || !nu || !khi || nq_file < 2 || !(dq > 0) || !cf)
+        return MDFT_EINVAL;
+    const int L = c->L, mrso = c->d.mrso;
+    // q_max of cproj_mrso.f90:366 and nq of read_c_luc.f90:101
+    double kxm = c->kx[0], kym = c->ky[0], kzm = c->kz[0];
+    for (double v : c->kx) kxm = std::max(kxm, v);
+    for (double v : c->ky) kym = std::max(kym, v);
+    for (int i = 0; i <= c->d.nz / 2; i++) kzm = std::max(kzm, c->kz[i]);
+    const double qmax = sqrt(kxm * kxm + kym * kym + kzm * kzm);
+    const int nq = (int)(qmax / dq + 0.5) + 2;
+    if (nq > nq_file) return fail(c, MDFT_ERANGE, "c(k) file has fewer |q| rows than the grid needs");
+    std::map<std::tuple<int, int, int, int, int>, int> pos;
+    for (int i = 0; i < np_file; i++) {
+        if (mu[i] % mrso || nu[i] % mrso) continue;
+        pos[std::make_tuple(m[i], n[i], mu[i] / mrso, nu[i] / mrso, khi[i])] = i;
+    }
+    std::vector<int> src;
+    std::vector<double> sgn;
+    for (int mm = 0; mm <= L; mm++)
+        for (int k = -mm; k <= mm; k++)
+            for (int mu2 = 0; mu2 <= mm / mrso; mu2++)
+                for (int nn = std::abs(k); nn <= L; nn++)
+                    for (int nu2 = -nn / mrso; nu2 <= nn / mrso; nu2++) {
+                        auto it = pos.find(std::make_tuple(mm, nn, mu2, nu2, k));
+                        if (it == pos.end()) {
+                            char buf[160];
+                            snprintf(buf, sizeof buf, "c(k) table lacks (m,n,mu,nu,khi)=(%d,%d,%d,%d,%d)", mm, nn,
+                                     mu2 * mrso, nu2 * mrso, k);
+                            return fail(c, MDFT_EINVAL, buf);
+                        }
+                        src.push_back(it->second);
+                        const int nuv = nu2 * mrso;
+                        sgn.push_back(nuv < 0 ? ((std::abs(k + nuv) % 2) ? -1.0 : 1.0) : ((nn % 2) ? -1.0 : 1.0));
+                    }
+    const int npc = (int)src.size();
+    std::vector<double2> tab((size_t)npc * nq);
+    for (int iq = 0; iq < nq; iq++)
+        for (int ia = 0; ia < npc; ia++) {
+            const double *v = cf + 2 * ((size_t)iq * np_file + src[ia]);
+            // conj (cproj_mrso.f90:415) then the sign folding (:420-429)
+            tab[(size_t)iq * npc + ia] = make_double2(sgn[ia] * v[0], sgn[ia] * -v[1]);
+        }
+    if (c->d_ck) (void)hipFree(c->d_ck);
+    c->d_ck = nullptr;
+    HIPCHK(c, upload(c, &c->d_ck, tab.data(), tab.size()));
+    c->npc = npc;
+    c->nq = nq;
+    c->dq = dq;
+    const int rc = build_pairs(c);
+    if (rc != MDFT_OK) return rc;
+    c->have_ck = true;
+    return MDFT_OK;
+}
+
+int mdft_excess_stage(mdft_ctx *c, const double *xi, int stage, double *out, int flags) {
+    int rc = ready(c);
+    if (rc) return rc;
+    if (stage < 1 || stage > 4 || !xi || !out) return fail(c, MDFT_EINVAL, "bad stage arguments");
+    const bool dev = flags & MDFT_DEVICE_PTRS;
+    const double *xd = xi;
+    if (!dev) {
+        HIPCHK(c, ensure_stage(c, &c->d_xi_stage));
+        HIPCHK(c, hipMemcpyAsync(c->d_xi_stage, xi, sizeof(double) * c->N * c->no, hipMemcpyHostToDevice, c->stream));
+        xd = c->d_xi_stage;
+    }
+    rc = run_excess_pipeline(c, xd, stage, false);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_proj, sizeof(double2) * c->N * c->np,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+// mode 0: full energy_and_gradient, 1: excess only
+static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mode, double *sums) {
+    int rc = ready(c);
+    if (rc) return rc;
+    if (!xi) return fail(c, MDFT_EINVAL, "xi is null");
+    const bool dev = flags & MDFT_DEVICE_PTRS;
+    const bool timers = flags & MDFT_TIMERS;
+    if (timers && (rc = ensure_events(c))) return rc;
+    const size_t nbytes = sizeof(double) * c->N * c->no;
+    const double *xd = xi;
+    double *dd = df;
+    if (!dev) {
+        HIPCHK(c, ensure_stage(c, &c->d_xi_stage));
+        HIPCHK(c, hipMemcpyAsync(c->d_xi_stage, xi, nbytes, hipMemcpyHostToDevice, c->stream));
+        xd = c->d_xi_stage;
+        if (df) {
+            HIPCHK(c, ensure_stage(c, &c->d_df_stage));
+            if (mode == 1)
+                HIPCHK(c, hipMemcpyAsync(c->d_df_stage, df, nbytes, hipMemcpyHostToDevice, c->stream));
+            dd = c->d_df_stage;
+        }
+    }
+    rc = run_excess_pipeline(c, xd, 4, timers);
+    if (rc) return rc;
+    int nb = 0;
+    HIPCHK(c, launch_final_f64(c, c->d_proj, xd, dd, mode, &nb));
+    if (timers) HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    rc = read_sums(c, nb, sums);
+    if (rc) return rc;
+    if (timers) {
+        HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[6]));
+        if ((rc = finish_timers(c))) return rc;
+    }
+    if (!dev && df) {
+        HIPCHK(c, hipMemcpyAsync(df, dd, nbytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    sums[2] *= 0.5 * c->dv;  // ff = ff * 0.5 * dv (cproj_mrso.f90:745)
+    return MDFT_OK;
+}
+
+int mdft_energy_and_gradient(mdft_ctx *c, const double *xi, double *f_parts, double *df, int flags) {
+    if (!f_parts) return c ? fail(c, MDFT_EINVAL, "f_parts is null") : MDFT_EINVAL;
+    double s[3];
+    const int rc = evaluate(c, xi, df, flags, 0, s);
+    if (rc) return rc;
+    f_parts[0] = s[0];
+    f_parts[1] = s[1];
+    f_parts[2] = s[2];
+    return MDFT_OK;
+}
+
+int mdft_excess_energy(mdft_ctx *c, const double *xi, double *fexc, double *df, int flags) {
+    if (!fexc) return c ? fail(c, MDFT_EINVAL, "fexc is null") : MDFT_EINVAL;
+    double s[3];
+    const int rc = evaluate(c, xi, df, flags, 1, s);
+    if (rc) return rc;
+    *fexc = s[2];
+    return MDFT_OK;
+}
+
+int mdft_ideal_external(mdft_ctx *c, const double *xi, double *fid, double *fext, double *df, int flags) {
+    if (!c) return MDFT_EINVAL;
+    if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set (call mdft_set_vext first)");
+    if (!xi || !fid || !fext) return fail(c, MDFT_EINVAL, "null argument");
+    const bool dev = flags & MDFT_DEVICE_PTRS;
+    const size_t nbytes = sizeof(double) * c->N * c->no;
+    const double *xd = xi;
+    double *dd = df;
+    if (!dev) {
+        HIPCHK(c, ensure_stage(c, &c->d_xi_stage));
+        HIPCHK(c, hipMemcpyAsync(c->d_xi_stage, xi, nbytes, hipMemcpyHostToDevice, c->stream));
+        xd = c->d_xi_stage;
+        if (df) {
+            HIPCHK(c, ensure_stage(c, &c->d_df_stage));
+            HIPCHK(c, hipMemcpyAsync(c->d_df_stage, df, nbytes, hipMemcpyHostToDevice, c->stream));
+            dd = c->d_df_stage;
+        }
+    }
+    int nb = 0;
+    HIPCHK(c, launch_ideal_f64(c, xd, dd, &nb));
+    double s[3];
+    const int rc = read_sums(c, nb, s);
+    if (rc) return rc;
+    if (!dev && df) {
+        HIPCHK(c, hipMemcpyAsync(df, dd, nbytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    *fid = s[0];
+    *fext = s[1];
+    return MDFT_OK;
+}
+
+int mdft_phase_times(const mdft_ctx *c, double *ms, int n) {
+    if (!c || !ms) return MDFT_EINVAL;
+    const int k = std::min(n, c->nphase);
+    for (int i = 0; i < k; i++) ms[i] = c->phase_ms[i];
+    return k;
+}
+
+int mdft_device_alloc(mdft_ctx *c, size_t bytes, void **dptr) {
+    if (!c || !dptr) return MDFT_EINVAL;
+    HIPCHK(c, hipMalloc(dptr, bytes ? bytes : 1));
+    return MDFT_OK;
+}
+
+int mdft_device_free(mdft_ctx *c, void *dptr) {
+    if (!c) return MDFT_EINVAL;
+    if (dptr) HIPCHK(c, hipFree(dptr));
+    return MDFT_OK;
+}
+
+int mdft_memcpy_h2d(mdft_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c) return MDFT_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+int mdft_memcpy_d2h(mdft_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c) return MDFT_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+int mdft_synchronize(mdft_ctx *c) {
+    if (!c) return MDFT_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+}  // extern "C"

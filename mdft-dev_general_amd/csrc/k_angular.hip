@@ -1,0 +1,347 @@
+// k_angular.hip -- orientation <-> projection transforms and the fused
+// gradient/free-energy pass, FP64.
+//
+// angl2proj restates src/module_orientation_projection_transform.f90:159-199
+// (a 2D DFT over (psi, phi) for each theta node, then a Gauss-Legendre
+// contraction in theta with Wigner d and sqrt(2m+1)), driven as in
+// src/module_energy_cproj_mrso.f90:276-290 with o = rho0 (xi^2 - 1).
+// The final pass fuses proj2angl (:202-243), the F_exc reduction and dF update
+// of src/module_energy_cproj_mrso.f90:728-762 with the ideal+external term of
+// src/module_energy_ideal_and_external.f90:48-72, so xi is read once, vext once
+// and dF written once per evaluation.
+//
+// Layout: points r are processed in tiles of PT consecutive points per
+// workgroup; xi/vext/df tiles [PT][no] and projection tiles [PT][np] are
+// contiguous in HBM, so every global access is a coalesced stream.
+#include <cfloat>
+
+#include "mdft_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int PT>
+__global__ __launch_bounds__(kThreads) void angl2proj_f64(const double *__restrict__ xi, double2 *__restrict__ proj,
+                                                         size_t N, AngTables t, double rho0) {
+    extern __shared__ double smem[];
+    const int no = t.no, np = t.np, nth = t.ntheta, nphi = t.nphi, npsi = t.npsi, H1 = t.H + 1,
+              W = 2 * t.L + 1;
+    double *o = smem;                                           // [PT][no]
+    double2 *G = reinterpret_cast<double2 *>(o + PT * no);      // [PT][nth][nphi][H1]
+    double2 *F = G + PT * nth * nphi * H1;                      // [PT][nth][W][H1]
+    const size_t r0 = (size_t)blockIdx.x * PT;
+    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+    const int tid = threadIdx.x;
+    const double *src = xi + r0 * no;
+    for (int i = tid; i < npts * no; i += kThreads) {
+        double x = src[i];
+        o[i] = rho0 * (x * x - 1.0);
+    }
+    __syncthreads();
+    // DFT over psi: G = sum_ipsi o exp(+i mu2 psi)
+    const int nG = npts * nth * nphi * H1;
+    for (int i = tid; i < nG; i += kThreads) {
+        const int mu2 = i % H1;
+        const double *line = o + (i / H1) * npsi;
+        double re = 0.0, im = 0.0;
+        for (int ipsi = 0; ipsi < npsi; ipsi++) {
+            const int k = (mu2 * ipsi) % npsi;
+            const double v = line[ipsi];
+            re += v * t.cps[k];
+            im += v * t.sps[k];
+        }
+        G[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    // DFT over phi: F = sum_iphi G exp(+i mup phi) / (nphi npsi)
+    const double norm = 1.0 / (double)(nphi * npsi);
+    const int nF = npts * nth * W * H1;
+    for (int i = tid; i < nF; i += kThreads) {
+        const int mu2 = i % H1;
+        int rest = i / H1;
+        const int mup = rest % W - t.L;
+        rest /= W;  // pt*nth + it
+        const double2 *g = G + (size_t)rest * nphi * H1 + mu2;
+        double re = 0.0, im = 0.0;
+        for (int iphi = 0; iphi < nphi; iphi++) {
+            int k = (mup * iphi) % nphi;
+            k += (k < 0) ? nphi : 0;
+            const double2 v = g[iphi * H1];
+            re += v.x * t.cph[k] - v.y * t.sph[k];
+            im += v.x * t.sph[k] + v.y * t.cph[k];
+        }
+        F[i] = make_double2(re * norm, im * norm);
+    }
+    __syncthreads();
+    // Gauss-Legendre contraction in theta with Wigner d
+    double2 *dst = proj + r0 * np;
+    for (int i = tid; i < npts * np; i += kThreads) {
+        const int pt = i / np, ip = i - pt * np;
+        const int mup = t.pmup[ip], mu2 = t.pmu[ip] / t.mrso;
+        const double2 *f = F + ((size_t)pt * nth * W + (mup + t.L)) * H1 + mu2;
+        const double *a = t.a2p + ip * nth;
+        double re = 0.0, im = 0.0;
+        for (int it = 0; it < nth; it++) {
+            const double2 v = f[(size_t)it * W * H1];
+            re += v.x * a[it];
+            im += v.y * a[it];
+        }
+        dst[i] = make_double2(re * t.fm[ip], im * t.fm[ip]);
+    }
+}
+
+__device__ inline void block_reduce3(double &a, double &b, double &c, double *partials) {
+    __shared__ double red[3][kThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wv] = a;
+        red[1][wv] = b;
+        red[2][wv] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int k = 0; k < kThreads / 64; k++) {
+            s0 += red[0][k];
+            s1 += red[1][k];
+            s2 += red[2][k];
+        }
+        partials[blockIdx.x * 3 + 0] = s0;
+        partials[blockIdx.x * 3 + 1] = s1;
+        partials[blockIdx.x * 3 + 2] = s2;
+    }
+}
+
+struct FinalConsts {
+    double rho0, kT, dv, prefactor;
+};
+
+// mode 0: df = ideal+ext gradient + excess gradient (energy_and_gradient)
+// mode 1: df += excess gradient (excess_energy plug-in)
+template <int PT>
+__global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict__ proj, const double *__restrict__ xi,
+                                                     const double *__restrict__ vext, double *__restrict__ df,
+                                                     double *__restrict__ partials, size_t N, AngTables t,
+                                                     FinalConsts k, int mode) {
+    extern __shared__ double smem[];
+    const int no = t.no, np = t.np, nth = t.ntheta, nphi = t.nphi, npsi = t.npsi, H1 = t.H + 1,
+              W = 2 * t.L + 1, L = t.L;
+    double2 *Gm = reinterpret_cast<double2 *>(smem);   // [PT][np]
+    double2 *f = Gm + PT * np;                         // [PT][nth][W][H1]
+    double2 *h = f + PT * nth * W * H1;                // [PT][nth][nphi][H1]
+    const size_t r0 = (size_t)blockIdx.x * PT;
+    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+    const int tid = threadIdx.x;
+    const double2 *src = proj + r0 * np;
+    for (int i = tid; i < npts * np; i += kThreads) Gm[i] = src[i];
+    __syncthreads();
+    // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m)   (:215-224)
+    const int nf = npts * nth * W * H1;
+    for (int i = tid; i < nf; i += kThreads) {
+        const int mu2 = i % H1;
+        int rest = i / H1;
+        const int mup = rest % W - L;
+        rest /= W;
+        const int it = rest % nth, pt = rest / nth;
+        const int a = abs(mup), b = t.mrso * mu2;
+        const int mlo = a > b ? a : b;
+        double re = 0.0, im = 0.0;
+        for (int m = mlo; m <= L; m++) {
+            const int ip = t.pidx[(m * W + mup + L) * H1 + mu2];
+            const double2 v = Gm[pt * np + ip];
+            const double c = t.p2a[ip * nth + it];
+            re += v.x * c;
+            im += v.y * c;
+        }
+        f[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    // h(theta, phi, mu2) = sum_mup conj(f) exp(+i mup phi)
+    const int nh = npts * nth * nphi * H1;
+    for (int i = tid; i < nh; i += kThreads) {
+        const int mu2 = i % H1;
+        int rest = i / H1;
+        const int iphi = rest % nphi;
+        rest /= nphi;  // pt*nth + it
+        const double2 *fr = f + (size_t)rest * W * H1 + mu2;
+        double re = 0.0, im = 0.0;
+        for (int mup = -L; mup <= L; mup++) {
+            int kk = (mup * iphi) % nphi;
+            kk += (kk < 0) ? nphi : 0;
+            const double2 v = fr[(mup + L) * H1];
+            // conj(v) * (c + i s)
+            re += v.x * t.cph[kk] + v.y * t.sph[kk];
+            im += v.x * t.sph[kk] - v.y * t.cph[kk];
+        }
+        h[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    double fid = 0.0, fext = 0.0, fexc = 0.0;
+    const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
+    for (int i = tid; i < npts * no; i += kThreads) {
+        const int pt = i / no, io = i - pt * no;
+        const int ipsi = io % npsi, iphi = (io / npsi) % nphi, it = io / (npsi * nphi);
+        const double2 *hr = h + ((size_t)(pt * nth + it) * nphi + iphi) * H1;
+        // c2r: real part of the Hermitian-weighted sum over mu2 (weight 2 for mu2 > 0)
+        double v = hr[0].x;
+        for (int mu2 = 1; mu2 < H1; mu2++) {
+            const int kk = (mu2 * ipsi) % npsi;
+            v += 2.0 * (hr[mu2].x * t.cps[kk] - hr[mu2].y * t.sps[kk]);
+        }
+        const double wio = t.w[io];
+        const double vexc = k.prefactor * wio * v;
+        const size_t g = (r0 + pt) * (size_t)no + io;
+        const double x = xi[g];
+        const double rho = x * x * rho0;
+        fexc += (x * x * rho0 - rho0) * vexc;
+        if (mode == 0) {
+            const double ve = vext[g];
+            double gid = 0.0;
+            if (rho > DBL_EPSILON) {
+                const double lg = log(rho / rho0);
+                fid += kT * dv * wio * (rho * lg - rho + rho0);
+                fext += dv * wio * rho * ve;
+                gid = wio * 2.0 * rho0 * x * (kT * lg + ve);
+            } else {
+                fid += kT * dv * wio * rho0;
+            }
+            if (df) df[g] = gid + 2.0 * rho0 * x * vexc;
+        } else {
+            if (df) df[g] += 2.0 * rho0 * x * vexc;
+        }
+    }
+    block_reduce3(fid, fext, fexc, partials);
+}
+
+// Ideal + external term alone (module_energy_ideal_and_external.f90:48-72); df accumulated.
+__global__ __launch_bounds__(kThreads) void ideal_f64(const double *__restrict__ xi, const double *__restrict__ vext,
+                                                     double *__restrict__ df, double *__restrict__ partials,
+                                                     size_t total, int no, const double *__restrict__ w,
+                                                     FinalConsts k) {
+    double fid = 0.0, fext = 0.0, zero = 0.0;
+    const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
+    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads) {
+        const int io = (int)(g % (size_t)no);
+        const double x = xi[g], wio = w[io];
+        const double rho = x * x * rho0;
+        if (rho > DBL_EPSILON) {
+            const double lg = log(rho / rho0);
+            const double ve = vext[g];
+            fid += kT * dv * wio * (rho * lg - rho + rho0);
+            fext += dv * wio * rho * ve;
+            if (df) df[g] += wio * 2.0 * rho0 * x * (kT * lg + ve);
+        } else {
+            fid += kT * dv * wio * rho0;
+        }
+    }
+    block_reduce3(fid, fext, zero, partials);
+}
+
+__global__ __launch_bounds__(kThreads) void reduce3(const double *__restrict__ partials, int nblocks,
+                                                   double *__restrict__ out) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += kThreads) {
+        a += partials[3 * i + 0];
+        b += partials[3 * i + 1];
+        c += partials[3 * i + 2];
+    }
+    __shared__ double red[3][kThreads];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+            red[2][threadIdx.x] += red[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+        out[2] = red[2][0];
+    }
+}
+
+constexpr int kPT = 4;  // points per workgroup tile (keeps the LDS tile under 64 KB at mmax = 5)
+
+size_t a2p_lds(const AngTables &t) {
+    return sizeof(double) * kPT * t.no + sizeof(double2) * kPT * t.ntheta * t.nphi * (t.H + 1) +
+           sizeof(double2) * kPT * t.ntheta * (2 * t.L + 1) * (t.H + 1);
+}
+size_t final_lds(const AngTables &t) {
+    return sizeof(double2) * kPT * t.np + sizeof(double2) * kPT * t.ntheta * (2 * t.L + 1) * (t.H + 1) +
+           sizeof(double2) * kPT * t.ntheta * t.nphi * (t.H + 1);
+}
+
+hipError_t ensure_partials(mdft_ctx *c, size_t nblocks) {
+    if (c->partials_cap >= nblocks * 3) return hipSuccess;
+    if (c->d_partials) (void)hipFree(c->d_partials);
+    c->d_partials = nullptr;
+    hipError_t e = hipMalloc(&c->d_partials, sizeof(double) * nblocks * 3);
+    if (e == hipSuccess) c->partials_cap = nblocks * 3;
+    return e;
+}
+
+}  // namespace
+
+hipError_t launch_angl2proj_f64(mdft_ctx *c, const double *xi, double2 *proj) {
+    const size_t nb = (c->N + kPT - 1) / kPT;
+    if (a2p_lds(c->ang) > c->lds_limit) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(angl2proj_f64<kPT>, dim3((unsigned)nb), dim3(kThreads), a2p_lds(c->ang), c->stream, xi, proj,
+                       c->N, c->ang, c->rho0);
+    return hipGetLastError();
+}
+
+hipError_t launch_final_f64(mdft_ctx *c, const double2 *proj, const double *xi, double *df, int mode,
+                            int *nblocks_out) {
+    const size_t nb = (c->N + kPT - 1) / kPT;
+    if (final_lds(c->ang) > c->lds_limit) return hipErrorInvalidConfiguration;
+    hipError_t e = ensure_partials(c, nb);
+    if (e != hipSuccess) return e;
+    FinalConsts k;
+    k.rho0 = c->rho0;
+    k.kT = c->d.kT;
+    k.dv = c->dv;
+    const double pi = acos(-1.0);
+    const double fourpisq = 4.0 * pi * pi;
+    k.prefactor = -c->d.kT * 2.0 * fourpisq / c->d.mrso / c->d.n0;  // cproj_mrso.f90:728
+    hipLaunchKernelGGL(final_f64<kPT>, dim3((unsigned)nb), dim3(kThreads), final_lds(c->ang), c->stream, proj, xi,
+                       c->d_vext, df, c->d_partials, c->N, c->ang, k, mode);
+    *nblocks_out = (int)nb;
+    return hipGetLastError();
+}
+
+hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblocks_out) {
+    const size_t total = c->N * (size_t)c->no;
+    size_t nb = (total + kThreads - 1) / kThreads;
+    if (nb > 65536) nb = 65536;
+    hipError_t e = ensure_partials(c, nb);
+    if (e != hipSuccess) return e;
+    FinalConsts k;
+    k.rho0 = c->rho0;
+    k.kT = c->d.kT;
+    k.dv = c->dv;
+    k.prefactor = 0.0;
+    hipLaunchKernelGGL(ideal_f64, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, c->d_vext, df, c->d_partials,
+                       total, c->no, c->d_w, k);
+    *nblocks_out = (int)nb;
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce3(mdft_ctx *c, int nblocks) {
+    hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials, nblocks, c->d_sums);
+    return hipGetLastError();
+}

@@ -1,0 +1,341 @@
+// k_oz.hip -- rotation to the q frame, molecular Ornstein-Zernike contraction
+// with c(|q|), rotation back, FP64.
+//
+// Restates the q-loop of src/module_energy_cproj_mrso.f90:441-674 for each
+// (q, -q) pair: R^m_{mup,khi}(q) by the recursion of
+// src/module_rotation.f90:119-247 with the FLT_EPSILON chop of :491-492,
+// rotation (:505-523), linear interpolation of c in |q| (:534-539), the OZ
+// sums (:561-606), the DBL_EPSILON chop (:618-623), rotation back with conj(R)
+// (:611-641) and the store rule for self-conjugate Nyquist points (:648-656).
+// Both chops go through Fortran's default-kind CMPLX, so the component they
+// keep is rounded to single precision; that rounding is reproduced here.
+//
+// Work layout: pairs are pre-sorted by interpolation interval iq and cut into
+// chunks; one workgroup takes one chunk, stages the two c rows it needs in LDS
+// once, and its waves each take one pair at a time.  A pair's projections are
+// contiguous (np complex values per grid point), so its loads and stores are
+// contiguous runs.
+#include <cfloat>
+
+#include "mdft_internal.h"
+
+namespace {
+
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cmulc(double2 a, double2 b) {  // a * conj(b)
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+__device__ inline double2 chop_R(double2 z) {
+    double re = z.x, im = z.y;
+    if (fabs(re) < (double)FLT_EPSILON) { re = 0.0; im = (double)(float)im; }
+    if (fabs(im) < (double)FLT_EPSILON) { im = 0.0; re = (double)(float)re; }
+    return make_double2(re, im);
+}
+
+__device__ inline double2 chop_gamma(double2 z) {
+    double re = z.x, im = z.y;
+    if (fabs(re) < DBL_EPSILON && fabs(im) < DBL_EPSILON) { re = 0.0; im = 0.0; }
+    if (fabs(re) < DBL_EPSILON) { re = 0.0; im = (double)(float)im; }
+    if (fabs(im) < DBL_EPSILON) { im = 0.0; re = (double)(float)re; }
+    return make_double2(re, im);
+}
+
+__device__ inline double sqrtof(int k) { return k <= 0 ? 0.0 : sqrt((double)k); }
+
+// compact index of R(m, mup, khi): sum_{m'<m} (2m'+1)^2 + (mup+m)(2m+1) + khi+m
+__device__ inline int roff(int m) { return m * (4 * m * m - 1) / 3; }
+__device__ inline int ridx(int m, int a, int b) { return roff(m) + (a + m) * (2 * m + 1) + (b + m); }
+
+struct OzArgs {
+    int nx, ny, nz, np, npc, L, H, mrso, W;
+    double dq;
+    const double *kx, *ky, *kz;
+    const int *pidx, *pm, *pmup, *pmu, *ia0;
+    const int *pq, *pmq, *pflag, *cstart, *ccount, *ciq;
+    const double2 *ck;
+};
+
+__device__ inline int P(const OzArgs &a, int m, int mup, int mu2) {
+    return a.pidx[(m * a.W + mup + a.L) * (a.H + 1) + mu2];
+}
+
+// level-l entries of R read from level l-1 (compact storage), zero outside |m|,|m1| <= l-1
+__device__ inline double2 Rl(const double2 *R, int l1, int m, int m1) {
+    if (m < -l1 || m > l1 || m1 < -l1 || m1 > l1) return make_double2(0.0, 0.0);
+    return R[ridx(l1, m, m1)];
+}
+
+template <int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj, OzArgs a) {
+    constexpr int kThreads = 64 * kWaves;
+    extern __shared__ double2 sm[];
+    const int np = a.np, npc = a.npc, L = a.L;
+    const int nR = roff(L + 1);
+    double2 *c0 = sm, *c1 = c0 + npc;
+    double2 *wbase = c1 + npc + (threadIdx.x >> 6) * (nR + 6 * np);
+    double2 *R = wbase;
+    double2 *X0 = R + nR, *X1 = X0 + np, *A = X1 + np, *B = A + np, *GQ = B + np, *GMQ = GQ + np;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int chunk = blockIdx.x;
+    const int iq = a.ciq[chunk];  // 1-based lower row
+    for (int i = threadIdx.x; i < npc; i += kThreads) {
+        c0[i] = a.ck[(size_t)(iq - 1) * npc + i];
+        c1[i] = a.ck[(size_t)iq * npc + i];
+    }
+    __syncthreads();
+    const int start = a.cstart[chunk], count = a.ccount[chunk];
+    const int rounds = (count + kWaves - 1) / kWaves;
+    for (int rd = 0; rd < rounds; rd++) {
+        const int k = rd * kWaves + wv;
+        const bool active = k < count;
+        int gq = 0, gmq = 0, flag = 0;
+        double q0 = 0, q1 = 0, q2 = 0, alpha = 0;
+        if (active) {
+            const int pr = start + k;
+            gq = a.pq[pr];
+            gmq = a.pmq[pr];
+            flag = a.pflag[pr];
+            const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
+            q0 = a.kx[ix];
+            q1 = a.ky[iy];
+            q2 = a.kz[iz];
+            // alpha relative to the chunk's row so host and device agree on the interval
+            const double eff = sqrt(q0 * q0 + q1 * q1 + q2 * q2) / a.dq + 1.0;
+            alpha = eff - (double)iq;
+            for (int i = lane; i < np; i += 64) {
+                X0[i] = proj[(size_t)gq * np + i];
+                X1[i] = proj[(size_t)gmq * np + i];
+            }
+            // ---- R(q): frame and m = 1 block (module_rotation.f90:151-193)
+            double r3[3], r2[3], r1[3];
+            if (q0 == 0.0 && q1 == 0.0 && q2 == 0.0) { r3[0] = 0.0; r3[1] = 0.0; r3[2] = 1.0; }
+            else { r3[0] = q0; r3[1] = q1; r3[2] = q2; }
+            if (r3[0] != 0.0 || r3[1] != 0.0) { r2[0] = -r3[1]; r2[1] = r3[0]; r2[2] = 0.0; }
+            else { r2[0] = 0.0; r2[1] = r3[2]; r2[2] = -r3[1]; }
+            const double n3 = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]);
+            const double n2 = sqrt(r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+            for (int i = 0; i < 3; i++) { r3[i] /= n3; r2[i] /= n2; }
+            r1[0] = r2[1] * r3[2] - r2[2] * r3[1];
+            r1[1] = r2[2] * r3[0] - r2[0] * r3[2];
+            r1[2] = r2[0] * r3[1] - r2[1] * r3[0];
+            if (lane == 0) {
+                R[0] = make_double2(1.0, 0.0);
+                if (L >= 1) {
+                    const double s2 = sqrt(2.0);
+                    // rmat(i,j) = component i of column j: (r1, r2, r3)
+                    const double m11 = r1[0], m21 = r1[1], m31 = r1[2];
+                    const double m12 = r2[0], m22 = r2[1], m32 = r2[2];
+                    const double m13 = r3[0], m23 = r3[1], m33 = r3[2];
+                    R[ridx(1, -1, -1)] = make_double2((m22 + m11) * 0.5, (m21 - m12) * 0.5);
+                    R[ridx(1, -1, 0)] = make_double2(m13 / s2, m23 / s2);
+                    R[ridx(1, -1, 1)] = make_double2((m22 - m11) * 0.5, (-m21 - m12) * 0.5);
+                    R[ridx(1, 0, -1)] = make_double2(m31 / s2, -m32 / s2);
+                    R[ridx(1, 0, 0)] = make_double2(m33, 0.0);
+                    R[ridx(1, 0, 1)] = make_double2(-m31 / s2, -m32 / s2);
+                    R[ridx(1, 1, -1)] = make_double2((m22 - m11) * 0.5, (m21 + m12) * 0.5);
+                    R[ridx(1, 1, 0)] = make_double2(-m13 / s2, m23 / s2);
+                    R[ridx(1, 1, 1)] = make_double2((m22 + m11) * 0.5, (m12 - m21) * 0.5);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- R(q): recursion for l >= 2 (module_rotation.f90:202-245)
+        for (int l = 2; l <= L; l++) {
+            if (active) {
+                const int l1 = l - 1;
+                const double s2 = sqrt(2.0);
+                const double2 pm0 = R[ridx(1, -1, 0)], pp0 = R[ridx(1, 1, 0)], p00 = R[ridx(1, 0, 0)];
+                const double2 pm1 = R[ridx(1, -1, 1)], pp1 = R[ridx(1, 1, 1)], p01 = R[ridx(1, 0, 1)];
+                const int ntask = (l + 1) * (l + 1) + l * l;
+                for (int task = lane; task < ntask; task += 64) {
+                    // decode (m, m1): m <= 0 rows have m1 = 0..l, m > 0 rows m1 = 1..l
+                    int m, m1;
+                    const int nneg = (l + 1) * (l + 1);
+                    if (task < nneg) { m = -l + task / (l + 1); m1 = task % (l + 1); }
+                    else { const int t2 = task - nneg; m = 1 + t2 / l; m1 = 1 + t2 % l; }
+                    double fv, gv;
+                    if (m1 <= l - 1) {
+                        const double am = sqrtof(l + m) * sqrtof(l - m) / (sqrtof(l + m1) * sqrtof(l - m1));
+                        const double bp = sqrtof(l + m) * sqrtof(l + m - 1) / (s2 * sqrtof(l + m1) * sqrtof(l - m1));
+                        const double bm = sqrtof(l - m) * sqrtof(l - m - 1) / (s2 * sqrtof(l + m1) * sqrtof(l - m1));
+                        if (m == -l) {
+                            const double2 u = Rl(R, l1, m + 1, m1);
+                            fv = bm * (pm0.x * u.x - pm0.y * u.y);
+                            gv = bm * (pm0.x * u.y + pm0.y * u.x);
+                        } else if (m == l) {
+                            const double2 u = Rl(R, l1, m - 1, m1);
+                            fv = bp * (pp0.x * u.x - pp0.y * u.y);
+                            gv = bp * (pp0.x * u.y + pp0.y * u.x);
+                        } else {
+                            const double2 u0 = Rl(R, l1, m, m1), um = Rl(R, l1, m - 1, m1), up = Rl(R, l1, m + 1, m1);
+                            fv = am * (p00.x * u0.x) + bp * (pp0.x * um.x - pp0.y * um.y) + bm * (pm0.x * up.x - pm0.y * up.y);
+                            gv = am * (p00.x * u0.y) + bp * (pp0.x * um.y + pp0.y * um.x) + bm * (pm0.x * up.y + pm0.y * up.x);
+                        }
+                    } else {  // m1 == l
+                        const double cm = s2 * sqrtof(l + m) * sqrtof(l - m) / (sqrtof(l + l) * sqrtof(l + l - 1));
+                        const double dp = sqrtof(l + m) * sqrtof(l + m - 1) / (sqrtof(l + l) * sqrtof(l + l - 1));
+                        const double dm = sqrtof(l - m) * sqrtof(l - m - 1) / (sqrtof(l + l) * sqrtof(l + l - 1));
+                        if (m == -l) {
+                            const double2 u = Rl(R, l1, m + 1, m1 - 1);
+                            fv = dm * (pm1.x * u.x - pm1.y * u.y);
+                            gv = dm * (pm1.x * u.y + pm1.y * u.x);
+                        } else if (m == l) {
+                            const double2 u = Rl(R, l1, m - 1, m1 - 1);
+                            fv = dp * (pp1.x * u.x - pp1.y * u.y);
+                            gv = dp * (pp1.x * u.y + pp1.y * u.x);
+                        } else {
+                            const double2 u0 = Rl(R, l1, m, m1 - 1), um = Rl(R, l1, m - 1, m1 - 1), up = Rl(R, l1, m + 1, m1 - 1);
+                            fv = cm * (p01.x * u0.x - p01.y * u0.y) + dp * (pp1.x * um.x - pp1.y * um.y) + dm * (pm1.x * up.x - pm1.y * up.y);
+                            gv = cm * (p01.x * u0.y + p01.y * u0.x) + dp * (pp1.x * um.y + pp1.y * um.x) + dm * (pm1.x * up.y + pm1.y * up.x);
+                        }
+                    }
+                    const double sg = ((m + m1) & 1) ? -1.0 : 1.0;
+                    if (m == 0 && m1 == 0) {
+                        R[ridx(l, 0, 0)] = make_double2(fv, -gv);  // the mirror statement negates g in place
+                    } else {
+                        R[ridx(l, m, m1)] = make_double2(fv, gv);
+                        R[ridx(l, -m, -m1)] = make_double2(sg * fv, -sg * gv);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (active)
+            for (int i = lane; i < nR; i += 64) R[i] = chop_R(R[i]);
+        __syncthreads();
+        // ---- rotation to the q frame (:505-523)
+        if (active) {
+            for (int j = lane; j < 2 * np; j += 64) {
+                const int ip = j < np ? j : j - np;
+                const int m = a.pm[ip], khi = a.pmup[ip], mu2 = a.pmu[ip] / a.mrso;
+                double2 s = make_double2(0.0, 0.0);
+                if (j < np) {
+                    for (int mup = -m; mup <= m; mup++) {
+                        const double2 v = cmul(X0[P(a, m, mup, mu2)], R[ridx(m, mup, khi)]);
+                        s.x += v.x;
+                        s.y += v.y;
+                    }
+                    A[ip] = s;
+                } else {
+                    for (int mup = -m; mup <= m; mup++) {
+                        const double2 v = cmul(X1[P(a, m, mup, mu2)], R[ridx(m, mup, -khi)]);
+                        s.x += v.x;
+                        s.y += v.y;
+                    }
+                    const double sg = (m & 1) ? -1.0 : 1.0;
+                    B[ip] = make_double2(s.x * sg, s.y * sg);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- molecular OZ with interpolated c (:534-606)
+        if (active) {
+            for (int j = lane; j < 2 * np; j += 64) {
+                const int ip = j < np ? j : j - np;
+                const int khi = a.pmup[ip];
+                const double2 *src_neg = (j < np) ? A : B;  // nu2 < 0 term
+                const double2 *src_pos = (j < np) ? B : A;  // conj term
+                int ia = a.ia0[ip];
+                double2 s = make_double2(0.0, 0.0);
+                for (int n = abs(khi); n <= L; n++) {
+                    for (int nu2 = -n / a.mrso; nu2 <= n / a.mrso; nu2++, ia++) {
+                        const double2 lo = c0[ia], hi = c1[ia];
+                        const double2 ce = make_double2(alpha * hi.x + (1.0 - alpha) * lo.x,
+                                                        alpha * hi.y + (1.0 - alpha) * lo.y);
+                        double2 v;
+                        if (nu2 < 0) {
+                            v = cmul(ce, src_neg[P(a, n, khi, -nu2)]);
+                        } else {
+                            const double2 x = src_pos[P(a, n, khi, nu2)];
+                            v = cmul(ce, make_double2(x.x, -x.y));
+                        }
+                        s.x += v.x;
+                        s.y += v.y;
+                    }
+                }
+                if (j < np) GQ[ip] = chop_gamma(s);
+                else GMQ[ip] = chop_gamma(s);
+            }
+        }
+        __syncthreads();
+        // ---- rotation back with conj(R) (:611-641) and store (:648-656)
+        if (active) {
+            const bool eq = flag & 1, nyq = flag & 2;
+            for (int j = lane; j < 2 * np; j += 64) {
+                const int ip = j < np ? j : j - np;
+                const int m = a.pm[ip], mup = a.pmup[ip], mu2 = a.pmu[ip] / a.mrso;
+                double2 s = make_double2(0.0, 0.0);
+                if (j < np) {
+                    if (eq) continue;  // for q == -q the -q store overwrites this one
+                    for (int khi = -m; khi <= m; khi++) {
+                        const double2 v = cmulc(GQ[P(a, m, khi, mu2)], R[ridx(m, mup, khi)]);
+                        s.x += v.x;
+                        s.y += v.y;
+                    }
+                    proj[(size_t)gq * np + ip] = s;
+                } else {
+                    for (int khi = -m; khi <= m; khi++) {
+                        const double2 v = cmulc(GMQ[P(a, m, khi, mu2)], R[ridx(m, mup, -khi)]);
+                        s.x += v.x;
+                        s.y += v.y;
+                    }
+                    const double sg = (m & 1) ? -1.0 : 1.0;
+                    s.x *= sg;
+                    s.y *= sg;
+                    if (eq && nyq) s.y = -s.y;
+                    proj[(size_t)gmq * np + ip] = s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) {
+    OzArgs a;
+    a.nx = c->d.nx;
+    a.ny = c->d.ny;
+    a.nz = c->d.nz;
+    a.np = c->np;
+    a.npc = c->npc;
+    a.L = c->L;
+    a.H = c->H;
+    a.mrso = c->d.mrso;
+    a.W = 2 * c->L + 1;
+    a.dq = c->dq;
+    a.kx = c->d_kx;
+    a.ky = c->d_ky;
+    a.kz = c->d_kz;
+    a.pidx = c->d_pidx;
+    a.pm = c->d_pm;
+    a.pmup = c->d_pmup;
+    a.pmu = c->d_pmu;
+    a.ia0 = c->d_ia0;
+    a.pq = c->pairs.pq;
+    a.pmq = c->pairs.pmq;
+    a.pflag = c->pairs.pflag;
+    a.cstart = c->pairs.cstart;
+    a.ccount = c->pairs.ccount;
+    a.ciq = c->pairs.ciq;
+    a.ck = c->d_ck;
+    const int nR = (c->L + 1) * (4 * (c->L + 1) * (c->L + 1) - 1) / 3;
+    auto lds_for = [&](int waves) {
+        return sizeof(double2) * (2 * (size_t)c->npc + (size_t)waves * (nR + 6 * (size_t)c->np));
+    };
+    const dim3 grid((unsigned)c->pairs.nchunks);
+    if (lds_for(4) <= c->lds_limit)
+        hipLaunchKernelGGL(oz_f64<4>, grid, dim3(256), lds_for(4), c->stream, proj, a);
+    else if (lds_for(2) <= c->lds_limit)
+        hipLaunchKernelGGL(oz_f64<2>, grid, dim3(128), lds_for(2), c->stream, proj, a);
+    else if (lds_for(1) <= c->lds_limit)
+        hipLaunchKernelGGL(oz_f64<1>, grid, dim3(64), lds_for(1), c->stream, proj, a);
+    else
+        return hipErrorInvalidConfiguration;
+    return hipGetLastError();
+}

@@ -1,0 +1,100 @@
+// mdft_internal.h -- device-side context of the MI355X MDFT engine and the
+// launchers shared between the translation units of libmdft_gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mdft_gpu.h"
+
+#define MDFT_MAX_MMAX 6
+#define MDFT_FFT_MAX_STAGES 12
+
+// Angular tables of one context (all device pointers).
+struct AngTables {
+    int ntheta, nphi, npsi, no, np, L, H, mrso;
+    const double *w;     // [no] quadrature weights (module_grid.f90:139)
+    const double *a2p;   // [np][ntheta] wigner_d * wtheta
+    const double *p2a;   // [np][ntheta] wigner_d * fm(m)
+    const double *fm;    // [np] sqrt(2m+1) of each projection
+    const double *cph;   // [nphi] cos(2 pi k / nphi)
+    const double *sph;   // [nphi] sin(2 pi k / nphi)
+    const double *cps;   // [npsi] cos(2 pi k / npsi)
+    const double *sps;   // [npsi] sin(2 pi k / npsi)
+    const int *pidx;     // [(L+1)][2L+1][H+1] projection index of (m, mup, mu2)
+    const int *pm, *pmup, *pmu;  // [np]
+};
+
+struct FftAxisPlan {
+    int n;                       // transform length
+    int nst;                     // number of radix stages
+    int radix[MDFT_FFT_MAX_STAGES];
+    double2 *tw;                 // device [n]: exp(+2 pi i k / n)
+};
+
+// Schedule of the (q, -q) pairs of the OZ step, grouped in chunks of pairs that
+// share the same |q| interpolation interval.
+struct PairSchedule {
+    int npairs = 0, nchunks = 0;
+    int *pq = nullptr;        // linear grid index of q, per pair
+    int *pmq = nullptr;       // linear grid index of -q
+    int *pflag = nullptr;     // bit0: q == -q, bit1: q on a Nyquist plane
+    int *cstart = nullptr;    // first pair of each chunk
+    int *ccount = nullptr;    // pairs in each chunk
+    int *ciq = nullptr;       // 1-based lower interpolation row iq of each chunk
+};
+
+struct mdft_ctx {
+    mdft_desc d{};
+    int ntheta = 0, nphi = 0, npsi = 0, no = 0, np = 0, nmu2 = 0, L = 0, H = 0;
+    size_t N = 0;
+    double dv = 0, rho0 = 0;
+    // host tables
+    std::vector<double> theta, wtheta, w, wig;
+    std::vector<int> pm, pmup, pmu, pidx;
+    std::vector<double> kx, ky, kz;
+    // device tables
+    AngTables ang{};
+    double *d_w = nullptr, *d_a2p = nullptr, *d_p2a = nullptr, *d_fm = nullptr;
+    double *d_cph = nullptr, *d_sph = nullptr, *d_cps = nullptr, *d_sps = nullptr;
+    int *d_pidx = nullptr, *d_pm = nullptr, *d_pmup = nullptr, *d_pmu = nullptr;
+    int *d_ia0 = nullptr;         // [np] first OZ coefficient of each projection row
+    double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr;
+    FftAxisPlan fft[3]{};
+    // c(k)
+    int npc = 0, nq = 0;
+    double dq = 0;
+    double2 *d_ck = nullptr;      // [nq][npc]
+    PairSchedule pairs;
+    // fields
+    double *d_vext = nullptr;     // [N][no]
+    double2 *d_proj = nullptr;    // [N][np] (reference deltarho_p(np,nx,ny,nz))
+    double *d_xi_stage = nullptr; // staging for host xi
+    double *d_df_stage = nullptr; // staging for host df
+    double *d_partials = nullptr; // block partial sums
+    double *d_sums = nullptr;     // [8] final reductions
+    double *h_sums = nullptr;     // pinned mirror of d_sums
+    size_t partials_cap = 0;
+    bool have_vext = false, have_ck = false;
+    hipStream_t stream = nullptr;
+    size_t lds_limit = 65536;      // max dynamic LDS per workgroup on this device
+    hipEvent_t ev[8]{};
+    bool timers_ready = false;
+    double phase_ms[8]{};
+    int nphase = 0;
+    std::string err;
+};
+
+// ---- launchers (return hipError_t of the launch) ----
+hipError_t launch_angl2proj_f64(mdft_ctx *c, const double *xi, double2 *proj);
+// mode: 0 = energy_and_gradient (df overwritten with ideal+exc gradient),
+//       1 = excess only (df accumulated), 2 = ideal only (df accumulated)
+hipError_t launch_final_f64(mdft_ctx *c, const double2 *proj, const double *xi, double *df, int mode,
+                            int *nblocks_out);
+hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblocks_out);
+hipError_t launch_reduce3(mdft_ctx *c, int nblocks);
+hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
+hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
+int fft_make_plan(int n, int *radix);
