@@ -117,6 +117,27 @@ int mdft_excess_stage(mdft_ctx *ctx, const double *xi, int stage, double *out, i
  * number of entries written (<= n). */
 int mdft_phase_times(const mdft_ctx *ctx, double *ms, int n);
 
+/* External potential of a solute computed on the device into the context
+ * (replaces init_vext, src/module_vext.f90:14-115, for the Lennard-Jones part
+ * of src/module_lennardjones.f90:9-226 and a minimum-image Coulomb term),
+ * clipped at qfact/2 (src/module_vext.f90:638-640).
+ * atoms[natom][6] = x, y, z (A, box frame), q (e), sigma (A), epsilon (kJ/mol);
+ * sites[nsite][6] = x, y, z (A, molecular frame), q, sigma, epsilon. */
+int mdft_compute_vext(mdft_ctx *ctx, int natom, const double *atoms, int nsite, const double *sites,
+                      double rcut);
+/* Copies the context's vext(no,nx,ny,nz) out (host or, with MDFT_DEVICE_PTRS, device). */
+int mdft_get_vext(mdft_ctx *ctx, double *out, int flags);
+
+/* FP64 vector algebra on device vectors of length n (16-byte aligned device
+ * pointers), for a device-resident minimiser: the dot/axpy/copy/scal loops of
+ * src/module_blas_etc.f90:8-184 over n = no*nx*ny*nz. */
+int mdft_vec_dot(mdft_ctx *ctx, size_t n, const double *x, const double *y, double *out);
+/* y += a*x; if z and out are non-null, *out = z.y (one pass) */
+int mdft_vec_axpy_dot(mdft_ctx *ctx, size_t n, double a, const double *x, double *y, const double *z,
+                      double *out);
+/* z = a*x + b*y (y nullable: z = a*x); z may alias x or y */
+int mdft_vec_lincomb(mdft_ctx *ctx, size_t n, double a, const double *x, double b, const double *y, double *z);
+
 /* Device memory owned by the caller, for device-resident fields
  * (MDFT_DEVICE_PTRS) and the device L-BFGS vectors. */
 int mdft_device_alloc(mdft_ctx *ctx, size_t bytes, void **dptr);

@@ -437,7 +437,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
     void *bufs[] = {c->d_w,   c->d_a2p,  c->d_p2a,  c->d_fm,   c->d_cph,      c->d_sph,        c->d_cps,
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
-                    c->d_partials, c->d_sums};
+                    c->d_partials, c->d_sums, c->d_vpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);
@@ -651,6 +651,80 @@ int mdft_ideal_external(mdft_ctx *c, const double *xi, double *fid, double *fext
     }
     *fid = s[0];
     *fext = s[1];
+    return MDFT_OK;
+}
+
+int mdft_compute_vext(mdft_ctx *c, int natom, const double *atoms, int nsite, const double *sites, double rcut) {
+    if (!c || natom < 0 || nsite < 1 || (natom > 0 && !atoms) || !sites || !(rcut > 0))
+        return c ? fail(c, MDFT_EINVAL, "bad vext arguments") : MDFT_EINVAL;
+    // physical constants of src/module_constants.f90:54-57
+    const double pi = acos(-1.0);
+    const double eps0 = 8.854187817e-12, qunit = 1.602176565e-19, navo = 6.02214129e23;
+    const double qfact = qunit * qunit * 1.0e-3 * navo / (4.0 * pi * eps0 * 1.0e-10);
+    // site offsets per orientation: rotation matrix of src/module_grid.f90:187-206
+    std::vector<double> off((size_t)c->no * nsite * 3), sp((size_t)nsite * 3), at((size_t)(natom > 0 ? natom : 1) * 6);
+    for (int it = 0, io = 0; it < c->ntheta; it++)
+        for (int iphi = 0; iphi < c->nphi; iphi++)
+            for (int ipsi = 0; ipsi < c->npsi; ipsi++, io++) {
+                const double th = c->theta[it], ph = iphi * 2.0 * pi / c->nphi;
+                const double ps = ipsi * 2.0 * pi / c->d.mrso / c->npsi;
+                const double ct = cos(th), st = sin(th), cp = cos(ph), sph = sin(ph), cs = cos(ps), ss = sin(ps);
+                const double R[3][3] = {{ct * cp * cs - sph * ss, -ct * cp * ss - sph * cs, st * cp},
+                                        {ct * sph * cs + cp * ss, -ct * sph * ss + cp * cs, st * sph},
+                                        {-st * cs, st * ss, ct}};
+                for (int s = 0; s < nsite; s++)
+                    for (int k = 0; k < 3; k++)
+                        off[((size_t)io * nsite + s) * 3 + k] =
+                            R[k][0] * sites[6 * s] + R[k][1] * sites[6 * s + 1] + R[k][2] * sites[6 * s + 2];
+            }
+    for (int s = 0; s < nsite; s++) {
+        sp[3 * s] = sites[6 * s + 3];
+        sp[3 * s + 1] = sites[6 * s + 4];
+        sp[3 * s + 2] = sites[6 * s + 5];
+    }
+    for (int i = 0; i < natom * 6; i++) at[i] = atoms[i];
+    double *d_off = nullptr, *d_sp = nullptr, *d_at = nullptr;
+    HIPCHK(c, upload(c, &d_off, off.data(), off.size()));
+    HIPCHK(c, upload(c, &d_sp, sp.data(), sp.size()));
+    HIPCHK(c, upload(c, &d_at, at.data(), at.size()));
+    hipError_t e = launch_vext(c, natom, d_at, nsite, d_sp, d_off, rcut, qfact, qfact / 2.0);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_off);
+    (void)hipFree(d_sp);
+    (void)hipFree(d_at);
+    HIPCHK(c, e);
+    c->have_vext = true;
+    return MDFT_OK;
+}
+
+int mdft_get_vext(mdft_ctx *c, double *out, int flags) {
+    if (!c || !out) return MDFT_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_vext, sizeof(double) * c->N * c->no,
+                             (flags & MDFT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+int mdft_vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out) {
+    if (!c || !x || !y || !out) return MDFT_EINVAL;
+    if (!aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    HIPCHK(c, vec_dot(c, n, x, y, out));
+    return MDFT_OK;
+}
+
+int mdft_vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out) {
+    if (!c || !x || !y) return MDFT_EINVAL;
+    if (!aligned16(x) || !aligned16(y) || (z && !aligned16(z)))
+        return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    HIPCHK(c, vec_axpy_dot(c, n, a, x, y, z, out));
+    return MDFT_OK;
+}
+
+int mdft_vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z) {
+    if (!c || !x || !z) return MDFT_EINVAL;
+    HIPCHK(c, vec_lincomb(c, n, a, x, b, y, z));
     return MDFT_OK;
 }
 

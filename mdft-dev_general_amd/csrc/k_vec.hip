@@ -1,0 +1,145 @@
+// k_vec.hip -- FP64 vector algebra over device-resident length-n vectors for
+// the minimiser (replaces the dot/axpy/copy/scal loops of
+// src/module_blas_etc.f90:8-184 that the L-BFGS of
+// src/module_lbfgs_nocedal_mdft.f90 runs over n = no*nx*ny*nz).
+//
+// All kernels stream with 16-byte loads; dot products reduce per block in
+// FP64 and then in one final block, so results are deterministic.
+#include "mdft_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxBlocks = 4096;
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ inline void block_store_sum(double v, double *partials) {
+    __shared__ double red[kThreads / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < kThreads / 64; k++) s += red[k];
+        partials[blockIdx.x] = s;
+    }
+}
+
+// out[blockIdx] = sum x*y over the block's grid-stride share
+__global__ __launch_bounds__(kThreads) void dot_partial(size_t n, const double *__restrict__ x,
+                                                       const double *__restrict__ y, double *__restrict__ partials) {
+    const size_t n2 = n / 2;
+    const double2 *x2 = reinterpret_cast<const double2 *>(x);
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += (size_t)gridDim.x * kThreads) {
+        const double2 a = x2[i], b = y2[i];
+        s += a.x * b.x + a.y * b.y;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) s += x[n - 1] * y[n - 1];
+    block_store_sum(s, partials);
+}
+
+// y += a x, then partial sums of z . y
+__global__ __launch_bounds__(kThreads) void axpy_dot_partial(size_t n, double a, const double *__restrict__ x,
+                                                            double *__restrict__ y, const double *__restrict__ z,
+                                                            double *__restrict__ partials) {
+    const size_t n2 = n / 2;
+    const double2 *x2 = reinterpret_cast<const double2 *>(x);
+    double2 *y2 = reinterpret_cast<double2 *>(y);
+    const double2 *z2 = reinterpret_cast<const double2 *>(z);
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += (size_t)gridDim.x * kThreads) {
+        double2 v = y2[i];
+        const double2 u = x2[i];
+        v.x += a * u.x;
+        v.y += a * u.y;
+        y2[i] = v;
+        if (z) {
+            const double2 w = z2[i];
+            s += w.x * v.x + w.y * v.y;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+        y[n - 1] += a * x[n - 1];
+        if (z) s += z[n - 1] * y[n - 1];
+    }
+    block_store_sum(s, partials);
+}
+
+__global__ __launch_bounds__(kThreads) void sum_partials(const double *__restrict__ partials, int nb,
+                                                        double *__restrict__ out) {
+    __shared__ double red[kThreads];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kThreads) s += partials[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = kThreads / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// z = a x + b y   (z may alias x or y)
+__global__ __launch_bounds__(kThreads) void lincomb(size_t n, double a, const double *x, double b, const double *y,
+                                                   double *z) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        z[i] = a * x[i] + (y ? b * y[i] : 0.0);
+}
+
+unsigned blocks_for(size_t n) {
+    size_t b = (n / 2 + kThreads - 1) / kThreads;
+    if (b < 1) b = 1;
+    if (b > kMaxBlocks) b = kMaxBlocks;
+    return (unsigned)b;
+}
+
+hipError_t ensure_vec_partials(mdft_ctx *c) {
+    if (c->d_vpart) return hipSuccess;
+    return hipMalloc(&c->d_vpart, sizeof(double) * (kMaxBlocks + 8));
+}
+
+hipError_t finish_sum(mdft_ctx *c, unsigned nb, double *out) {
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kThreads), 0, c->stream, c->d_vpart, (int)nb, c->d_vpart + kMaxBlocks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(c->h_sums + 4, c->d_vpart + kMaxBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) *out = c->h_sums[4];
+    return e;
+}
+
+}  // namespace
+
+hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out) {
+    hipError_t e = ensure_vec_partials(c);
+    if (e != hipSuccess) return e;
+    const unsigned nb = blocks_for(n);
+    hipLaunchKernelGGL(dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, x, y, c->d_vpart);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return finish_sum(c, nb, out);
+}
+
+hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out) {
+    hipError_t e = ensure_vec_partials(c);
+    if (e != hipSuccess) return e;
+    const unsigned nb = blocks_for(n);
+    hipLaunchKernelGGL(axpy_dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, a, x, y, z, c->d_vpart);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (!z || !out) return hipStreamSynchronize(c->stream);
+    return finish_sum(c, nb, out);
+}
+
+hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z) {
+    const unsigned nb = blocks_for(2 * n);
+    hipLaunchKernelGGL(lincomb, dim3(nb), dim3(kThreads), 0, c->stream, n, a, x, b, y, z);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}

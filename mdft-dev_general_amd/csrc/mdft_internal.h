@@ -75,6 +75,7 @@ struct mdft_ctx {
     double *d_df_stage = nullptr; // staging for host df
     double *d_partials = nullptr; // block partial sums
     double *d_sums = nullptr;     // [8] final reductions
+    double *d_vpart = nullptr;    // vector-op block partials
     double *h_sums = nullptr;     // pinned mirror of d_sums
     size_t partials_cap = 0;
     bool have_vext = false, have_ck = false;
@@ -98,3 +99,8 @@ hipError_t launch_reduce3(mdft_ctx *c, int nblocks);
 hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 int fft_make_plan(int n, int *radix);
+hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out);
+hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out);
+hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z);
+hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,
+                       const double *d_site_off, double rcut, double qfact, double clip);
