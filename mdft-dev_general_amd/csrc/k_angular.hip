@@ -275,6 +275,186 @@ __global__ __launch_bounds__(kThreads) void reduce3(const double *__restrict__ p
     }
 }
 
+// ---- specialisations for mrso = 2 with compile-time angular sizes: all
+// index arithmetic uses constant divisors, loops have fixed trip counts, and
+// the phase / Wigner / weight tables are staged in LDS once per workgroup.
+template <int L>
+struct Dims {
+    static constexpr int NTH = L + 1, NPHI = 2 * L + 1, H = L / 2, H1 = H + 1, NPSI = 2 * H + 1, W = 2 * L + 1;
+    static constexpr int NO = NTH * NPHI * NPSI;
+    static constexpr int np_of(int m) { return m < 0 ? 0 : np_of(m - 1) + (2 * m + 1) * (m / 2 + 1); }
+    static constexpr int NP = np_of(L);
+};
+
+template <int L, int PT>
+__global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, double2 *__restrict__ proj,
+                                                       size_t N, AngTables t, double rho0) {
+    using D = Dims<L>;
+    constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
+    __shared__ double o[PT * NO];
+    __shared__ double2 G[PT * NTH * NPHI * H1];
+    __shared__ double2 F[PT * NTH * W * H1];
+    __shared__ double cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], a2p[NP * NTH], fmv[NP];
+    __shared__ int pmup[NP], pmu2[NP];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = t.cph[i]; sph[i] = t.sph[i]; }
+    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = t.cps[i]; sps[i] = t.sps[i]; }
+    for (int i = tid; i < NP * NTH; i += kThreads) a2p[i] = t.a2p[i];
+    for (int i = tid; i < NP; i += kThreads) { fmv[i] = t.fm[i]; pmup[i] = t.pmup[i]; pmu2[i] = t.pmu[i] / 2; }
+    const size_t r0 = (size_t)blockIdx.x * PT;
+    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+    const double *src = xi + r0 * NO;
+    for (int i = tid; i < npts * NO; i += kThreads) {
+        const double x = src[i];
+        o[i] = rho0 * (x * x - 1.0);
+    }
+    __syncthreads();
+    for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
+        const int mu2 = i % H1;
+        const double *line = o + (i / H1) * NPSI;
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int ipsi = 0; ipsi < NPSI; ipsi++) {
+            const int k = (mu2 * ipsi) % NPSI;
+            const double v = line[ipsi];
+            re += v * cps[k];
+            im += v * sps[k];
+        }
+        G[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    const double norm = 1.0 / (double)(NPHI * NPSI);
+    for (int i = tid; i < npts * NTH * W * H1; i += kThreads) {
+        const int mu2 = i % H1;
+        const int rest = i / H1;
+        const int mup = rest % W - L;
+        const double2 *g = G + (rest / W) * NPHI * H1 + mu2;
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int iphi = 0; iphi < NPHI; iphi++) {
+            int k = (mup * iphi) % NPHI;
+            k += (k < 0) ? NPHI : 0;
+            const double2 v = g[iphi * H1];
+            re += v.x * cph[k] - v.y * sph[k];
+            im += v.x * sph[k] + v.y * cph[k];
+        }
+        F[i] = make_double2(re * norm, im * norm);
+    }
+    __syncthreads();
+    double2 *dst = proj + r0 * NP;
+    for (int i = tid; i < npts * NP; i += kThreads) {
+        const int pt = i / NP, ip = i - pt * NP;
+        const double2 *f = F + (pt * NTH * W + (pmup[ip] + L)) * H1 + pmu2[ip];
+        const double *a = a2p + ip * NTH;
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int it = 0; it < NTH; it++) {
+            const double2 v = f[it * W * H1];
+            re += v.x * a[it];
+            im += v.y * a[it];
+        }
+        dst[i] = make_double2(re * fmv[ip], im * fmv[ip]);
+    }
+}
+
+template <int L, int PT>
+__global__ __launch_bounds__(kThreads) void final_t(const double2 *__restrict__ proj, const double *__restrict__ xi,
+                                                   const double *__restrict__ vext, double *__restrict__ df,
+                                                   double *__restrict__ partials, size_t N, AngTables t,
+                                                   FinalConsts k, int mode) {
+    using D = Dims<L>;
+    constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
+    __shared__ double2 Gm[PT * NP];
+    __shared__ double2 f[PT * NTH * W * H1];
+    __shared__ double2 h[PT * NTH * NPHI * H1];
+    __shared__ double cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], p2a[NP * NTH], wq[NO];
+    __shared__ int pidx[(L + 1) * W * H1];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
+    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = t.cph[i]; sph[i] = t.sph[i]; }
+    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = t.cps[i]; sps[i] = t.sps[i]; }
+    for (int i = tid; i < NP * NTH; i += kThreads) p2a[i] = t.p2a[i];
+    for (int i = tid; i < NO; i += kThreads) wq[i] = t.w[i];
+    const size_t r0 = (size_t)blockIdx.x * PT;
+    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+    const double2 *src = proj + r0 * NP;
+    for (int i = tid; i < npts * NP; i += kThreads) Gm[i] = src[i];
+    __syncthreads();
+    // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m)
+    for (int i = tid; i < npts * NTH * W * H1; i += kThreads) {
+        const int mu2 = i % H1;
+        int rest = i / H1;
+        const int mup = rest % W - L;
+        rest /= W;
+        const int it = rest % NTH, pt = rest / NTH;
+        const int a = mup < 0 ? -mup : mup, b = 2 * mu2;
+        const int mlo = a > b ? a : b;
+        double re = 0.0, im = 0.0;
+        for (int m = mlo; m <= L; m++) {
+            const int p = pidx[(m * W + mup + L) * H1 + mu2];
+            const double2 v = Gm[pt * NP + p];
+            const double c = p2a[p * NTH + it];
+            re += v.x * c;
+            im += v.y * c;
+        }
+        f[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
+        const int mu2 = i % H1;
+        int rest = i / H1;
+        const int iphi = rest % NPHI;
+        rest /= NPHI;
+        const double2 *fr = f + rest * W * H1 + mu2;
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int mup = -L; mup <= L; mup++) {
+            int kk = (mup * iphi) % NPHI;
+            kk += (kk < 0) ? NPHI : 0;
+            const double2 v = fr[(mup + L) * H1];
+            re += v.x * cph[kk] + v.y * sph[kk];
+            im += v.x * sph[kk] - v.y * cph[kk];
+        }
+        h[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    double fid = 0.0, fext = 0.0, fexc = 0.0;
+    const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
+    for (int i = tid; i < npts * NO; i += kThreads) {
+        const int pt = i / NO, io = i - pt * NO;
+        const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
+        const double2 *hr = h + (pt * NTH * NPHI + line) * H1;
+        double v = hr[0].x;
+#pragma unroll
+        for (int mu2 = 1; mu2 < H1; mu2++) {
+            const int kk = (mu2 * ipsi) % NPSI;
+            v += 2.0 * (hr[mu2].x * cps[kk] - hr[mu2].y * sps[kk]);
+        }
+        const double wio = wq[io];
+        const double vexc = k.prefactor * wio * v;
+        const size_t g = (r0 + pt) * (size_t)NO + io;
+        const double x = xi[g];
+        const double rho = x * x * rho0;
+        fexc += (x * x * rho0 - rho0) * vexc;
+        if (mode == 0) {
+            const double ve = vext[g];
+            double gid = 0.0;
+            if (rho > DBL_EPSILON) {
+                const double lg = log(rho / rho0);
+                fid += kT * dv * wio * (rho * lg - rho + rho0);
+                fext += dv * wio * rho * ve;
+                gid = wio * 2.0 * rho0 * x * (kT * lg + ve);
+            } else {
+                fid += kT * dv * wio * rho0;
+            }
+            if (df) df[g] = gid + 2.0 * rho0 * x * vexc;
+        } else {
+            if (df) df[g] += 2.0 * rho0 * x * vexc;
+        }
+    }
+    block_reduce3(fid, fext, fexc, partials);
+}
+
 constexpr int kPT = 4;  // points per workgroup tile (keeps the LDS tile under 64 KB at mmax = 5)
 
 size_t a2p_lds(const AngTables &t) {
@@ -297,7 +477,34 @@ hipError_t ensure_partials(mdft_ctx *c, size_t nblocks) {
 
 }  // namespace
 
+template <int PT>
+bool launch_a2p_t(mdft_ctx *c, const double *xi, double2 *proj) {
+    const size_t nb = (c->N + PT - 1) / PT;
+    switch (c->L) {
+#define A2P_CASE(LL) \
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0); return true;
+        A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
+#undef A2P_CASE
+        default: return false;
+    }
+}
+
+template <int PT>
+bool launch_final_t(mdft_ctx *c, const double2 *proj, const double *xi, double *df, int mode, FinalConsts k) {
+    const size_t nb = (c->N + PT - 1) / PT;
+    switch (c->L) {
+#define FIN_CASE(LL) \
+    case LL: hipLaunchKernelGGL((final_t<LL, PT>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, c->N, c->ang, k, mode); return true;
+        FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
+#undef FIN_CASE
+        default: return false;
+    }
+}
+
+constexpr int kPTt = 4;  // points per workgroup of the specialised kernels
+
 hipError_t launch_angl2proj_f64(mdft_ctx *c, const double *xi, double2 *proj) {
+    if (c->d.mrso == 2 && launch_a2p_t<kPTt>(c, xi, proj)) return hipGetLastError();
     const size_t nb = (c->N + kPT - 1) / kPT;
     if (a2p_lds(c->ang) > c->lds_limit) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(angl2proj_f64<kPT>, dim3((unsigned)nb), dim3(kThreads), a2p_lds(c->ang), c->stream, xi, proj,
@@ -318,6 +525,14 @@ hipError_t launch_final_f64(mdft_ctx *c, const double2 *proj, const double *xi, 
     const double pi = acos(-1.0);
     const double fourpisq = 4.0 * pi * pi;
     k.prefactor = -c->d.kT * 2.0 * fourpisq / c->d.mrso / c->d.n0;  // cproj_mrso.f90:728
+    if (c->d.mrso == 2) {
+        const size_t nbt = (c->N + kPTt - 1) / kPTt;
+        if ((e = ensure_partials(c, nbt)) != hipSuccess) return e;
+        if (launch_final_t<kPTt>(c, proj, xi, df, mode, k)) {
+            *nblocks_out = (int)nbt;
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL(final_f64<kPT>, dim3((unsigned)nb), dim3(kThreads), final_lds(c->ang), c->stream, proj, xi,
                        c->d_vext, df, c->d_partials, c->N, c->ang, k, mode);
     *nblocks_out = (int)nb;

@@ -59,8 +59,12 @@ struct OzArgs {
     const double2 *ck;
 };
 
-__device__ inline int P(const OzArgs &a, int m, int mup, int mu2) {
-    return a.pidx[(m * a.W + mup + a.L) * (a.H + 1) + mu2];
+// Per-wave LDS visibility: the lanes of one wave exchange data through LDS
+// without a workgroup barrier (every pair is owned by exactly one wave).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // level-l entries of R read from level l-1 (compact storage), zero outside |m|,|m1| <= l-1
@@ -73,10 +77,12 @@ template <int kWaves>
 __global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj, OzArgs a) {
     constexpr int kThreads = 64 * kWaves;
     extern __shared__ double2 sm[];
-    const int np = a.np, npc = a.npc, L = a.L;
+    const int np = a.np, npc = a.npc, L = a.L, mrso = a.mrso;
     const int nR = roff(L + 1);
     double2 *c0 = sm, *c1 = c0 + npc;
     double2 *wbase = c1 + npc + (threadIdx.x >> 6) * (nR + 6 * np);
+    int *tabs = reinterpret_cast<int *>(c1 + npc + kWaves * (nR + 6 * np));
+    int *tm = tabs, *tmup = tm + np, *tmu2 = tmup + np, *tia0 = tmu2 + np, *poff = tia0 + np;
     double2 *R = wbase;
     double2 *X0 = R + nR, *X1 = X0 + np, *A = X1 + np, *B = A + np, *GQ = B + np, *GMQ = GQ + np;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -86,213 +92,215 @@ __global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj
         c0[i] = a.ck[(size_t)(iq - 1) * npc + i];
         c1[i] = a.ck[(size_t)iq * npc + i];
     }
+    for (int i = threadIdx.x; i < np; i += kThreads) {
+        tm[i] = a.pm[i];
+        tmup[i] = a.pmup[i];
+        tmu2[i] = a.pmu[i] / mrso;
+        tia0[i] = a.ia0[i];
+    }
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int m = 0; m <= L; m++) {
+            poff[m] = o;
+            o += (2 * m + 1) * (m / mrso + 1);
+        }
+    }
     __syncthreads();
+    // projection index of (m, mup, mu2): cproj_mrso.f90:156-175 ordering
+#define PIDX(m, mup, mu2) (poff[(m)] + ((mup) + (m)) * ((m) / mrso + 1) + (mu2))
     const int start = a.cstart[chunk], count = a.ccount[chunk];
-    const int rounds = (count + kWaves - 1) / kWaves;
-    for (int rd = 0; rd < rounds; rd++) {
-        const int k = rd * kWaves + wv;
-        const bool active = k < count;
-        int gq = 0, gmq = 0, flag = 0;
-        double q0 = 0, q1 = 0, q2 = 0, alpha = 0;
-        if (active) {
-            const int pr = start + k;
-            gq = a.pq[pr];
-            gmq = a.pmq[pr];
-            flag = a.pflag[pr];
-            const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
-            q0 = a.kx[ix];
-            q1 = a.ky[iy];
-            q2 = a.kz[iz];
-            // alpha relative to the chunk's row so host and device agree on the interval
-            const double eff = sqrt(q0 * q0 + q1 * q1 + q2 * q2) / a.dq + 1.0;
-            alpha = eff - (double)iq;
-            for (int i = lane; i < np; i += 64) {
-                X0[i] = proj[(size_t)gq * np + i];
-                X1[i] = proj[(size_t)gmq * np + i];
-            }
-            // ---- R(q): frame and m = 1 block (module_rotation.f90:151-193)
-            double r3[3], r2[3], r1[3];
-            if (q0 == 0.0 && q1 == 0.0 && q2 == 0.0) { r3[0] = 0.0; r3[1] = 0.0; r3[2] = 1.0; }
-            else { r3[0] = q0; r3[1] = q1; r3[2] = q2; }
-            if (r3[0] != 0.0 || r3[1] != 0.0) { r2[0] = -r3[1]; r2[1] = r3[0]; r2[2] = 0.0; }
-            else { r2[0] = 0.0; r2[1] = r3[2]; r2[2] = -r3[1]; }
-            const double n3 = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]);
-            const double n2 = sqrt(r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
-            for (int i = 0; i < 3; i++) { r3[i] /= n3; r2[i] /= n2; }
-            r1[0] = r2[1] * r3[2] - r2[2] * r3[1];
-            r1[1] = r2[2] * r3[0] - r2[0] * r3[2];
-            r1[2] = r2[0] * r3[1] - r2[1] * r3[0];
-            if (lane == 0) {
-                R[0] = make_double2(1.0, 0.0);
-                if (L >= 1) {
-                    const double s2 = sqrt(2.0);
-                    // rmat(i,j) = component i of column j: (r1, r2, r3)
-                    const double m11 = r1[0], m21 = r1[1], m31 = r1[2];
-                    const double m12 = r2[0], m22 = r2[1], m32 = r2[2];
-                    const double m13 = r3[0], m23 = r3[1], m33 = r3[2];
-                    R[ridx(1, -1, -1)] = make_double2((m22 + m11) * 0.5, (m21 - m12) * 0.5);
-                    R[ridx(1, -1, 0)] = make_double2(m13 / s2, m23 / s2);
-                    R[ridx(1, -1, 1)] = make_double2((m22 - m11) * 0.5, (-m21 - m12) * 0.5);
-                    R[ridx(1, 0, -1)] = make_double2(m31 / s2, -m32 / s2);
-                    R[ridx(1, 0, 0)] = make_double2(m33, 0.0);
-                    R[ridx(1, 0, 1)] = make_double2(-m31 / s2, -m32 / s2);
-                    R[ridx(1, 1, -1)] = make_double2((m22 - m11) * 0.5, (m21 + m12) * 0.5);
-                    R[ridx(1, 1, 0)] = make_double2(-m13 / s2, m23 / s2);
-                    R[ridx(1, 1, 1)] = make_double2((m22 + m11) * 0.5, (m12 - m21) * 0.5);
-                }
+    const double s2 = sqrt(2.0);
+    for (int k = wv; k < count; k += kWaves) {
+        const int pr = start + k;
+        const int gq = a.pq[pr], gmq = a.pmq[pr], flag = a.pflag[pr];
+        const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
+        const double q0 = a.kx[ix], q1 = a.ky[iy], q2 = a.kz[iz];
+        // alpha relative to the chunk's row so host and device agree on the interval
+        const double alpha = sqrt(q0 * q0 + q1 * q1 + q2 * q2) / a.dq + 1.0 - (double)iq;
+        const double2 *src0 = proj + (size_t)gq * np, *src1 = proj + (size_t)gmq * np;
+        for (int i = lane; i < np; i += 64) {
+            X0[i] = src0[i];
+            X1[i] = src1[i];
+        }
+        // ---- R(q): frame and m = 1 block (module_rotation.f90:151-193)
+        double r3[3], r2[3], r1[3];
+        if (q0 == 0.0 && q1 == 0.0 && q2 == 0.0) { r3[0] = 0.0; r3[1] = 0.0; r3[2] = 1.0; }
+        else { r3[0] = q0; r3[1] = q1; r3[2] = q2; }
+        if (r3[0] != 0.0 || r3[1] != 0.0) { r2[0] = -r3[1]; r2[1] = r3[0]; r2[2] = 0.0; }
+        else { r2[0] = 0.0; r2[1] = r3[2]; r2[2] = -r3[1]; }
+        const double n3 = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]);
+        const double n2 = sqrt(r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+        for (int i = 0; i < 3; i++) { r3[i] /= n3; r2[i] /= n2; }
+        r1[0] = r2[1] * r3[2] - r2[2] * r3[1];
+        r1[1] = r2[2] * r3[0] - r2[0] * r3[2];
+        r1[2] = r2[0] * r3[1] - r2[1] * r3[0];
+        // rmat(i,j) = component i of column j: (r1, r2, r3)
+        const double m11 = r1[0], m21 = r1[1], m31 = r1[2];
+        const double m12 = r2[0], m22 = r2[1], m32 = r2[2];
+        const double m13 = r3[0], m23 = r3[1], m33 = r3[2];
+        const double2 p00 = make_double2(m33, 0.0);
+        const double2 pm0 = make_double2(m13 / s2, m23 / s2), pp0 = make_double2(-m13 / s2, m23 / s2);
+        const double2 pm1 = make_double2((m22 - m11) * 0.5, (-m21 - m12) * 0.5);
+        const double2 pp1 = make_double2((m22 + m11) * 0.5, (m12 - m21) * 0.5);
+        const double2 p01 = make_double2(-m31 / s2, -m32 / s2);
+        if (lane == 0) {
+            R[0] = make_double2(1.0, 0.0);
+            if (L >= 1) {
+                R[ridx(1, -1, -1)] = make_double2((m22 + m11) * 0.5, (m21 - m12) * 0.5);
+                R[ridx(1, -1, 0)] = pm0;
+                R[ridx(1, -1, 1)] = pm1;
+                R[ridx(1, 0, -1)] = make_double2(m31 / s2, -m32 / s2);
+                R[ridx(1, 0, 0)] = p00;
+                R[ridx(1, 0, 1)] = p01;
+                R[ridx(1, 1, -1)] = make_double2((m22 - m11) * 0.5, (m21 + m12) * 0.5);
+                R[ridx(1, 1, 0)] = pp0;
+                R[ridx(1, 1, 1)] = pp1;
             }
         }
-        __syncthreads();
+        wave_sync();
         // ---- R(q): recursion for l >= 2 (module_rotation.f90:202-245)
         for (int l = 2; l <= L; l++) {
-            if (active) {
-                const int l1 = l - 1;
-                const double s2 = sqrt(2.0);
-                const double2 pm0 = R[ridx(1, -1, 0)], pp0 = R[ridx(1, 1, 0)], p00 = R[ridx(1, 0, 0)];
-                const double2 pm1 = R[ridx(1, -1, 1)], pp1 = R[ridx(1, 1, 1)], p01 = R[ridx(1, 0, 1)];
-                const int ntask = (l + 1) * (l + 1) + l * l;
-                for (int task = lane; task < ntask; task += 64) {
-                    // decode (m, m1): m <= 0 rows have m1 = 0..l, m > 0 rows m1 = 1..l
-                    int m, m1;
-                    const int nneg = (l + 1) * (l + 1);
-                    if (task < nneg) { m = -l + task / (l + 1); m1 = task % (l + 1); }
-                    else { const int t2 = task - nneg; m = 1 + t2 / l; m1 = 1 + t2 % l; }
-                    double fv, gv;
-                    if (m1 <= l - 1) {
-                        const double am = sqrtof(l + m) * sqrtof(l - m) / (sqrtof(l + m1) * sqrtof(l - m1));
-                        const double bp = sqrtof(l + m) * sqrtof(l + m - 1) / (s2 * sqrtof(l + m1) * sqrtof(l - m1));
-                        const double bm = sqrtof(l - m) * sqrtof(l - m - 1) / (s2 * sqrtof(l + m1) * sqrtof(l - m1));
-                        if (m == -l) {
-                            const double2 u = Rl(R, l1, m + 1, m1);
-                            fv = bm * (pm0.x * u.x - pm0.y * u.y);
-                            gv = bm * (pm0.x * u.y + pm0.y * u.x);
-                        } else if (m == l) {
-                            const double2 u = Rl(R, l1, m - 1, m1);
-                            fv = bp * (pp0.x * u.x - pp0.y * u.y);
-                            gv = bp * (pp0.x * u.y + pp0.y * u.x);
-                        } else {
-                            const double2 u0 = Rl(R, l1, m, m1), um = Rl(R, l1, m - 1, m1), up = Rl(R, l1, m + 1, m1);
-                            fv = am * (p00.x * u0.x) + bp * (pp0.x * um.x - pp0.y * um.y) + bm * (pm0.x * up.x - pm0.y * up.y);
-                            gv = am * (p00.x * u0.y) + bp * (pp0.x * um.y + pp0.y * um.x) + bm * (pm0.x * up.y + pm0.y * up.x);
-                        }
-                    } else {  // m1 == l
-                        const double cm = s2 * sqrtof(l + m) * sqrtof(l - m) / (sqrtof(l + l) * sqrtof(l + l - 1));
-                        const double dp = sqrtof(l + m) * sqrtof(l + m - 1) / (sqrtof(l + l) * sqrtof(l + l - 1));
-                        const double dm = sqrtof(l - m) * sqrtof(l - m - 1) / (sqrtof(l + l) * sqrtof(l + l - 1));
-                        if (m == -l) {
-                            const double2 u = Rl(R, l1, m + 1, m1 - 1);
-                            fv = dm * (pm1.x * u.x - pm1.y * u.y);
-                            gv = dm * (pm1.x * u.y + pm1.y * u.x);
-                        } else if (m == l) {
-                            const double2 u = Rl(R, l1, m - 1, m1 - 1);
-                            fv = dp * (pp1.x * u.x - pp1.y * u.y);
-                            gv = dp * (pp1.x * u.y + pp1.y * u.x);
-                        } else {
-                            const double2 u0 = Rl(R, l1, m, m1 - 1), um = Rl(R, l1, m - 1, m1 - 1), up = Rl(R, l1, m + 1, m1 - 1);
-                            fv = cm * (p01.x * u0.x - p01.y * u0.y) + dp * (pp1.x * um.x - pp1.y * um.y) + dm * (pm1.x * up.x - pm1.y * up.y);
-                            gv = cm * (p01.x * u0.y + p01.y * u0.x) + dp * (pp1.x * um.y + pp1.y * um.x) + dm * (pm1.x * up.y + pm1.y * up.x);
-                        }
-                    }
-                    const double sg = ((m + m1) & 1) ? -1.0 : 1.0;
-                    if (m == 0 && m1 == 0) {
-                        R[ridx(l, 0, 0)] = make_double2(fv, -gv);  // the mirror statement negates g in place
+            const int l1 = l - 1;
+            const int ntask = (l + 1) * (l + 1) + l * l;
+            for (int task = lane; task < ntask; task += 64) {
+                // m <= 0 rows have m1 = 0..l, m > 0 rows m1 = 1..l
+                int m, m1;
+                const int nneg = (l + 1) * (l + 1);
+                if (task < nneg) { m = -l + task / (l + 1); m1 = task % (l + 1); }
+                else { const int t2 = task - nneg; m = 1 + t2 / l; m1 = 1 + t2 % l; }
+                double fv, gv;
+                if (m1 <= l - 1) {
+                    const double am = sqrtof(l + m) * sqrtof(l - m) / (sqrtof(l + m1) * sqrtof(l - m1));
+                    const double bp = sqrtof(l + m) * sqrtof(l + m - 1) / (s2 * sqrtof(l + m1) * sqrtof(l - m1));
+                    const double bm = sqrtof(l - m) * sqrtof(l - m - 1) / (s2 * sqrtof(l + m1) * sqrtof(l - m1));
+                    if (m == -l) {
+                        const double2 u = Rl(R, l1, m + 1, m1);
+                        fv = bm * (pm0.x * u.x - pm0.y * u.y);
+                        gv = bm * (pm0.x * u.y + pm0.y * u.x);
+                    } else if (m == l) {
+                        const double2 u = Rl(R, l1, m - 1, m1);
+                        fv = bp * (pp0.x * u.x - pp0.y * u.y);
+                        gv = bp * (pp0.x * u.y + pp0.y * u.x);
                     } else {
-                        R[ridx(l, m, m1)] = make_double2(fv, gv);
-                        R[ridx(l, -m, -m1)] = make_double2(sg * fv, -sg * gv);
+                        const double2 u0 = Rl(R, l1, m, m1), um = Rl(R, l1, m - 1, m1), up = Rl(R, l1, m + 1, m1);
+                        fv = am * (p00.x * u0.x) + bp * (pp0.x * um.x - pp0.y * um.y) + bm * (pm0.x * up.x - pm0.y * up.y);
+                        gv = am * (p00.x * u0.y) + bp * (pp0.x * um.y + pp0.y * um.x) + bm * (pm0.x * up.y + pm0.y * up.x);
+                    }
+                } else {  // m1 == l
+                    const double cm = s2 * sqrtof(l + m) * sqrtof(l - m) / (sqrtof(l + l) * sqrtof(l + l - 1));
+                    const double dp = sqrtof(l + m) * sqrtof(l + m - 1) / (sqrtof(l + l) * sqrtof(l + l - 1));
+                    const double dm = sqrtof(l - m) * sqrtof(l - m - 1) / (sqrtof(l + l) * sqrtof(l + l - 1));
+                    if (m == -l) {
+                        const double2 u = Rl(R, l1, m + 1, m1 - 1);
+                        fv = dm * (pm1.x * u.x - pm1.y * u.y);
+                        gv = dm * (pm1.x * u.y + pm1.y * u.x);
+                    } else if (m == l) {
+                        const double2 u = Rl(R, l1, m - 1, m1 - 1);
+                        fv = dp * (pp1.x * u.x - pp1.y * u.y);
+                        gv = dp * (pp1.x * u.y + pp1.y * u.x);
+                    } else {
+                        const double2 u0 = Rl(R, l1, m, m1 - 1), um = Rl(R, l1, m - 1, m1 - 1), up = Rl(R, l1, m + 1, m1 - 1);
+                        fv = cm * (p01.x * u0.x - p01.y * u0.y) + dp * (pp1.x * um.x - pp1.y * um.y) + dm * (pm1.x * up.x - pm1.y * up.y);
+                        gv = cm * (p01.x * u0.y + p01.y * u0.x) + dp * (pp1.x * um.y + pp1.y * um.x) + dm * (pm1.x * up.y + pm1.y * up.x);
                     }
                 }
+                const double sg = ((m + m1) & 1) ? -1.0 : 1.0;
+                if (m == 0 && m1 == 0) {
+                    R[ridx(l, 0, 0)] = make_double2(fv, -gv);  // the mirror statement negates g in place
+                } else {
+                    R[ridx(l, m, m1)] = make_double2(fv, gv);
+                    R[ridx(l, -m, -m1)] = make_double2(sg * fv, -sg * gv);
+                }
             }
-            __syncthreads();
+            wave_sync();
         }
-        if (active)
-            for (int i = lane; i < nR; i += 64) R[i] = chop_R(R[i]);
-        __syncthreads();
+        for (int i = lane; i < nR; i += 64) R[i] = chop_R(R[i]);
+        wave_sync();
         // ---- rotation to the q frame (:505-523)
-        if (active) {
-            for (int j = lane; j < 2 * np; j += 64) {
-                const int ip = j < np ? j : j - np;
-                const int m = a.pm[ip], khi = a.pmup[ip], mu2 = a.pmu[ip] / a.mrso;
-                double2 s = make_double2(0.0, 0.0);
-                if (j < np) {
-                    for (int mup = -m; mup <= m; mup++) {
-                        const double2 v = cmul(X0[P(a, m, mup, mu2)], R[ridx(m, mup, khi)]);
-                        s.x += v.x;
-                        s.y += v.y;
-                    }
-                    A[ip] = s;
-                } else {
-                    for (int mup = -m; mup <= m; mup++) {
-                        const double2 v = cmul(X1[P(a, m, mup, mu2)], R[ridx(m, mup, -khi)]);
-                        s.x += v.x;
-                        s.y += v.y;
-                    }
-                    const double sg = (m & 1) ? -1.0 : 1.0;
-                    B[ip] = make_double2(s.x * sg, s.y * sg);
-                }
+        for (int j = lane; j < 2 * np; j += 64) {
+            const bool first = j < np;
+            const int ip = first ? j : j - np;
+            const int m = tm[ip], khi = tmup[ip], mu2 = tmu2[ip];
+            const double2 *X = first ? X0 : X1;
+            const int kk = first ? khi : -khi;
+            const int base = PIDX(m, -m, mu2), step = m / mrso + 1;
+            const double2 *Rrow = R + ridx(m, -m, kk);
+            const int rstep = 2 * m + 1;
+            double sx = 0.0, sy = 0.0;
+            for (int t = 0; t <= 2 * m; t++) {
+                const double2 v = cmul(X[base + t * step], Rrow[t * rstep]);
+                sx += v.x;
+                sy += v.y;
+            }
+            if (first) A[ip] = make_double2(sx, sy);
+            else {
+                const double sg = (m & 1) ? -1.0 : 1.0;
+                B[ip] = make_double2(sx * sg, sy * sg);
             }
         }
-        __syncthreads();
+        wave_sync();
         // ---- molecular OZ with interpolated c (:534-606)
-        if (active) {
-            for (int j = lane; j < 2 * np; j += 64) {
-                const int ip = j < np ? j : j - np;
-                const int khi = a.pmup[ip];
-                const double2 *src_neg = (j < np) ? A : B;  // nu2 < 0 term
-                const double2 *src_pos = (j < np) ? B : A;  // conj term
-                int ia = a.ia0[ip];
-                double2 s = make_double2(0.0, 0.0);
-                for (int n = abs(khi); n <= L; n++) {
-                    for (int nu2 = -n / a.mrso; nu2 <= n / a.mrso; nu2++, ia++) {
-                        const double2 lo = c0[ia], hi = c1[ia];
-                        const double2 ce = make_double2(alpha * hi.x + (1.0 - alpha) * lo.x,
-                                                        alpha * hi.y + (1.0 - alpha) * lo.y);
-                        double2 v;
-                        if (nu2 < 0) {
-                            v = cmul(ce, src_neg[P(a, n, khi, -nu2)]);
-                        } else {
-                            const double2 x = src_pos[P(a, n, khi, nu2)];
-                            v = cmul(ce, make_double2(x.x, -x.y));
-                        }
-                        s.x += v.x;
-                        s.y += v.y;
+        for (int j = lane; j < 2 * np; j += 64) {
+            const bool first = j < np;
+            const int ip = first ? j : j - np;
+            const int khi = tmup[ip];
+            const double2 *src_neg = first ? A : B;  // nu2 < 0 term
+            const double2 *src_pos = first ? B : A;  // conj term
+            int ia = tia0[ip];
+            double sx = 0.0, sy = 0.0;
+            for (int n = abs(khi); n <= L; n++) {
+                const int h = n / mrso;
+                const int base = PIDX(n, khi, 0);
+                for (int nu2 = -h; nu2 <= h; nu2++, ia++) {
+                    const double2 lo = c0[ia], hi = c1[ia];
+                    const double2 ce = make_double2(alpha * hi.x + (1.0 - alpha) * lo.x,
+                                                    alpha * hi.y + (1.0 - alpha) * lo.y);
+                    double2 v;
+                    if (nu2 < 0) {
+                        v = cmul(ce, src_neg[base - nu2]);
+                    } else {
+                        const double2 x = src_pos[base + nu2];
+                        v = cmul(ce, make_double2(x.x, -x.y));
                     }
+                    sx += v.x;
+                    sy += v.y;
                 }
-                if (j < np) GQ[ip] = chop_gamma(s);
-                else GMQ[ip] = chop_gamma(s);
             }
+            if (first) GQ[ip] = chop_gamma(make_double2(sx, sy));
+            else GMQ[ip] = chop_gamma(make_double2(sx, sy));
         }
-        __syncthreads();
+        wave_sync();
         // ---- rotation back with conj(R) (:611-641) and store (:648-656)
-        if (active) {
-            const bool eq = flag & 1, nyq = flag & 2;
-            for (int j = lane; j < 2 * np; j += 64) {
-                const int ip = j < np ? j : j - np;
-                const int m = a.pm[ip], mup = a.pmup[ip], mu2 = a.pmu[ip] / a.mrso;
-                double2 s = make_double2(0.0, 0.0);
-                if (j < np) {
-                    if (eq) continue;  // for q == -q the -q store overwrites this one
-                    for (int khi = -m; khi <= m; khi++) {
-                        const double2 v = cmulc(GQ[P(a, m, khi, mu2)], R[ridx(m, mup, khi)]);
-                        s.x += v.x;
-                        s.y += v.y;
-                    }
-                    proj[(size_t)gq * np + ip] = s;
-                } else {
-                    for (int khi = -m; khi <= m; khi++) {
-                        const double2 v = cmulc(GMQ[P(a, m, khi, mu2)], R[ridx(m, mup, -khi)]);
-                        s.x += v.x;
-                        s.y += v.y;
-                    }
-                    const double sg = (m & 1) ? -1.0 : 1.0;
-                    s.x *= sg;
-                    s.y *= sg;
-                    if (eq && nyq) s.y = -s.y;
-                    proj[(size_t)gmq * np + ip] = s;
-                }
+        const bool eq = flag & 1, nyq = flag & 2;
+        for (int j = lane; j < 2 * np; j += 64) {
+            const bool first = j < np;
+            if (first && eq) continue;  // for q == -q the -q store overwrites this one
+            const int ip = first ? j : j - np;
+            const int m = tm[ip], mup = tmup[ip], mu2 = tmu2[ip];
+            const double2 *G = first ? GQ : GMQ;
+            const int base = PIDX(m, -m, mu2), step = m / mrso + 1;
+            const double2 *Rrow = R + ridx(m, mup, 0);
+            double sx = 0.0, sy = 0.0;
+            for (int t = 0; t <= 2 * m; t++) {
+                const int khi = t - m;
+                const double2 r = Rrow[first ? khi : -khi];
+                const double2 v = cmulc(G[base + t * step], r);
+                sx += v.x;
+                sy += v.y;
+            }
+            if (first) {
+                proj[(size_t)gq * np + ip] = make_double2(sx, sy);
+            } else {
+                const double sg = (m & 1) ? -1.0 : 1.0;
+                sx *= sg;
+                sy *= sg;
+                if (eq && nyq) sy = -sy;
+                proj[(size_t)gmq * np + ip] = make_double2(sx, sy);
             }
         }
-        __syncthreads();
+        wave_sync();
     }
+#undef PIDX
 }
 
 }  // namespace
@@ -326,10 +334,13 @@ hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) {
     a.ck = c->d_ck;
     const int nR = (c->L + 1) * (4 * (c->L + 1) * (c->L + 1) - 1) / 3;
     auto lds_for = [&](int waves) {
-        return sizeof(double2) * (2 * (size_t)c->npc + (size_t)waves * (nR + 6 * (size_t)c->np));
+        return sizeof(double2) * (2 * (size_t)c->npc + (size_t)waves * (nR + 6 * (size_t)c->np)) +
+               sizeof(int) * (4 * (size_t)c->np + MDFT_MAX_MMAX + 2);
     };
     const dim3 grid((unsigned)c->pairs.nchunks);
-    if (lds_for(4) <= c->lds_limit)
+    if (lds_for(8) <= c->lds_limit)
+        hipLaunchKernelGGL(oz_f64<8>, grid, dim3(512), lds_for(8), c->stream, proj, a);
+    else if (lds_for(4) <= c->lds_limit)
         hipLaunchKernelGGL(oz_f64<4>, grid, dim3(256), lds_for(4), c->stream, proj, a);
     else if (lds_for(2) <= c->lds_limit)
         hipLaunchKernelGGL(oz_f64<2>, grid, dim3(128), lds_for(2), c->stream, proj, a);

@@ -201,7 +201,8 @@ def _lbasis_orbits(mmax: int, mrso: int):
 
 
 def synthetic_ck(mmax: int, nk: int, dk: float, seed: int, n0: float, mrso: int = 2,
-                 bound: float = 0.8, a_range=(-2.0, 2.0), s_range=(0.5, 1.5), r_range=(2.0, 4.0)) -> CkTable:
+                 bound: float = 0.8, a_range=(-2.0, 2.0), s_range=(0.5, 1.5), r_range=(2.0, 4.0),
+                 self_term: float = 1.0) -> CkTable:
     """Seeded synthetic c(k) table in the reference file order.
 
     Each independent real l-basis component is n0*A*exp(-k^2 s^2/2)*j_l(kR)
@@ -242,7 +243,21 @@ def synthetic_ck(mmax: int, nk: int, dk: float, seed: int, n0: float, mrso: int 
     norm = spectral_norm_max(t, mmax, mrso)
     if norm > bound:
         t.c *= bound / norm
+    if self_term:
+        add_self_term(t, -self_term)
     return t
+
+
+def add_self_term(t: CkTable, lam: float) -> None:
+    """Adds lam * I to the table, where I is the identity operator of the
+    excess functional in the reference's conventions:
+    c(m, n, mu, nu; khi) = delta_mn delta_{mu,-nu} (-1)^khi.  With lam = -1 the
+    excess term gains the local penalty +(kT/2) ||drho||^2 / rho0, which
+    exactly matches the curvature of the ideal term at rho0; it keeps the
+    functional convex at any density, including next to charged sites."""
+    for i in range(t.np_file):
+        if t.m[i] == t.n[i] and t.mu[i] == -t.nu[i]:
+            t.c[:, i] += lam * (-1.0) ** int(t.khi[i])
 
 
 def khi_blocks(t: CkTable, mmax: int, mrso: int = 2):

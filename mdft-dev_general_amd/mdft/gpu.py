@@ -26,7 +26,8 @@ MDFT_EXPORTS = (
     "mdft_ctx_create", "mdft_ctx_destroy", "mdft_last_error", "mdft_get_info", "mdft_get_quadrature",
     "mdft_set_vext", "mdft_set_ck", "mdft_energy_and_gradient", "mdft_excess_energy",
     "mdft_ideal_external", "mdft_excess_stage", "mdft_phase_times", "mdft_device_alloc",
-    "mdft_device_free", "mdft_memcpy_h2d", "mdft_memcpy_d2h", "mdft_synchronize",
+    "mdft_device_free", "mdft_memcpy_h2d", "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext",
+    "mdft_get_vext", "mdft_vec_dot", "mdft_vec_axpy_dot", "mdft_vec_lincomb",
 )
 
 
@@ -77,10 +78,16 @@ def load_library(path: str = LIBPATH):
     lib.mdft_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
     lib.mdft_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
     lib.mdft_synchronize.argtypes = [vp]
+    lib.mdft_compute_vext.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double]
+    lib.mdft_get_vext.argtypes = [vp, vp, C.c_int]
+    lib.mdft_vec_dot.argtypes = [vp, C.c_size_t, vp, vp, _dp]
+    lib.mdft_vec_axpy_dot.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp, _dp]
+    lib.mdft_vec_lincomb.argtypes = [vp, C.c_size_t, C.c_double, vp, C.c_double, vp, vp]
     for name in ("mdft_get_info", "mdft_get_quadrature", "mdft_set_vext", "mdft_set_ck",
                  "mdft_energy_and_gradient", "mdft_excess_energy", "mdft_ideal_external", "mdft_excess_stage",
                  "mdft_phase_times", "mdft_device_alloc", "mdft_device_free", "mdft_memcpy_h2d",
-                 "mdft_memcpy_d2h", "mdft_synchronize"):
+                 "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext", "mdft_get_vext", "mdft_vec_dot",
+                 "mdft_vec_axpy_dot", "mdft_vec_lincomb"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib
@@ -181,6 +188,23 @@ class Engine:
         self._check(self.lib.mdft_set_ck(self._h, len(arr[0]), *[a.ctypes.data_as(_ip) for a in arr],
                                          c.shape[0], float(table.dq), c.view(np.float64).ctypes.data_as(_dp)))
         self._refresh()
+
+    def compute_vext(self, solute, solvent, rcut: float = 10.0):
+        """vext of `solute` (mdft.workloads.Solute) for `solvent` (mdft.constants.Solvent), on the device."""
+        atoms = np.ascontiguousarray(np.column_stack([solute.xyz, solute.q, solute.sig, solute.eps]), dtype=np.float64)
+        sites = np.ascontiguousarray([[*s.r, s.q, s.sig, s.eps] for s in solvent.sites], dtype=np.float64)
+        self._check(self.lib.mdft_compute_vext(self._h, len(atoms), atoms.ctypes.data_as(_dp), len(sites),
+                                               sites.ctypes.data_as(_dp), float(rcut)))
+
+    def get_vext(self):
+        out = np.empty(self.N * self.no)
+        self._check(self.lib.mdft_get_vext(self._h, out.ctypes.data, 0))
+        return out
+
+    def dot(self, x: "DeviceBuffer", y: "DeviceBuffer") -> float:
+        out = np.zeros(1)
+        self._check(self.lib.mdft_vec_dot(self._h, x.nbytes // 8, x.ptr, y.ptr, _p(out)))
+        return float(out[0])
 
     def alloc_field(self) -> DeviceBuffer:
         return DeviceBuffer(self, self.N * self.no * 8)

@@ -1,0 +1,78 @@
+"""Device-resident L-BFGS minimisation through the Fortran host library.
+
+`minimize` calls mdft_minimize (fortran/mdft_lbfgs.f90, bound with
+ISO_C_BINDING) on an Engine's context: xi and the gradient stay on the device,
+the two-loop recursion and line search scalars are FP64 on the host.  This
+replaces the reference's minimisation loop (src/energy_minimization.f90:232-290).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .gpu import PKG, DeviceBuffer, Engine, MdftError
+
+HOSTLIB = os.path.join(PKG, "lib", "libmdft_host.so")
+_HOST = None
+
+STATUS = {0: "converged (gradient)", 1: "converged (F change)", 2: "evaluation budget", 3: "line search failed",
+          4: "F within the target band"}
+
+
+def host_library():
+    global _HOST
+    if _HOST is None:
+        if not os.path.exists(HOSTLIB):
+            raise FileNotFoundError(f"{HOSTLIB} is not built (run __graft_entry__.build())")
+        lib = C.CDLL(HOSTLIB)
+        dp = C.POINTER(C.c_double)
+        lib.mdft_minimize.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int]
+        lib.mdft_minimize.restype = C.c_int
+        _HOST = lib
+    return _HOST
+
+
+@dataclass
+class MinResult:
+    F: float
+    parts: np.ndarray
+    grad_ratio: float
+    evaluations: int
+    iterations: int
+    status: int
+    g0: float
+    trace: np.ndarray
+
+    @property
+    def converged(self) -> bool:
+        return self.status in (0, 1, 4)
+
+
+def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: int = 5, gtol: float = 1e-6,
+             ftol: float = 0.0, fnoise: float = 1e-7, maxeval: int = 2000, ntrace: int = 2000,
+             ftarget: float = 0.0, fband: float = 0.0, lib=None) -> MinResult:
+    """Minimise F over xi (in place).  Returns the final F and statistics.
+    fnoise is the relative noise level of F used to relax the sufficient-decrease
+    test; with fband > 0 the run stops at the first iterate with |F - ftarget| <= fband."""
+    lib = lib or host_library()
+    own = g is None
+    if own:
+        g = eng.alloc_field()
+    stats = np.zeros(16)
+    trace = np.zeros((max(ntrace, 1), 3))
+    rc = lib.mdft_minimize(eng._h, eng.N * eng.no, xi.ptr, g.ptr, int(m), float(gtol), float(ftol), float(fnoise),
+                           float(ftarget), float(fband), int(maxeval),
+                           stats.ctypes.data_as(C.POINTER(C.c_double)),
+                           trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace))
+    if rc != 0:
+        raise MdftError(rc, eng.lib.mdft_last_error(eng._h).decode())
+    it = int(stats[6])
+    res = MinResult(float(stats[0]), stats[1:4].copy(), float(stats[4]), int(stats[5]), it, int(stats[7]),
+                    float(stats[8]), trace[:min(it, ntrace)].copy())
+    if own:
+        g.free()
+    return res

@@ -171,6 +171,13 @@ def vext_numpy(w: Workload, solute: Solute, rcut: float = 10.0, clip: float | No
     return np.minimum(v, cap)
 
 
+def guess_xi(vext: np.ndarray, kT: float) -> np.ndarray:
+    """Cold start of the reference (src/module_density.f90:267-284): xi = 0 where
+    vext >= (-ln(eps_dp) - 0.05) kT, xi = 1 elsewhere."""
+    vmax = (-np.log(np.finfo(np.float64).eps) - 0.05) * kT
+    return np.where(vext >= vmax, 0.0, 1.0)
+
+
 def boltzmann_xi(vext: np.ndarray, kT: float) -> np.ndarray:
     """Cold-start guess xi = exp(-beta vext / 2) (rho = rho0 exp(-beta vext))."""
     return np.exp(-0.5 * np.minimum(vext, 50.0 * kT) / kT)
@@ -181,5 +188,5 @@ def perturbed_xi(vext: np.ndarray, kT: float, seed: int, amp: float = 0.05) -> n
     return boltzmann_xi(vext, kT) * (1.0 + amp * rng.standard_normal(vext.shape))
 
 
-__all__ = ["Workload", "WORKLOADS", "Solute", "make_solute", "vext_numpy", "boltzmann_xi", "perturbed_xi", "small",
+__all__ = ["Workload", "WORKLOADS", "Solute", "make_solute", "vext_numpy", "guess_xi", "boltzmann_xi", "perturbed_xi", "small",
            "fp32"]

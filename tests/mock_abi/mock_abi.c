@@ -1,0 +1,134 @@
+/*
+ * mock_abi.c -- TEST INFRASTRUCTURE: a CPU implementation of the subset of the
+ * C-ABI (include/mdft_gpu.h) that the Fortran host library calls, backed by the
+ * FP64 oracle.  Linking the Fortran L-BFGS (mdft-dev_general_amd/fortran)
+ * against it lets the host logic run and be checked without a GPU.  "Device"
+ * pointers are plain host memory here.
+ */
+#include <complex.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mdft_gpu.h"
+#include "mdft_oracle.h"
+
+struct mdft_ctx {
+    or_ctx *o;
+    double *vext;
+    size_t nvar;
+    int no, np;
+    char err[256];
+};
+
+static char g_err[256] = "";
+
+int mdft_ctx_create(const mdft_desc *d, mdft_ctx **out)
+{
+    or_ctx *o = or_create(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->mmax, d->mrso, d->kT, d->n0);
+    if (!o) { strcpy(g_err, "bad description"); return MDFT_EINVAL; }
+    mdft_ctx *c = calloc(1, sizeof(mdft_ctx));
+    int s[7];
+    or_sizes(o, s);
+    c->o = o;
+    c->no = s[3];
+    c->np = s[4];
+    c->nvar = (size_t)d->nx * d->ny * d->nz * s[3];
+    *out = c;
+    return 0;
+}
+
+void mdft_ctx_destroy(mdft_ctx *c)
+{
+    if (!c) return;
+    or_destroy(c->o);
+    free(c->vext);
+    free(c);
+}
+
+const char *mdft_last_error(const mdft_ctx *c) { return c ? c->err : g_err; }
+
+int mdft_get_info(const mdft_ctx *c, int *info)
+{
+    int s[7];
+    or_sizes(c->o, s);
+    for (int i = 0; i < 7; i++) info[i] = s[i];
+    info[7] = (int)(c->nvar / (size_t)c->no);
+    info[8] = 0;
+    info[9] = 2;
+    return 0;
+}
+
+int mdft_set_ck(mdft_ctx *c, int np_file, const int *m, const int *n, const int *mu, const int *nu, const int *khi,
+                int nq_file, double dq, const double *cf)
+{
+    return or_set_ck(c->o, np_file, m, n, mu, nu, khi, nq_file, dq, (const double complex *)cf) ? MDFT_ERANGE : 0;
+}
+
+int mdft_compute_vext(mdft_ctx *c, int natom, const double *atoms, int nsite, const double *sites, double rcut)
+{
+    (void)natom; (void)atoms; (void)nsite; (void)sites; (void)rcut;
+    strcpy(c->err, "compute_vext is not available in the CPU mock");
+    return MDFT_EINVAL;
+}
+
+int mdft_set_vext(mdft_ctx *c, const double *vext, int flags)
+{
+    (void)flags;
+    free(c->vext);
+    c->vext = malloc(sizeof(double) * c->nvar);
+    memcpy(c->vext, vext, sizeof(double) * c->nvar);
+    return 0;
+}
+
+int mdft_energy_and_gradient(mdft_ctx *c, const double *xi, double *parts, double *df, int flags)
+{
+    (void)flags;
+    or_energy_and_gradient(c->o, xi, c->vext, parts, df);
+    return 0;
+}
+
+int mdft_excess_energy(mdft_ctx *c, const double *xi, double *fexc, double *df, int flags)
+{
+    (void)flags;
+    or_excess(c->o, xi, fexc, df);
+    return 0;
+}
+
+int mdft_device_alloc(mdft_ctx *c, size_t bytes, void **p)
+{
+    (void)c;
+    *p = aligned_alloc(256, (bytes + 255) / 256 * 256);
+    return *p ? 0 : MDFT_ENOMEM;
+}
+
+int mdft_device_free(mdft_ctx *c, void *p) { (void)c; free(p); return 0; }
+int mdft_memcpy_h2d(mdft_ctx *c, void *d, const void *s, size_t b) { (void)c; memcpy(d, s, b); return 0; }
+int mdft_memcpy_d2h(mdft_ctx *c, void *d, const void *s, size_t b) { (void)c; memcpy(d, s, b); return 0; }
+
+int mdft_vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out)
+{
+    (void)c;
+    double s = 0.0;
+    for (size_t i = 0; i < n; i++) s += x[i] * y[i];
+    *out = s;
+    return 0;
+}
+
+int mdft_vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out)
+{
+    (void)c;
+    double s = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        y[i] += a * x[i];
+        if (z) s += z[i] * y[i];
+    }
+    if (z && out) *out = s;
+    return 0;
+}
+
+int mdft_vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z)
+{
+    (void)c;
+    for (size_t i = 0; i < n; i++) z[i] = a * x[i] + (y ? b * y[i] : 0.0);
+    return 0;
+}

@@ -178,9 +178,9 @@ def test_angular_projector(oracle_mod, mmax):
 
 
 # ---------------------------------------------------------------- excess functional
-def _setup(oracle_mod, mmax, n=8, length=10.0, seed=4):
+def _setup(oracle_mod, mmax, n=8, length=10.0, seed=4, self_term=1.0):
     o = oracle_mod.Oracle((n, n, n), (length,) * 3, mmax, 2, 2.4777, 0.0333)
-    t = ckmod.synthetic_ck(mmax, 256, 0.05, seed=seed, n0=0.0333)
+    t = ckmod.synthetic_ck(mmax, 256, 0.05, seed=seed, n0=0.0333, self_term=self_term)
     o.set_ck(t)
     return o, t
 
@@ -247,11 +247,11 @@ def test_excess_operator_symmetry_and_nyquist_quirk(oracle_mod):
     the symmetry for mmax >= 1.  Odd grids and Nyquist-free fields are
     symmetric to the single-precision chop level."""
     rng = np.random.default_rng(1)
-    o5, _ = _setup(oracle_mod, 2, n=5)
+    o5, _ = _setup(oracle_mod, 2, n=5, self_term=0.0)
     a = 0.2 * o5.rho0 * rng.uniform(-1, 1, o5.N * o5.no)
     b = 0.2 * o5.rho0 * rng.uniform(-1, 1, o5.N * o5.no)
     assert abs(_asym(o5, a, b)) < 1e-6
-    o6, _ = _setup(oracle_mod, 2, n=6)
+    o6, _ = _setup(oracle_mod, 2, n=6, self_term=0.0)
     a = 0.2 * o6.rho0 * rng.uniform(-1, 1, o6.N * o6.no)
     b = 0.2 * o6.rho0 * rng.uniform(-1, 1, o6.N * o6.no)
     assert abs(_asym(o6, _no_nyquist(o6, a, 6), _no_nyquist(o6, b, 6))) < 1e-6
