@@ -210,6 +210,10 @@ int ready(mdft_ctx *c) {
     return MDFT_OK;
 }
 
+size_t proj_elem_bytes(const mdft_ctx *c) {
+    return c->d.precision == MDFT_PREC_FP32 ? sizeof(float2) : sizeof(double2);
+}
+
 hipError_t ensure_stage(mdft_ctx *c, double **buf) {
     if (*buf) return hipSuccess;
     return hipMalloc((void **)buf, sizeof(double) * c->N * c->no);
@@ -217,14 +221,17 @@ hipError_t ensure_stage(mdft_ctx *c, double **buf) {
 
 // Runs the excess-term projection pipeline up to `stage` (1..4) on device xi.
 int run_excess_pipeline(mdft_ctx *c, const double *xi, int stage, bool timers) {
+    const bool f32 = c->d.precision == MDFT_PREC_FP32;
+    double2 *p64 = static_cast<double2 *>(c->d_proj);
+    float2 *p32 = static_cast<float2 *>(c->d_proj);
     if (timers) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, launch_angl2proj_f64(c, xi, c->d_proj));
+    HIPCHK(c, f32 ? launch_angl2proj_f32(c, xi, p32) : launch_angl2proj_f64(c, xi, p64));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    if (stage >= 2) HIPCHK(c, launch_fft3d_f64(c, c->d_proj, +1));
+    if (stage >= 2) HIPCHK(c, f32 ? launch_fft3d_f32(c, p32, +1) : launch_fft3d_f64(c, p64, +1));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    if (stage >= 3) HIPCHK(c, launch_oz_f64(c, c->d_proj));
+    if (stage >= 3) HIPCHK(c, f32 ? launch_oz_f32(c, p32) : launch_oz_f64(c, p64));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    if (stage >= 4) HIPCHK(c, launch_fft3d_f64(c, c->d_proj, -1));
+    if (stage >= 4) HIPCHK(c, f32 ? launch_fft3d_f32(c, p32, -1) : launch_fft3d_f64(c, p64, -1));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     return MDFT_OK;
 }
@@ -268,8 +275,12 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
         g_create_error = "invalid grid/solvent description";
         return MDFT_EINVAL;
     }
-    if (d.precision != MDFT_PREC_FP64) {
-        g_create_error = "only MDFT_PREC_FP64 is available in this build";
+    if (d.precision != MDFT_PREC_FP64 && d.precision != MDFT_PREC_FP32) {
+        g_create_error = "precision must be MDFT_PREC_FP64 or MDFT_PREC_FP32 (MDFT_PREC_BF16 is not available)";
+        return MDFT_EINVAL;
+    }
+    if (d.precision == MDFT_PREC_FP32 && (d.mrso != 2 || d.mmax > 5)) {
+        g_create_error = "MDFT_PREC_FP32 needs molecular symmetry order 2 and mmax <= 5";
         return MDFT_EINVAL;
     }
     if ((long long)d.nx * d.ny * d.nz >= (1LL << 31)) {
@@ -400,7 +411,7 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
               upload(c, &c->d_kx, c->kx.data(), c->kx.size()) == hipSuccess &&
               upload(c, &c->d_ky, c->ky.data(), c->ky.size()) == hipSuccess &&
               upload(c, &c->d_kz, c->kz.data(), c->kz.size()) == hipSuccess;
-    ok = ok && hipMalloc(&c->d_proj, sizeof(double2) * c->N * np) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_proj, proj_elem_bytes(c) * c->N * np) == hipSuccess;
     ok = ok && hipMalloc(&c->d_vext, sizeof(double) * c->N * c->no) == hipSuccess;
     ok = ok && hipMalloc(&c->d_sums, sizeof(double) * 8) == hipSuccess;
     ok = ok && hipHostMalloc(&c->h_sums, sizeof(double) * 8, hipHostMallocDefault) == hipSuccess;
@@ -554,9 +565,23 @@ int mdft_excess_stage(mdft_ctx *c, const double *xi, int stage, double *out, int
     }
     rc = run_excess_pipeline(c, xd, stage, false);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(out, c->d_proj, sizeof(double2) * c->N * c->np,
-                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t ne = c->N * c->np;
+    if (c->d.precision == MDFT_PREC_FP64) {
+        HIPCHK(c, hipMemcpyAsync(out, c->d_proj, sizeof(double2) * ne,
+                                 dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else if (dev) {
+        HIPCHK(c, launch_widen_f32(c, static_cast<const float2 *>(c->d_proj), reinterpret_cast<double2 *>(out), ne));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float2> tmp(ne);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), c->d_proj, sizeof(float2) * ne, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < ne; i++) {
+            out[2 * i] = tmp[i].x;
+            out[2 * i + 1] = tmp[i].y;
+        }
+    }
     return MDFT_OK;
 }
 
@@ -585,7 +610,10 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     rc = run_excess_pipeline(c, xd, 4, timers);
     if (rc) return rc;
     int nb = 0;
-    HIPCHK(c, launch_final_f64(c, c->d_proj, xd, dd, mode, &nb));
+    if (c->d.precision == MDFT_PREC_FP32)
+        HIPCHK(c, launch_final_f32(c, static_cast<const float2 *>(c->d_proj), xd, dd, mode, &nb));
+    else
+        HIPCHK(c, launch_final_f64(c, static_cast<const double2 *>(c->d_proj), xd, dd, mode, &nb));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     rc = read_sums(c, nb, sums);
     if (rc) return rc;

@@ -25,6 +25,11 @@ __device__ inline double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+template <typename T> struct C2;
+template <> struct C2<double> { using t = double2; };
+template <> struct C2<float> { using t = float2; };
+template <typename V> __device__ inline V mk(decltype(V::x) a, decltype(V::x) b) { V v; v.x = a; v.y = b; return v; }
+
 template <int PT>
 __global__ __launch_bounds__(kThreads) void angl2proj_f64(const double *__restrict__ xi, double2 *__restrict__ proj,
                                                          size_t N, AngTables t, double rho0) {
@@ -286,98 +291,101 @@ struct Dims {
     static constexpr int NP = np_of(L);
 };
 
-template <int L, int PT>
-__global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, double2 *__restrict__ proj,
+template <int L, int PT, typename T>
+__global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
                                                        size_t N, AngTables t, double rho0) {
     using D = Dims<L>;
+    using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
-    __shared__ double o[PT * NO];
-    __shared__ double2 G[PT * NTH * NPHI * H1];
-    __shared__ double2 F[PT * NTH * W * H1];
-    __shared__ double cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], a2p[NP * NTH], fmv[NP];
+    __shared__ T o[PT * NO];
+    __shared__ V G[PT * NTH * NPHI * H1];
+    __shared__ V F[PT * NTH * W * H1];
+    __shared__ T cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], a2p[NP * NTH], fmv[NP];
     __shared__ int pmup[NP], pmu2[NP];
     const int tid = threadIdx.x;
-    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = t.cph[i]; sph[i] = t.sph[i]; }
-    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = t.cps[i]; sps[i] = t.sps[i]; }
-    for (int i = tid; i < NP * NTH; i += kThreads) a2p[i] = t.a2p[i];
-    for (int i = tid; i < NP; i += kThreads) { fmv[i] = t.fm[i]; pmup[i] = t.pmup[i]; pmu2[i] = t.pmu[i] / 2; }
+    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = (T)t.cph[i]; sph[i] = (T)t.sph[i]; }
+    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = (T)t.cps[i]; sps[i] = (T)t.sps[i]; }
+    for (int i = tid; i < NP * NTH; i += kThreads) a2p[i] = (T)t.a2p[i];
+    for (int i = tid; i < NP; i += kThreads) { fmv[i] = (T)t.fm[i]; pmup[i] = t.pmup[i]; pmu2[i] = t.pmu[i] / 2; }
     const size_t r0 = (size_t)blockIdx.x * PT;
     const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
     const double *src = xi + r0 * NO;
     for (int i = tid; i < npts * NO; i += kThreads) {
         const double x = src[i];
-        o[i] = rho0 * (x * x - 1.0);
+        o[i] = (T)(rho0 * (x * x - 1.0));
     }
     __syncthreads();
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
         const int mu2 = i % H1;
-        const double *line = o + (i / H1) * NPSI;
-        double re = 0.0, im = 0.0;
+        const T *line = o + (i / H1) * NPSI;
+        T re = 0, im = 0;
 #pragma unroll
         for (int ipsi = 0; ipsi < NPSI; ipsi++) {
             const int k = (mu2 * ipsi) % NPSI;
-            const double v = line[ipsi];
+            const T v = line[ipsi];
             re += v * cps[k];
             im += v * sps[k];
         }
-        G[i] = make_double2(re, im);
+        G[i] = mk<V>(re, im);
     }
     __syncthreads();
-    const double norm = 1.0 / (double)(NPHI * NPSI);
+    const T norm = (T)(1.0 / (double)(NPHI * NPSI));
     for (int i = tid; i < npts * NTH * W * H1; i += kThreads) {
         const int mu2 = i % H1;
         const int rest = i / H1;
         const int mup = rest % W - L;
-        const double2 *g = G + (rest / W) * NPHI * H1 + mu2;
-        double re = 0.0, im = 0.0;
+        const V *g = G + (rest / W) * NPHI * H1 + mu2;
+        T re = 0, im = 0;
 #pragma unroll
         for (int iphi = 0; iphi < NPHI; iphi++) {
             int k = (mup * iphi) % NPHI;
             k += (k < 0) ? NPHI : 0;
-            const double2 v = g[iphi * H1];
+            const V v = g[iphi * H1];
             re += v.x * cph[k] - v.y * sph[k];
             im += v.x * sph[k] + v.y * cph[k];
         }
-        F[i] = make_double2(re * norm, im * norm);
+        F[i] = mk<V>(re * norm, im * norm);
     }
     __syncthreads();
-    double2 *dst = proj + r0 * NP;
+    V *dst = proj + r0 * NP;
     for (int i = tid; i < npts * NP; i += kThreads) {
         const int pt = i / NP, ip = i - pt * NP;
-        const double2 *f = F + (pt * NTH * W + (pmup[ip] + L)) * H1 + pmu2[ip];
-        const double *a = a2p + ip * NTH;
-        double re = 0.0, im = 0.0;
+        const V *f = F + (pt * NTH * W + (pmup[ip] + L)) * H1 + pmu2[ip];
+        const T *a = a2p + ip * NTH;
+        T re = 0, im = 0;
 #pragma unroll
         for (int it = 0; it < NTH; it++) {
-            const double2 v = f[it * W * H1];
+            const V v = f[it * W * H1];
             re += v.x * a[it];
             im += v.y * a[it];
         }
-        dst[i] = make_double2(re * fmv[ip], im * fmv[ip]);
+        dst[i] = mk<V>(re * fmv[ip], im * fmv[ip]);
     }
 }
 
-template <int L, int PT>
-__global__ __launch_bounds__(kThreads) void final_t(const double2 *__restrict__ proj, const double *__restrict__ xi,
+template <int L, int PT, typename T>
+__global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode) {
     using D = Dims<L>;
+    using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
-    __shared__ double2 Gm[PT * NP];
-    __shared__ double2 f[PT * NTH * W * H1];
-    __shared__ double2 h[PT * NTH * NPHI * H1];
-    __shared__ double cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], p2a[NP * NTH], wq[NO];
+    __shared__ V Gm[PT * NP];
+    __shared__ V f[PT * NTH * W * H1];
+    __shared__ V h[PT * NTH * NPHI * H1];
+    __shared__ T cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], p2a[NP * NTH];
+    __shared__ double wq[NO];
     __shared__ int pidx[(L + 1) * W * H1];
     const int tid = threadIdx.x;
     for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
-    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = t.cph[i]; sph[i] = t.sph[i]; }
-    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = t.cps[i]; sps[i] = t.sps[i]; }
-    for (int i = tid; i < NP * NTH; i += kThreads) p2a[i] = t.p2a[i];
+    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = (T)t.cph[i]; sph[i] = (T)t.sph[i]; }
+    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = (T)t.cps[i]; sps[i] = (T)t.sps[i]; }
+    for (int i = tid; i < NP * NTH; i += kThreads) p2a[i] = (T)t.p2a[i];
     for (int i = tid; i < NO; i += kThreads) wq[i] = t.w[i];
     const size_t r0 = (size_t)blockIdx.x * PT;
     const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
-    const double2 *src = proj + r0 * NP;
+    const V *src = proj + r0 * NP;
     for (int i = tid; i < npts * NP; i += kThreads) Gm[i] = src[i];
     __syncthreads();
     // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m)
@@ -389,15 +397,15 @@ __global__ __launch_bounds__(kThreads) void final_t(const double2 *__restrict__ 
         const int it = rest % NTH, pt = rest / NTH;
         const int a = mup < 0 ? -mup : mup, b = 2 * mu2;
         const int mlo = a > b ? a : b;
-        double re = 0.0, im = 0.0;
+        T re = 0, im = 0;
         for (int m = mlo; m <= L; m++) {
             const int p = pidx[(m * W + mup + L) * H1 + mu2];
-            const double2 v = Gm[pt * NP + p];
-            const double c = p2a[p * NTH + it];
+            const V v = Gm[pt * NP + p];
+            const T c = p2a[p * NTH + it];
             re += v.x * c;
             im += v.y * c;
         }
-        f[i] = make_double2(re, im);
+        f[i] = mk<V>(re, im);
     }
     __syncthreads();
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
@@ -405,17 +413,17 @@ __global__ __launch_bounds__(kThreads) void final_t(const double2 *__restrict__ 
         int rest = i / H1;
         const int iphi = rest % NPHI;
         rest /= NPHI;
-        const double2 *fr = f + rest * W * H1 + mu2;
-        double re = 0.0, im = 0.0;
+        const V *fr = f + rest * W * H1 + mu2;
+        T re = 0, im = 0;
 #pragma unroll
         for (int mup = -L; mup <= L; mup++) {
             int kk = (mup * iphi) % NPHI;
             kk += (kk < 0) ? NPHI : 0;
-            const double2 v = fr[(mup + L) * H1];
+            const V v = fr[(mup + L) * H1];
             re += v.x * cph[kk] + v.y * sph[kk];
             im += v.x * sph[kk] - v.y * cph[kk];
         }
-        h[i] = make_double2(re, im);
+        h[i] = mk<V>(re, im);
     }
     __syncthreads();
     double fid = 0.0, fext = 0.0, fexc = 0.0;
@@ -423,13 +431,15 @@ __global__ __launch_bounds__(kThreads) void final_t(const double2 *__restrict__ 
     for (int i = tid; i < npts * NO; i += kThreads) {
         const int pt = i / NO, io = i - pt * NO;
         const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
-        const double2 *hr = h + (pt * NTH * NPHI + line) * H1;
-        double v = hr[0].x;
+        const V *hr = h + (pt * NTH * NPHI + line) * H1;
+        T vt = hr[0].x;
 #pragma unroll
         for (int mu2 = 1; mu2 < H1; mu2++) {
             const int kk = (mu2 * ipsi) % NPSI;
-            v += 2.0 * (hr[mu2].x * cps[kk] - hr[mu2].y * sps[kk]);
+            vt += (T)2 * (hr[mu2].x * cps[kk] - hr[mu2].y * sps[kk]);
         }
+        // from here on FP64: free-energy reductions, rho ln rho and the gradient
+        const double v = (double)vt;
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
         const size_t g = (r0 + pt) * (size_t)NO + io;
@@ -477,24 +487,24 @@ hipError_t ensure_partials(mdft_ctx *c, size_t nblocks) {
 
 }  // namespace
 
-template <int PT>
-bool launch_a2p_t(mdft_ctx *c, const double *xi, double2 *proj) {
+template <int PT, typename T>
+bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj) {
     const size_t nb = (c->N + PT - 1) / PT;
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0); return true;
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0); return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;
     }
 }
 
-template <int PT>
-bool launch_final_t(mdft_ctx *c, const double2 *proj, const double *xi, double *df, int mode, FinalConsts k) {
+template <int PT, typename T>
+bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi, double *df, int mode, FinalConsts k) {
     const size_t nb = (c->N + PT - 1) / PT;
     switch (c->L) {
 #define FIN_CASE(LL) \
-    case LL: hipLaunchKernelGGL((final_t<LL, PT>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, c->N, c->ang, k, mode); return true;
+    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, c->N, c->ang, k, mode); return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
         default: return false;
@@ -504,7 +514,7 @@ bool launch_final_t(mdft_ctx *c, const double2 *proj, const double *xi, double *
 constexpr int kPTt = 4;  // points per workgroup of the specialised kernels
 
 hipError_t launch_angl2proj_f64(mdft_ctx *c, const double *xi, double2 *proj) {
-    if (c->d.mrso == 2 && launch_a2p_t<kPTt>(c, xi, proj)) return hipGetLastError();
+    if (c->d.mrso == 2 && launch_a2p_t<kPTt, double>(c, xi, proj)) return hipGetLastError();
     const size_t nb = (c->N + kPT - 1) / kPT;
     if (a2p_lds(c->ang) > c->lds_limit) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(angl2proj_f64<kPT>, dim3((unsigned)nb), dim3(kThreads), a2p_lds(c->ang), c->stream, xi, proj,
@@ -528,7 +538,7 @@ hipError_t launch_final_f64(mdft_ctx *c, const double2 *proj, const double *xi, 
     if (c->d.mrso == 2) {
         const size_t nbt = (c->N + kPTt - 1) / kPTt;
         if ((e = ensure_partials(c, nbt)) != hipSuccess) return e;
-        if (launch_final_t<kPTt>(c, proj, xi, df, mode, k)) {
+        if (launch_final_t<kPTt, double>(c, proj, xi, df, mode, k)) {
             *nblocks_out = (int)nbt;
             return hipGetLastError();
         }
@@ -558,5 +568,29 @@ hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblo
 
 hipError_t launch_reduce3(mdft_ctx *c, int nblocks) {
     hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials, nblocks, c->d_sums);
+    return hipGetLastError();
+}
+
+FinalConsts final_consts(mdft_ctx *c) {
+    FinalConsts k;
+    k.rho0 = c->rho0;
+    k.kT = c->d.kT;
+    k.dv = c->dv;
+    const double pi = acos(-1.0);
+    k.prefactor = -c->d.kT * 2.0 * (4.0 * pi * pi) / c->d.mrso / c->d.n0;  // cproj_mrso.f90:728
+    return k;
+}
+
+hipError_t launch_angl2proj_f32(mdft_ctx *c, const double *xi, float2 *proj) {
+    if (c->d.mrso != 2 || !launch_a2p_t<kPTt, float>(c, xi, proj)) return hipErrorNotSupported;
+    return hipGetLastError();
+}
+
+hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, double *df, int mode, int *nblocks_out) {
+    const size_t nbt = (c->N + kPTt - 1) / kPTt;
+    hipError_t e = ensure_partials(c, nbt);
+    if (e != hipSuccess) return e;
+    if (c->d.mrso != 2 || !launch_final_t<kPTt, float>(c, proj, xi, df, mode, final_consts(c))) return hipErrorNotSupported;
+    *nblocks_out = (int)nbt;
     return hipGetLastError();
 }

@@ -15,107 +15,120 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ inline double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-// multiply by sign*i
-__device__ inline double2 mul_si(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
+template <typename T> struct C2;
+template <> struct C2<double> { using t = double2; };
+template <> struct C2<float> { using t = float2; };
 
-template <int R>
-__device__ inline void dft_small(double2 *v, double s);
+template <typename V> __device__ inline V mk(decltype(V::x) a, decltype(V::x) b) { V v; v.x = a; v.y = b; return v; }
+template <typename V> __device__ inline V cmul(V a, V b) { return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename V> __device__ inline V cadd(V a, V b) { return mk<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ inline V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y); }
+// multiply by s*i
+template <typename V> __device__ inline V mul_si(V a, decltype(V::x) s) { return mk<V>(-s * a.y, s * a.x); }
 
-template <>
-__device__ inline void dft_small<2>(double2 *v, double) {
-    double2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
+template <int R, typename V>
+struct Dft;
 
-template <>
-__device__ inline void dft_small<4>(double2 *v, double s) {
-    double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-    double2 b0 = cadd(v[1], v[3]), b1 = mul_si(csub(v[1], v[3]), s);
-    v[0] = cadd(a0, b0);
-    v[2] = csub(a0, b0);
-    v[1] = cadd(a1, b1);
-    v[3] = csub(a1, b1);
-}
-
-template <>
-__device__ inline void dft_small<3>(double2 *v, double s) {
-    const double c1 = -0.5, s1 = 0.86602540378443864676;  // cos, sin of 2 pi / 3
-    double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    double2 y0 = cadd(v[0], t1);
-    double2 m1 = make_double2(v[0].x + c1 * t1.x, v[0].y + c1 * t1.y);
-    double2 m2 = mul_si(t2, s * s1);
-    v[0] = y0;
-    v[1] = cadd(m1, m2);
-    v[2] = csub(m1, m2);
-}
-
-template <>
-__device__ inline void dft_small<5>(double2 *v, double s) {
-    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-    double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-    double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    double2 y0 = cadd(v[0], cadd(t1, t2));
-    double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-    double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-    double2 b1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-    double2 b2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
-    b1 = mul_si(b1, s);
-    b2 = mul_si(b2, s);
-    v[0] = y0;
-    v[1] = cadd(a1, b1);
-    v[4] = csub(a1, b1);
-    v[2] = cadd(a2, b2);
-    v[3] = csub(a2, b2);
-}
-
-template <>
-__device__ inline void dft_small<8>(double2 *v, double s) {
-    const double r2 = 0.70710678118654752440;
-    // radix-2 over even/odd halves of a 4-point structure
-    double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-    dft_small<4>(e, s);
-    dft_small<4>(o, s);
-    // twiddles w8^k for k = 0..3 with sign s: exp(s i pi k / 4)
-    double2 w1 = make_double2(r2, s * r2), w3 = make_double2(-r2, s * r2);
-    o[1] = cmul(o[1], w1);
-    o[2] = mul_si(o[2], s);
-    o[3] = cmul(o[3], w3);
-    for (int k = 0; k < 4; k++) {
-        v[k] = cadd(e[k], o[k]);
-        v[k + 4] = csub(e[k], o[k]);
+template <typename V>
+struct Dft<2, V> {
+    static __device__ inline void run(V *v, decltype(V::x)) {
+        V a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
     }
-}
+};
 
-// One Stockham stage of radix R over PB rows of length n held in LDS.
-template <int R>
-__device__ inline void stockham_stage(const double2 *src, double2 *dst, int n, int ld, int Ns, int rows,
-                                      const double2 *__restrict__ tw, double s) {
+template <typename V>
+struct Dft<4, V> {
+    static __device__ inline void run(V *v, decltype(V::x) s) {
+        V a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        V b0 = cadd(v[1], v[3]), b1 = mul_si(csub(v[1], v[3]), s);
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[3] = csub(a1, b1);
+    }
+};
+
+template <typename V>
+struct Dft<3, V> {
+    static __device__ inline void run(V *v, decltype(V::x) s) {
+        using T = decltype(V::x);
+        const T c1 = (T)-0.5, s1 = (T)0.86602540378443864676;  // cos, sin of 2 pi / 3
+        V t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        V y0 = cadd(v[0], t1);
+        V m1 = mk<V>(v[0].x + c1 * t1.x, v[0].y + c1 * t1.y);
+        V m2 = mul_si(t2, s * s1);
+        v[0] = y0;
+        v[1] = cadd(m1, m2);
+        v[2] = csub(m1, m2);
+    }
+};
+
+template <typename V>
+struct Dft<5, V> {
+    static __device__ inline void run(V *v, decltype(V::x) s) {
+        using T = decltype(V::x);
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+        V t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        V t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        V y0 = cadd(v[0], cadd(t1, t2));
+        V a1 = mk<V>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        V a2 = mk<V>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        V b1 = mul_si(mk<V>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), s);
+        V b2 = mul_si(mk<V>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), s);
+        v[0] = y0;
+        v[1] = cadd(a1, b1);
+        v[4] = csub(a1, b1);
+        v[2] = cadd(a2, b2);
+        v[3] = csub(a2, b2);
+    }
+};
+
+template <typename V>
+struct Dft<8, V> {
+    static __device__ inline void run(V *v, decltype(V::x) s) {
+        using T = decltype(V::x);
+        const T r2 = (T)0.70710678118654752440;
+        V e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, V>::run(e, s);
+        Dft<4, V>::run(o, s);
+        V w1 = mk<V>(r2, s * r2), w3 = mk<V>(-r2, s * r2);
+        o[1] = cmul(o[1], w1);
+        o[2] = mul_si(o[2], s);
+        o[3] = cmul(o[3], w3);
+        for (int k = 0; k < 4; k++) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+};
+
+// One Stockham stage of radix R over `rows` lines of length n held in LDS;
+// twiddles read from the FP64 table and rounded to the working precision.
+template <int R, typename V>
+__device__ inline void stockham_stage(const V *src, V *dst, int n, int ld, int Ns, int rows,
+                                      const double2 *__restrict__ tw, decltype(V::x) s) {
+    using T = decltype(V::x);
     const int nb = n / R;
     const int tstep = n / (Ns * R);
     for (int task = threadIdx.x; task < rows * nb; task += kThreads) {
         const int row = task / nb, j = task - row * nb;
-        const double2 *in = src + row * ld;
-        double2 *out = dst + row * ld;
+        const V *in = src + row * ld;
+        V *out = dst + row * ld;
         const int k = j % Ns;
-        double2 v[R];
+        V v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            double2 x = in[j + r * nb];
+            V x = in[j + r * nb];
             if (r > 0 && k > 0) {
-                double2 w = tw[(k * r * tstep) % n];
-                w.y *= s;
-                x = cmul(x, w);
+                const double2 w = tw[(k * r * tstep) % n];
+                x = cmul(x, mk<V>((T)w.x, (T)w.y * s));
             }
             v[r] = x;
         }
-        dft_small<R>(v, s);
+        Dft<R, V>::run(v, s);
         const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) out[base + r * Ns] = v[r];
@@ -130,12 +143,15 @@ struct LineGeom {
     long long sb;   // stride of the second other axis (points)
 };
 
-template <int PB>
-__global__ __launch_bounds__(kThreads) void fft_axis_f64(double2 *__restrict__ data, int np, LineGeom g, FftAxisPlan plan,
-                                                        double sgn, double scale) {
-    extern __shared__ double2 sbuf[];
+template <int PB, typename V>
+__global__ __launch_bounds__(kThreads) void fft_axis(V *__restrict__ data, int np, LineGeom g, FftAxisPlan plan,
+                                                    double sgn_d, double scale_d) {
+    using T = decltype(V::x);
+    extern __shared__ __align__(16) unsigned char fft_smem[];
+    V *sbuf = reinterpret_cast<V *>(fft_smem);
+    const T sgn = (T)sgn_d, scale = (T)scale_d;
     const int n = g.n, ld = n + 1;
-    double2 *A = sbuf, *B = sbuf + PB * ld;
+    V *A = sbuf, *B = sbuf + PB * ld;
     const int line = blockIdx.x;
     const int a = line % g.na, b = line / g.na;
     const long long base = (long long)a * g.sa + (long long)b * g.sb;
@@ -146,32 +162,62 @@ __global__ __launch_bounds__(kThreads) void fft_axis_f64(double2 *__restrict__ d
         if (p < pb) A[p * ld + e] = data[(base + (long long)e * g.es) * np + p0 + p];
     }
     __syncthreads();
-    double2 *src = A, *dst = B;
+    V *src = A, *dst = B;
     int Ns = 1;
     for (int st = 0; st < plan.nst; st++) {
         const int R = plan.radix[st];
         switch (R) {
-            case 8: stockham_stage<8>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
-            case 4: stockham_stage<4>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
-            case 2: stockham_stage<2>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
-            case 3: stockham_stage<3>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
-            default: stockham_stage<5>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
+            case 8: stockham_stage<8, V>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
+            case 4: stockham_stage<4, V>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
+            case 2: stockham_stage<2, V>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
+            case 3: stockham_stage<3, V>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
+            default: stockham_stage<5, V>(src, dst, n, ld, Ns, pb, plan.tw, sgn); break;
         }
         Ns *= R;
         __syncthreads();
-        double2 *t = src;
+        V *t = src;
         src = dst;
         dst = t;
     }
     for (int idx = threadIdx.x; idx < n * PB; idx += kThreads) {
         const int p = idx % PB, e = idx / PB;
         if (p < pb) {
-            double2 v = src[p * ld + e];
+            V v = src[p * ld + e];
             v.x *= scale;
             v.y *= scale;
             data[(base + (long long)e * g.es) * np + p0 + p] = v;
         }
     }
+}
+
+template <typename V>
+hipError_t launch_fft3d(mdft_ctx *c, V *proj, int sign) {
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, np = c->np;
+    const double scale_last = (sign < 0) ? 1.0 / (double)((long long)nx * ny * nz) : 1.0;
+    LineGeom geo[3] = {
+        {nx, 1, ny, (long long)nx, (long long)nx * ny},
+        {ny, (long long)nx, nx, 1, (long long)nx * ny},
+        {nz, (long long)nx * ny, nx, 1, (long long)nx},
+    };
+    // PB projections per workgroup: 8 (FP64) or 16 (FP32) keeps 128-byte runs per grid point
+    constexpr int PBmax = sizeof(V) == 16 ? 8 : 16;
+    const int PB = (np % PBmax == 0 || np > PBmax) ? PBmax : 4;
+    for (int ax = 0; ax < 3; ax++) {
+        const LineGeom &g = geo[ax];
+        const size_t lines = c->N / (size_t)g.n;
+        dim3 grid((unsigned)lines, (unsigned)((np + PB - 1) / PB));
+        const double scale = (ax == 2) ? scale_last : 1.0;
+        const size_t lds = sizeof(V) * 2 * (size_t)PB * (g.n + 1);
+        if (PB == PBmax)
+            hipLaunchKernelGGL((fft_axis<PBmax, V>), grid, dim3(kThreads), lds, c->stream, proj, np, g, c->fft[ax],
+                               (double)sign, scale);
+        else
+            hipLaunchKernelGGL((fft_axis<4, V>), grid, dim3(kThreads), lds, c->stream, proj, np, g, c->fft[ax],
+                               (double)sign, scale);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -187,29 +233,6 @@ int fft_make_plan(int n, int *radix) {
     return (n == 1) ? k : -1;
 }
 
-hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign) {
-    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, np = c->np;
-    const double scale_last = (sign < 0) ? 1.0 / (double)((long long)nx * ny * nz) : 1.0;
-    LineGeom geo[3] = {
-        {nx, 1, ny, (long long)nx, (long long)nx * ny},
-        {ny, (long long)nx, nx, 1, (long long)nx * ny},
-        {nz, (long long)nx * ny, nx, 1, (long long)nx},
-    };
-    const int PB = (np % 8 == 0) ? 8 : 4;
-    for (int ax = 0; ax < 3; ax++) {
-        const LineGeom &g = geo[ax];
-        const size_t lines = c->N / (size_t)g.n;
-        dim3 grid((unsigned)lines, (unsigned)((np + PB - 1) / PB));
-        const double scale = (ax == 2) ? scale_last : 1.0;
-        const size_t lds = sizeof(double2) * 2 * (size_t)PB * (g.n + 1);
-        if (PB == 8)
-            hipLaunchKernelGGL(fft_axis_f64<8>, grid, dim3(kThreads), lds, c->stream, proj, np, g, c->fft[ax],
-                               (double)sign, scale);
-        else
-            hipLaunchKernelGGL(fft_axis_f64<4>, grid, dim3(kThreads), lds, c->stream, proj, np, g, c->fft[ax],
-                               (double)sign, scale);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
-}
+hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign) { return launch_fft3d<double2>(c, proj, sign); }
+
+hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign) { return launch_fft3d<float2>(c, proj, sign); }

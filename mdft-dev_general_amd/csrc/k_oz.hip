@@ -22,11 +22,13 @@
 namespace {
 
 
-__device__ inline double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ inline double2 cmulc(double2 a, double2 b) {  // a * conj(b)
-    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+template <typename T> struct C2;
+template <> struct C2<double> { using t = double2; };
+template <> struct C2<float> { using t = float2; };
+template <typename V> __device__ inline V mk(decltype(V::x) a, decltype(V::x) b) { V v; v.x = a; v.y = b; return v; }
+template <typename V> __device__ inline V cmul(V a, V b) { return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename V> __device__ inline V cmulc(V a, V b) {  // a * conj(b)
+    return mk<V>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
 __device__ inline double2 chop_R(double2 z) {
@@ -36,12 +38,14 @@ __device__ inline double2 chop_R(double2 z) {
     return make_double2(re, im);
 }
 
-__device__ inline double2 chop_gamma(double2 z) {
-    double re = z.x, im = z.y;
-    if (fabs(re) < DBL_EPSILON && fabs(im) < DBL_EPSILON) { re = 0.0; im = 0.0; }
-    if (fabs(re) < DBL_EPSILON) { re = 0.0; im = (double)(float)im; }
-    if (fabs(im) < DBL_EPSILON) { im = 0.0; re = (double)(float)re; }
-    return make_double2(re, im);
+template <typename V>
+__device__ inline V chop_gamma(V z) {
+    using T = decltype(V::x);
+    T re = z.x, im = z.y;
+    if (fabs((double)re) < DBL_EPSILON && fabs((double)im) < DBL_EPSILON) { re = 0; im = 0; }
+    if (fabs((double)re) < DBL_EPSILON) { re = 0; im = (T)(float)im; }
+    if (fabs((double)im) < DBL_EPSILON) { im = 0; re = (T)(float)re; }
+    return mk<V>(re, im);
 }
 
 __device__ inline double sqrtof(int k) { return k <= 0 ? 0.0 : sqrt((double)k); }
@@ -73,24 +77,29 @@ __device__ inline double2 Rl(const double2 *R, int l1, int m, int m1) {
     return R[ridx(l1, m, m1)];
 }
 
-template <int kWaves>
-__global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj, OzArgs a) {
+// R(q) is built and kept in FP64 in both modes (it is small); the projection
+// vectors, c rows and contractions use T.
+template <int kWaves, typename T>
+__global__ __launch_bounds__(64 * kWaves) void oz_kernel(typename C2<T>::t *__restrict__ proj, OzArgs a) {
+    using V = typename C2<T>::t;
     constexpr int kThreads = 64 * kWaves;
-    extern __shared__ double2 sm[];
+    extern __shared__ __align__(16) unsigned char oz_smem[];
     const int np = a.np, npc = a.npc, L = a.L, mrso = a.mrso;
     const int nR = roff(L + 1);
-    double2 *c0 = sm, *c1 = c0 + npc;
-    double2 *wbase = c1 + npc + (threadIdx.x >> 6) * (nR + 6 * np);
-    int *tabs = reinterpret_cast<int *>(c1 + npc + kWaves * (nR + 6 * np));
+    double2 *Rall = reinterpret_cast<double2 *>(oz_smem);
+    V *c0 = reinterpret_cast<V *>(Rall + kWaves * nR), *c1 = c0 + npc;
+    V *vbase = c1 + npc + (threadIdx.x >> 6) * (6 * np);
+    int *tabs = reinterpret_cast<int *>(c1 + npc + kWaves * (6 * np));
     int *tm = tabs, *tmup = tm + np, *tmu2 = tmup + np, *tia0 = tmu2 + np, *poff = tia0 + np;
-    double2 *R = wbase;
-    double2 *X0 = R + nR, *X1 = X0 + np, *A = X1 + np, *B = A + np, *GQ = B + np, *GMQ = GQ + np;
+    double2 *R = Rall + (threadIdx.x >> 6) * nR;
+    V *X0 = vbase, *X1 = X0 + np, *A = X1 + np, *B = A + np, *GQ = B + np, *GMQ = GQ + np;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int chunk = blockIdx.x;
     const int iq = a.ciq[chunk];  // 1-based lower row
     for (int i = threadIdx.x; i < npc; i += kThreads) {
-        c0[i] = a.ck[(size_t)(iq - 1) * npc + i];
-        c1[i] = a.ck[(size_t)iq * npc + i];
+        const double2 u = a.ck[(size_t)(iq - 1) * npc + i], w = a.ck[(size_t)iq * npc + i];
+        c0[i] = mk<V>((T)u.x, (T)u.y);
+        c1[i] = mk<V>((T)w.x, (T)w.y);
     }
     for (int i = threadIdx.x; i < np; i += kThreads) {
         tm[i] = a.pm[i];
@@ -116,8 +125,9 @@ __global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj
         const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
         const double q0 = a.kx[ix], q1 = a.ky[iy], q2 = a.kz[iz];
         // alpha relative to the chunk's row so host and device agree on the interval
-        const double alpha = sqrt(q0 * q0 + q1 * q1 + q2 * q2) / a.dq + 1.0 - (double)iq;
-        const double2 *src0 = proj + (size_t)gq * np, *src1 = proj + (size_t)gmq * np;
+        const double alpha_d = sqrt(q0 * q0 + q1 * q1 + q2 * q2) / a.dq + 1.0 - (double)iq;
+        const T alpha = (T)alpha_d, beta = (T)(1.0 - alpha_d);
+        const V *src0 = proj + (size_t)gq * np, *src1 = proj + (size_t)gmq * np;
         for (int i = lane; i < np; i += 64) {
             X0[i] = src0[i];
             X1[i] = src1[i];
@@ -221,21 +231,22 @@ __global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj
             const bool first = j < np;
             const int ip = first ? j : j - np;
             const int m = tm[ip], khi = tmup[ip], mu2 = tmu2[ip];
-            const double2 *X = first ? X0 : X1;
+            const V *X = first ? X0 : X1;
             const int kk = first ? khi : -khi;
             const int base = PIDX(m, -m, mu2), step = m / mrso + 1;
             const double2 *Rrow = R + ridx(m, -m, kk);
             const int rstep = 2 * m + 1;
-            double sx = 0.0, sy = 0.0;
+            T sx = 0, sy = 0;
             for (int t = 0; t <= 2 * m; t++) {
-                const double2 v = cmul(X[base + t * step], Rrow[t * rstep]);
+                const double2 r = Rrow[t * rstep];
+                const V v = cmul(X[base + t * step], mk<V>((T)r.x, (T)r.y));
                 sx += v.x;
                 sy += v.y;
             }
-            if (first) A[ip] = make_double2(sx, sy);
+            if (first) A[ip] = mk<V>(sx, sy);
             else {
-                const double sg = (m & 1) ? -1.0 : 1.0;
-                B[ip] = make_double2(sx * sg, sy * sg);
+                const T sg = (m & 1) ? (T)-1 : (T)1;
+                B[ip] = mk<V>(sx * sg, sy * sg);
             }
         }
         wave_sync();
@@ -244,30 +255,29 @@ __global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj
             const bool first = j < np;
             const int ip = first ? j : j - np;
             const int khi = tmup[ip];
-            const double2 *src_neg = first ? A : B;  // nu2 < 0 term
-            const double2 *src_pos = first ? B : A;  // conj term
+            const V *src_neg = first ? A : B;  // nu2 < 0 term
+            const V *src_pos = first ? B : A;  // conj term
             int ia = tia0[ip];
-            double sx = 0.0, sy = 0.0;
+            T sx = 0, sy = 0;
             for (int n = abs(khi); n <= L; n++) {
                 const int h = n / mrso;
                 const int base = PIDX(n, khi, 0);
                 for (int nu2 = -h; nu2 <= h; nu2++, ia++) {
-                    const double2 lo = c0[ia], hi = c1[ia];
-                    const double2 ce = make_double2(alpha * hi.x + (1.0 - alpha) * lo.x,
-                                                    alpha * hi.y + (1.0 - alpha) * lo.y);
-                    double2 v;
+                    const V lo = c0[ia], hi = c1[ia];
+                    const V ce = mk<V>(alpha * hi.x + beta * lo.x, alpha * hi.y + beta * lo.y);
+                    V v;
                     if (nu2 < 0) {
                         v = cmul(ce, src_neg[base - nu2]);
                     } else {
-                        const double2 x = src_pos[base + nu2];
-                        v = cmul(ce, make_double2(x.x, -x.y));
+                        const V x = src_pos[base + nu2];
+                        v = cmul(ce, mk<V>(x.x, -x.y));
                     }
                     sx += v.x;
                     sy += v.y;
                 }
             }
-            if (first) GQ[ip] = chop_gamma(make_double2(sx, sy));
-            else GMQ[ip] = chop_gamma(make_double2(sx, sy));
+            if (first) GQ[ip] = chop_gamma(mk<V>(sx, sy));
+            else GMQ[ip] = chop_gamma(mk<V>(sx, sy));
         }
         wave_sync();
         // ---- rotation back with conj(R) (:611-641) and store (:648-656)
@@ -277,25 +287,25 @@ __global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj
             if (first && eq) continue;  // for q == -q the -q store overwrites this one
             const int ip = first ? j : j - np;
             const int m = tm[ip], mup = tmup[ip], mu2 = tmu2[ip];
-            const double2 *G = first ? GQ : GMQ;
+            const V *G = first ? GQ : GMQ;
             const int base = PIDX(m, -m, mu2), step = m / mrso + 1;
             const double2 *Rrow = R + ridx(m, mup, 0);
-            double sx = 0.0, sy = 0.0;
+            T sx = 0, sy = 0;
             for (int t = 0; t <= 2 * m; t++) {
                 const int khi = t - m;
                 const double2 r = Rrow[first ? khi : -khi];
-                const double2 v = cmulc(G[base + t * step], r);
+                const V v = cmulc(G[base + t * step], mk<V>((T)r.x, (T)r.y));
                 sx += v.x;
                 sy += v.y;
             }
             if (first) {
-                proj[(size_t)gq * np + ip] = make_double2(sx, sy);
+                proj[(size_t)gq * np + ip] = mk<V>(sx, sy);
             } else {
-                const double sg = (m & 1) ? -1.0 : 1.0;
+                const T sg = (m & 1) ? (T)-1 : (T)1;
                 sx *= sg;
                 sy *= sg;
                 if (eq && nyq) sy = -sy;
-                proj[(size_t)gmq * np + ip] = make_double2(sx, sy);
+                proj[(size_t)gmq * np + ip] = mk<V>(sx, sy);
             }
         }
         wave_sync();
@@ -305,7 +315,9 @@ __global__ __launch_bounds__(64 * kWaves) void oz_f64(double2 *__restrict__ proj
 
 }  // namespace
 
-hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) {
+template <typename T>
+hipError_t launch_oz(mdft_ctx *c, typename C2<T>::t *proj) {
+    using V = typename C2<T>::t;
     OzArgs a;
     a.nx = c->d.nx;
     a.ny = c->d.ny;
@@ -334,19 +346,23 @@ hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) {
     a.ck = c->d_ck;
     const int nR = (c->L + 1) * (4 * (c->L + 1) * (c->L + 1) - 1) / 3;
     auto lds_for = [&](int waves) {
-        return sizeof(double2) * (2 * (size_t)c->npc + (size_t)waves * (nR + 6 * (size_t)c->np)) +
+        return sizeof(double2) * (size_t)waves * nR + sizeof(V) * (2 * (size_t)c->npc + (size_t)waves * 6 * c->np) +
                sizeof(int) * (4 * (size_t)c->np + MDFT_MAX_MMAX + 2);
     };
     const dim3 grid((unsigned)c->pairs.nchunks);
     if (lds_for(8) <= c->lds_limit)
-        hipLaunchKernelGGL(oz_f64<8>, grid, dim3(512), lds_for(8), c->stream, proj, a);
+        hipLaunchKernelGGL((oz_kernel<8, T>), grid, dim3(512), lds_for(8), c->stream, proj, a);
     else if (lds_for(4) <= c->lds_limit)
-        hipLaunchKernelGGL(oz_f64<4>, grid, dim3(256), lds_for(4), c->stream, proj, a);
+        hipLaunchKernelGGL((oz_kernel<4, T>), grid, dim3(256), lds_for(4), c->stream, proj, a);
     else if (lds_for(2) <= c->lds_limit)
-        hipLaunchKernelGGL(oz_f64<2>, grid, dim3(128), lds_for(2), c->stream, proj, a);
+        hipLaunchKernelGGL((oz_kernel<2, T>), grid, dim3(128), lds_for(2), c->stream, proj, a);
     else if (lds_for(1) <= c->lds_limit)
-        hipLaunchKernelGGL(oz_f64<1>, grid, dim3(64), lds_for(1), c->stream, proj, a);
+        hipLaunchKernelGGL((oz_kernel<1, T>), grid, dim3(64), lds_for(1), c->stream, proj, a);
     else
         return hipErrorInvalidConfiguration;
     return hipGetLastError();
 }
+
+hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) { return launch_oz<double>(c, proj); }
+
+hipError_t launch_oz_f32(mdft_ctx *c, float2 *proj) { return launch_oz<float>(c, proj); }

