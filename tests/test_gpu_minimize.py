@@ -80,5 +80,7 @@ def test_lbfgs_reaches_the_cpu_minimum(oracle_mod):
     r = spmin(fg, xi0, jac=True, method="L-BFGS-B", options={"maxiter": 3000, "gtol": 1e-10, "ftol": 1e-15})
     say(f"  cpu: F={r.fun:.10f} nit={r.nit}")
     assert abs(res.F - r.fun) < 1e-7 * max(1.0, abs(r.fun))
+    # F depends on xi only through xi^2 (rho = rho0 xi^2), so the two minimisers
+    # may settle on opposite signs of xi where the density is small
     xs = x.download()
-    assert np.abs(xs - r.x).max() < 1e-3
+    assert np.abs(np.abs(xs) - np.abs(r.x)).max() < 1e-3
