@@ -448,7 +448,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
     void *bufs[] = {c->d_w,   c->d_a2p,  c->d_p2a,  c->d_fm,   c->d_cph,      c->d_sph,        c->d_cps,
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
-                    c->d_partials, c->d_sums, c->d_vpart};
+                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);

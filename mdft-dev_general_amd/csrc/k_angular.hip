@@ -252,10 +252,15 @@ __global__ __launch_bounds__(kThreads) void ideal_f64(const double *__restrict__
     block_reduce3(fid, fext, zero, partials);
 }
 
+// Sums of the per-workgroup (F_id, F_ext, F_exc) partials in a fixed order (so
+// repeated evaluations are bitwise identical): each workgroup of the first
+// pass sums one contiguous slice, one workgroup sums the slices.
 __global__ __launch_bounds__(kThreads) void reduce3(const double *__restrict__ partials, int nblocks,
                                                    double *__restrict__ out) {
+    const int per = (nblocks + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * per, hi = min(nblocks, lo + per);
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += kThreads) {
+    for (int i = lo + threadIdx.x; i < hi; i += kThreads) {
         a += partials[3 * i + 0];
         b += partials[3 * i + 1];
         c += partials[3 * i + 2];
@@ -274,9 +279,9 @@ __global__ __launch_bounds__(kThreads) void reduce3(const double *__restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = red[0][0];
-        out[1] = red[1][0];
-        out[2] = red[2][0];
+        out[3 * blockIdx.x + 0] = red[0][0];
+        out[3 * blockIdx.x + 1] = red[1][0];
+        out[3 * blockIdx.x + 2] = red[2][0];
     }
 }
 
@@ -567,7 +572,17 @@ hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblo
 }
 
 hipError_t launch_reduce3(mdft_ctx *c, int nblocks) {
-    hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials, nblocks, c->d_sums);
+    constexpr int kSlices = 512;
+    if (nblocks <= 4 * kThreads) {
+        hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials, nblocks, c->d_sums);
+        return hipGetLastError();
+    }
+    if (!c->d_slices) {
+        hipError_t e = hipMalloc(&c->d_slices, sizeof(double) * 3 * kSlices);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(reduce3, dim3(kSlices), dim3(kThreads), 0, c->stream, c->d_partials, nblocks, c->d_slices);
+    hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_slices, kSlices, c->d_sums);
     return hipGetLastError();
 }
 

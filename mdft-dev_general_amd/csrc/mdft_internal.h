@@ -74,6 +74,7 @@ struct mdft_ctx {
     double *d_xi_stage = nullptr; // staging for host xi
     double *d_df_stage = nullptr; // staging for host df
     double *d_partials = nullptr; // block partial sums
+    double *d_slices = nullptr;   // [3 * 512] first-pass sums of d_partials
     double *d_sums = nullptr;     // [8] final reductions
     double *d_vpart = nullptr;    // vector-op block partials
     double *h_sums = nullptr;     // pinned mirror of d_sums

@@ -210,7 +210,14 @@ def synthetic_ck(mmax: int, nk: int, dk: float, seed: int, n0: float, mrso: int 
     the spherical Bessel function of the component's order l (j0 for l = 0).  The khi
     components are c^{mn}_{mu nu,khi} = sum_l 3j(m n l; khi -khi 0) c^{mnl}_{mu nu}
     (times i for odd l).  The whole table is then scaled by one factor so that
-    the largest singular value of n0*C_khi(k) over all k and khi is <= bound."""
+    the largest singular value of n0*C_khi(k) over all k and khi is <= bound.
+
+    With self_term = s > 0 each row k then gains -s g(k) I, where g(k) is the
+    largest singular value of C_khi(k) over khi and I is the identity operator
+    of add_self_term: the excess term becomes convex (F_exc >= 0 for every
+    drho), so F is bounded below next to charged sites, and the added term
+    decays with k exactly as c does (it is negligible on the Nyquist planes,
+    where the reference's even-grid treatment is not symmetric)."""
     from scipy.special import spherical_jn
 
     rng = np.random.default_rng(seed)
@@ -244,17 +251,28 @@ def synthetic_ck(mmax: int, nk: int, dk: float, seed: int, n0: float, mrso: int 
     if norm > bound:
         t.c *= bound / norm
     if self_term:
-        add_self_term(t, -self_term)
+        add_self_term(t, -self_term * row_norms(t, mmax, mrso))
     return t
 
 
-def add_self_term(t: CkTable, lam: float) -> None:
-    """Adds lam * I to the table, where I is the identity operator of the
-    excess functional in the reference's conventions:
-    c(m, n, mu, nu; khi) = delta_mn delta_{mu,-nu} (-1)^khi.  With lam = -1 the
-    excess term gains the local penalty +(kT/2) ||drho||^2 / rho0, which
-    exactly matches the curvature of the ideal term at rho0; it keeps the
-    functional convex at any density, including next to charged sites."""
+def row_norms(t: CkTable, mmax: int, mrso: int = 2) -> np.ndarray:
+    """Largest singular value of C_khi(k) over khi, per row k."""
+    g = np.zeros(t.nq_file)
+    for _khi, (_keys, mat) in khi_blocks(t, mmax, mrso).items():
+        if mat.shape[1]:
+            g = np.maximum(g, np.linalg.norm(mat, ord=2, axis=(1, 2)))
+    return g
+
+
+def add_self_term(t: CkTable, lam) -> None:
+    """Adds lam * I to the table (lam a scalar or one value per row k), where I
+    is the identity operator of the excess functional in the reference's
+    conventions: c(m, n, mu, nu; khi) = delta_mn delta_{mu,-nu} (-1)^khi.
+    In the metric of the ideal term's curvature at rho0 (kT dv w / rho0), the
+    excess Hessian of lam * I is -lam times the orthogonal projector on the
+    represented projections (exact on odd grids; on even grids the reference's
+    Nyquist-plane treatment is not symmetric and flips it there)."""
+    lam = np.broadcast_to(np.asarray(lam, dtype=np.float64), (t.nq_file,))
     for i in range(t.np_file):
         if t.m[i] == t.n[i] and t.mu[i] == -t.nu[i]:
             t.c[:, i] += lam * (-1.0) ** int(t.khi[i])

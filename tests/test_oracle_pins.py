@@ -367,3 +367,47 @@ def test_stages_consistent(oracle_mod):
     for ip in range(o.np_):
         ref = np.fft.ifftn(f[..., ip]) * f[..., ip].size
         assert np.allclose(s2[:, ip].reshape(6, 6, 6), ref, atol=1e-12)
+
+
+def _excess_hessian_extremes(oracle_mod, mmax, n, table):
+    """Extreme eigenvalues of the excess Hessian in drho, in the metric of the
+    ideal term's curvature at rho0 (kT dv w_io / rho0), by Lanczos on
+    F_exc's exact gradient (F_exc is quadratic in drho)."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    from mdft.grid import angular_grid
+    o = oracle_mod.Oracle((n, n, n), (9.0, 9.0, 9.0), mmax, 2, 2.4777, 0.0333)
+    o.set_ck(table)
+    wq = angular_grid(mmax, 2)["w"]
+    dm = 1.0 / np.sqrt(np.tile(2.4777 * o.dv * wq / o.rho0, o.N))
+
+    def hv(v):
+        u = dm * np.asarray(v).ravel()
+        s = 1e-4 * o.rho0 / max(np.abs(u).max(), 1e-300)
+        xi = np.sqrt(1.0 + s * u / o.rho0)
+        _, df = o.excess(xi)
+        return dm * (df * o.dv / (2.0 * o.rho0 * xi)) / s
+
+    op = LinearOperator((o.N * o.no,) * 2, matvec=hv, dtype=np.float64)
+    lo = eigsh(op, k=1, which="SA", tol=1e-7, return_eigenvectors=False)[0]
+    hi = eigsh(op, k=1, which="LA", tol=1e-7, return_eigenvectors=False)[0]
+    return lo, hi
+
+
+def test_self_term_is_the_identity_and_makes_excess_convex(oracle_mod):
+    """On an odd grid the table I = delta_mn delta_{mu,-nu} (-1)^khi gives an
+    excess Hessian of exactly minus the projector (eigenvalues in [-1, 0] in the
+    ideal-curvature metric); the synthetic table with its self term gives a
+    positive semi-definite excess Hessian (F_exc >= 0 for every drho), while the
+    random part alone is indefinite."""
+    mmax, n = 2, 5
+    ident = ckmod.synthetic_ck(mmax, 256, 0.05, seed=4, n0=0.0333, self_term=0.0)
+    ident.c[:] = 0.0
+    ckmod.add_self_term(ident, 1.0)
+    lo, hi = _excess_hessian_extremes(oracle_mod, mmax, n, ident)
+    assert abs(lo + 1.0) < 1e-6 and abs(hi) < 1e-6, (lo, hi)
+    raw = ckmod.synthetic_ck(mmax, 256, 0.05, seed=4, n0=0.0333, self_term=0.0)
+    lo, hi = _excess_hessian_extremes(oracle_mod, mmax, n, raw)
+    assert lo < -1e-3 and hi > 1e-3, (lo, hi)
+    full = ckmod.synthetic_ck(mmax, 256, 0.05, seed=4, n0=0.0333)
+    lo, hi = _excess_hessian_extremes(oracle_mod, mmax, n, full)
+    assert lo > -1e-7, (lo, hi)
