@@ -202,7 +202,7 @@ def _lbasis_orbits(mmax: int, mrso: int):
 
 def synthetic_ck(mmax: int, nk: int, dk: float, seed: int, n0: float, mrso: int = 2,
                  bound: float = 0.8, a_range=(-2.0, 2.0), s_range=(0.5, 1.5), r_range=(2.0, 4.0),
-                 self_term: float = 1.0) -> CkTable:
+                 self_term: float = 15.0, self_kc: float = 3.0) -> CkTable:
     """Seeded synthetic c(k) table in the reference file order.
 
     Each independent real l-basis component is n0*A*exp(-k^2 s^2/2)*j_l(kR)
@@ -212,12 +212,15 @@ def synthetic_ck(mmax: int, nk: int, dk: float, seed: int, n0: float, mrso: int 
     (times i for odd l).  The whole table is then scaled by one factor so that
     the largest singular value of n0*C_khi(k) over all k and khi is <= bound.
 
-    With self_term = s > 0 each row k then gains -s g(k) I, where g(k) is the
-    largest singular value of C_khi(k) over khi and I is the identity operator
-    of add_self_term: the excess term becomes convex (F_exc >= 0 for every
-    drho), so F is bounded below next to charged sites, and the added term
-    decays with k exactly as c does (it is negligible on the Nyquist planes,
-    where the reference's even-grid treatment is not symmetric)."""
+    With self_term = s > 0 each row k then gains -(g(k) + s exp(-(k/kc)^8)) I,
+    where g(k) is the largest singular value of C_khi(k) over khi and I is the
+    identity operator of add_self_term.  The g(k) part makes the excess term
+    convex (F_exc >= 0 for every drho), so F is bounded below next to charged
+    sites; the s part is a compressibility term (rho0 c(k -> 0) = -s; s = 15 is
+    about water's, S(0) = 1/(1 + s) ~ 0.06) that keeps the density next to
+    charged sites finite.  Both parts vanish on the Nyquist planes of the
+    grids used here (|k| >= 6.28 A^-1, where it is below 1e-150), where the reference's even-grid
+    treatment is not symmetric."""
     from scipy.special import spherical_jn
 
     rng = np.random.default_rng(seed)
@@ -251,8 +254,14 @@ def synthetic_ck(mmax: int, nk: int, dk: float, seed: int, n0: float, mrso: int 
     if norm > bound:
         t.c *= bound / norm
     if self_term:
-        add_self_term(t, -self_term * row_norms(t, mmax, mrso))
+        add_self_term(t, -self_profile(t, mmax, mrso, self_term, self_kc))
     return t
+
+
+def self_profile(t: CkTable, mmax: int, mrso: int, s: float, kc: float) -> np.ndarray:
+    """Per-row strength of the self term of synthetic_ck."""
+    k = np.arange(t.nq_file) * t.dq  # row i holds k = i dq
+    return row_norms(t, mmax, mrso) + s * np.exp(-(k / kc) ** 8)
 
 
 def row_norms(t: CkTable, mmax: int, mrso: int = 2) -> np.ndarray:

@@ -41,7 +41,7 @@ def test_synthetic_obeys_symmetry_laws(mmax):
     assert max(_sym_errors(t)) < 1e-15
     r = ck.synthetic_ck(mmax, 128, 0.05, seed=3, n0=0.0333, self_term=0.0)  # the random part alone
     assert ck.spectral_norm_max(r, mmax) <= 0.8 + 1e-12
-    ck.add_self_term(r, -ck.row_norms(r, mmax))
+    ck.add_self_term(r, -ck.self_profile(r, mmax, 2, 15.0, 3.0))
     assert np.array_equal(r.c, t.c)
     # at k = 0 only the l = 0 part survives, which needs m = n
     for i in range(t.np_file):
