@@ -229,7 +229,7 @@ int run_excess_pipeline(mdft_ctx *c, const double *xi, int stage, bool timers) {
     if (timers) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (stage >= 2) HIPCHK(c, f32 ? launch_fft3d_f32(c, p32, +1) : launch_fft3d_f64(c, p64, +1));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    if (stage >= 3) HIPCHK(c, f32 ? launch_oz_f32(c, p32) : launch_oz_f64(c, p64));
+    if (stage >= 3) HIPCHK(c, f32 ? launch_oz32(c, p32) : launch_oz_f64(c, p64));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (stage >= 4) HIPCHK(c, f32 ? launch_fft3d_f32(c, p32, -1) : launch_fft3d_f64(c, p64, -1));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
@@ -416,6 +416,14 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
     ok = ok && hipMalloc(&c->d_sums, sizeof(double) * 8) == hipSuccess;
     ok = ok && hipHostMalloc(&c->h_sums, sizeof(double) * 8, hipHostMallocDefault) == hipSuccess;
     if (!ok) return bad(MDFT_ENOMEM, "device allocation failed");
+    if (d.precision == MDFT_PREC_FP32) {
+        std::vector<float> ftab;
+        std::vector<int> lanerow;
+        if (oz32_tables(c, ftab, lanerow) != MDFT_OK) return bad(MDFT_EINVAL, "too many projections per lane");
+        if (upload(c, &c->d_oz32_ftab, ftab.data(), ftab.size()) != hipSuccess ||
+            upload(c, &c->d_oz32_lanerow, lanerow.data(), lanerow.size()) != hipSuccess)
+            return bad(MDFT_ENOMEM, "device allocation failed");
+    }
     AngTables &t = c->ang;
     t.ntheta = c->ntheta;
     t.nphi = c->nphi;
@@ -448,7 +456,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
     void *bufs[] = {c->d_w,   c->d_a2p,  c->d_p2a,  c->d_fm,   c->d_cph,      c->d_sph,        c->d_cps,
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
-                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart};
+                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_oz32_ftab, c->d_oz32_lanerow};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);

@@ -365,4 +365,3 @@ hipError_t launch_oz(mdft_ctx *c, typename C2<T>::t *proj) {
 
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) { return launch_oz<double>(c, proj); }
 
-hipError_t launch_oz_f32(mdft_ctx *c, float2 *proj) { return launch_oz<float>(c, proj); }

@@ -10,6 +10,7 @@
 #include "mdft_gpu.h"
 
 #define MDFT_MAX_MMAX 6
+#define MDFT_OZ32_ROWS_PER_LANE 4  // OZ rows per lane in the FP32 q-space kernel (np <= 256)
 #define MDFT_FFT_MAX_STAGES 12
 
 // Angular tables of one context (all device pointers).
@@ -67,6 +68,8 @@ struct mdft_ctx {
     int npc = 0, nq = 0;
     double dq = 0;
     double2 *d_ck = nullptr;      // [nq][npc]
+    float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
+    int *d_oz32_lanerow = nullptr; // [MDFT_OZ32_ROWS_PER_LANE][64] OZ rows per lane
     PairSchedule pairs;
     // fields
     double *d_vext = nullptr;     // [N][no]
@@ -98,7 +101,8 @@ hipError_t launch_final_f64(mdft_ctx *c, const double2 *proj, const double *xi, 
 hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblocks_out);
 hipError_t launch_angl2proj_f32(mdft_ctx *c, const double *xi, float2 *proj);
 hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, double *df, int mode, int *nblocks_out);
-hipError_t launch_oz_f32(mdft_ctx *c, float2 *proj);
+hipError_t launch_oz32(mdft_ctx *c, float2 *proj);
+int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &lanerow);
 hipError_t launch_reduce3(mdft_ctx *c, int nblocks);
 hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
 hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
