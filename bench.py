@@ -11,10 +11,12 @@ projections), one GPU.
   its final F is the converged value Delta G_ref.
 * plain FP64 path: the same run timed up to the first iterate whose F is within
   0.05 kJ/mol of Delta G_ref.
-* fast path: FP32 evaluations (angular transforms, FFTs, OZ in FP32; ideal
-  term, reductions and L-BFGS in FP64) from the same start, timed up to the
-  first iterate within 0.05 kJ/mol of Delta G_ref plus one FP64 evaluation of
-  that iterate, which must itself be within the band.  `value` is that time.
+* fast path: the learned warm start (features of vext, network inference and
+  the guess, all inside the timed region) followed by FP32 evaluations
+  (angular transforms, FFTs, OZ in FP32; ideal term, reductions and L-BFGS in
+  FP64), timed up to the first iterate within 0.05 kJ/mol of Delta G_ref plus
+  one FP64 evaluation of that iterate, which must itself be within the band.
+  `value` is that time.  Also reported: the FP32 path from the cold start.
 
 Also reported: F+dF evaluations per second (one step = one evaluation over the
 whole grid, inputs resident in HBM), per-phase HIP-event timings on the
@@ -101,6 +103,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="evaluation precision for the rate/phase measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--weights", default=os.path.join(PKG, "mdft", "data", "warmstart_solute20.npz"),
+                    help="trained warm-start network")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -191,32 +195,49 @@ def main():
                          ftarget=dg_ref, fband=band)
         eng.synchronize()
         t_plain = time.perf_counter() - t0
-        # ---- fast path: FP32 evaluations to the band, confirmed by one FP64 evaluation
+        # ---- fast path: learned warm start + FP32 evaluations to the band,
+        # confirmed by one FP64 evaluation of the final iterate
+        from mdft import warmstart as wsm
         e32 = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, "fp32")
         e32.set_ck(ckt)
         e32.compute_vext(sol, w.solvent)
-        xi.upload(xi0)
-        e32.synchronize()
-        eng.synchronize()
-        t0 = time.perf_counter()
-        res_f = minimize(e32, xi, g=df, m=5, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6,
-                         ftarget=dg_ref, fband=band)
-        confirm, _ = eng.energy_and_gradient(xi, None)
-        eng.synchronize()
-        t_fast = time.perf_counter() - t0
-        dg_fast = float(np.sum(confirm))
+        net = wsm.WarmStartNet.load(args.weights) if os.path.exists(args.weights) else None
+
+        def fast(use_net):
+            e32.synchronize()
+            eng.synchronize()
+            t0 = time.perf_counter()
+            if use_net:
+                wsm.warm_start(e32, net, xi)
+            else:
+                xi.upload(xi0)
+            r = minimize(e32, xi, g=df, m=5, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref, fband=band)
+            c, _ = eng.energy_and_gradient(xi, None)
+            eng.synchronize()
+            return time.perf_counter() - t0, r, float(np.sum(c))
+
+        t_cold32, res_c32, dg_c32 = fast(False)
+        if net is not None:
+            t_fast, res_f, dg_fast = fast(True)
+        else:
+            t_fast, res_f, dg_fast = t_cold32, res_c32, dg_c32
         line.update({
             "value": t_fast, "unit": "s", "steps": res_f.evaluations,
             "ms_per_step": t_fast / max(res_f.evaluations, 1) * 1e3,
             "t_fast_s": t_fast, "t_plain_fp64_s": t_plain, "speedup_vs_plain": t_plain / t_fast,
-            "t_converge_fp64_s": t_conv,
-            "fast_path": "FP32 evaluations + FP64 L-BFGS, reference cold start, FP64 confirmation (no learned warm start yet)",
+            "t_converge_fp64_s": t_conv, "t_fp32_cold_start_s": t_cold32,
+            "fast_path": ("learned warm start (network inference inside the timed region) + FP32 evaluations + "
+                          "FP64 L-BFGS, FP64 confirmation") if net is not None else
+                         "FP32 evaluations + FP64 L-BFGS from the reference cold start, FP64 confirmation "
+                         "(no trained warm-start weights found)",
             "dG_ref_kJmol": dg_ref, "dG_fast_confirmed_kJmol": dg_fast,
             "dG_fast_error_kJmol": dg_fast - dg_ref, "within_band": bool(abs(dg_fast - dg_ref) <= band),
+            "dG_fp32_cold_error_kJmol": dg_c32 - dg_ref,
             "F_parts": [float(v) for v in res.parts],
             "converge_evaluations": res.evaluations, "converge_iterations": res.iterations,
             "converge_grad_ratio": res.grad_ratio, "converge_status": res.status,
             "plain_evaluations": res_p.evaluations, "plain_status": res_p.status,
+            "fp32_cold_evaluations": res_c32.evaluations,
             "fast_evaluations": res_f.evaluations, "fast_iterations": res_f.iterations,
             "fast_status": res_f.status,
         })

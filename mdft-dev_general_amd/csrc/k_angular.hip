@@ -133,6 +133,7 @@ struct FinalConsts {
 
 // mode 0: df = ideal+ext gradient + excess gradient (energy_and_gradient)
 // mode 1: df += excess gradient (excess_energy plug-in)
+// mode 2: df = gamma(r, Omega) = -(prefactor / kT) proj2angl(proj), the OZ field in units of ln(rho/rho0)
 template <int PT>
 __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict__ proj, const double *__restrict__ xi,
                                                      const double *__restrict__ vext, double *__restrict__ df,
@@ -221,8 +222,10 @@ __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict_
                 fid += kT * dv * wio * rho0;
             }
             if (df) df[g] = gid + 2.0 * rho0 * x * vexc;
-        } else {
+        } else if (mode == 1) {
             if (df) df[g] += 2.0 * rho0 * x * vexc;
+        } else {  // mode 2: gamma(r, Omega) in units of ln(rho/rho0)
+            if (df) df[g] = -(k.prefactor / kT) * v;
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -463,8 +466,10 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
                 fid += kT * dv * wio * rho0;
             }
             if (df) df[g] = gid + 2.0 * rho0 * x * vexc;
-        } else {
+        } else if (mode == 1) {
             if (df) df[g] += 2.0 * rho0 * x * vexc;
+        } else {  // mode 2: gamma(r, Omega) in units of ln(rho/rho0)
+            if (df) df[g] = -(k.prefactor / kT) * v;
         }
     }
     block_reduce3(fid, fext, fexc, partials);

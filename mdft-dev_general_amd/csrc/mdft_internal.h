@@ -70,6 +70,8 @@ struct mdft_ctx {
     double2 *d_ck = nullptr;      // [nq][npc]
     float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
     int *d_oz32_lanerow = nullptr; // [MDFT_OZ32_ROWS_PER_LANE][64] OZ rows per lane
+    double *d_wsB = nullptr;      // [4][no] low angular basis of the warm start
+    double *d_wsBw = nullptr;     // [4][no] the same times w_io / sum_io w_io B_k^2
     PairSchedule pairs;
     // fields
     double *d_vext = nullptr;     // [N][no]
@@ -108,6 +110,8 @@ hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
 hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
+hipError_t launch_ws_project(mdft_ctx *c, const double *f, float *out, int mode);
+hipError_t launch_ws_guess(mdft_ctx *c, const float *pred, double *xi);
 int fft_make_plan(int n, int *radix);
 hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out);
 hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out);

@@ -1,0 +1,237 @@
+"""Learned warm start of the minimisation (ctypes binding of include/mdft_wsnet.h).
+
+The reference has no learned component; the minimiser it drives
+(src/energy_minimization.f90) starts from the cold guess of
+src/module_density.f90:267-284.  Here a periodic dilated 3D convolutional
+network (hand-written HIP forward/backward/Adam kernels, FP32) maps per-point
+features of the external potential to the m <= 1 angular projections of the
+converged OZ field gamma = c (*) drho, and the minimiser starts from
+
+    xi = exp((-beta vext + sum_k pred_k(r) B_k(Omega)) / 2),
+
+the Euler-Lagrange form of the minimum with gamma truncated to m <= 1.  The
+minimisation afterwards is the plain one: the network changes the number of
+iterations, not the answer.
+
+Training data are converged solutions of the plain method on small boxes with
+the grid spacing of the target workload (the network is translation invariant
+and sees about 10 A around each point).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import time
+
+import numpy as np
+
+from .gpu import DeviceBuffer, Engine, load_library
+
+CHANNELS = (8, 32, 32, 32, 32, 4)
+DILATIONS = (1, 2, 4, 8, 1)
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA, "warmstart_solute20.npz")
+
+WSNET_EXPORTS = (
+    "mdft_wsnet_create", "mdft_wsnet_destroy", "mdft_wsnet_last_error", "mdft_wsnet_num_params",
+    "mdft_wsnet_get_params", "mdft_wsnet_set_params", "mdft_wsnet_get_grads", "mdft_wsnet_forward",
+    "mdft_wsnet_loss_grad", "mdft_wsnet_adam", "mdft_ws_features", "mdft_ws_gamma", "mdft_ws_guess",
+)
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    lib = load_library()
+    if not _bound:
+        vp, ip = C.c_void_p, C.POINTER(C.c_int)
+        lib.mdft_wsnet_create.argtypes = [C.c_int, ip, ip, C.c_ulonglong, C.c_int, C.POINTER(vp)]
+        lib.mdft_wsnet_destroy.argtypes = [vp]
+        lib.mdft_wsnet_destroy.restype = None
+        lib.mdft_wsnet_last_error.argtypes = [vp]
+        lib.mdft_wsnet_last_error.restype = C.c_char_p
+        lib.mdft_wsnet_num_params.argtypes = [vp]
+        lib.mdft_wsnet_num_params.restype = C.c_longlong
+        lib.mdft_wsnet_get_params.argtypes = [vp, vp]
+        lib.mdft_wsnet_set_params.argtypes = [vp, vp]
+        lib.mdft_wsnet_get_grads.argtypes = [vp, vp]
+        lib.mdft_wsnet_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp]
+        lib.mdft_wsnet_loss_grad.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(C.c_double)]
+        lib.mdft_wsnet_adam.argtypes = [vp, C.c_float]
+        lib.mdft_ws_features.argtypes = [vp, vp]
+        lib.mdft_ws_gamma.argtypes = [vp, vp, vp, vp]
+        lib.mdft_ws_guess.argtypes = [vp, vp, vp]
+        for n in WSNET_EXPORTS:
+            if n not in ("mdft_wsnet_destroy", "mdft_wsnet_last_error", "mdft_wsnet_num_params"):
+                getattr(lib, n).restype = C.c_int
+        _bound = True
+    return lib
+
+
+class WarmStartNet:
+    """The network: create with random (seeded) weights or load saved ones."""
+
+    def __init__(self, channels=CHANNELS, dilations=DILATIONS, seed: int = 0, device: int = 0):
+        self.lib = _lib()
+        self.channels = tuple(int(c) for c in channels)
+        self.dilations = tuple(int(d) for d in dilations)
+        ch = (C.c_int * len(self.channels))(*self.channels)
+        dl = (C.c_int * len(self.dilations))(*self.dilations)
+        h = C.c_void_p()
+        rc = self.lib.mdft_wsnet_create(len(self.dilations), ch, dl, int(seed), int(device), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"mdft_wsnet_create: {self.lib.mdft_wsnet_last_error(None).decode()}")
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"wsnet error {rc}: {self.lib.mdft_wsnet_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self.lib.mdft_wsnet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_params(self) -> int:
+        return int(self.lib.mdft_wsnet_num_params(self._h))
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        self._check(self.lib.mdft_wsnet_get_params(self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self._check(self.lib.mdft_wsnet_set_params(self._h, p.ctypes.data))
+
+    def get_grads(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        self._check(self.lib.mdft_wsnet_get_grads(self._h, out.ctypes.data))
+        return out
+
+    def layers(self, p=None):
+        """Split a parameter vector into [(W [co][ci][3][3][3], b [co])]."""
+        p = self.get_params() if p is None else p
+        out, off = [], 0
+        for ci, co in zip(self.channels[:-1], self.channels[1:]):
+            w = p[off:off + co * ci * 27].reshape(co, ci, 3, 3, 3)
+            off += co * ci * 27
+            out.append((w, p[off:off + co]))
+            off += co
+        return out
+
+    def forward(self, n, d_in, d_out) -> None:
+        self._check(self.lib.mdft_wsnet_forward(self._h, n[0], n[1], n[2], _ptr(d_in), _ptr(d_out)))
+
+    def loss_grad(self, n, d_in, d_target, d_mask=None, d_gin=None) -> float:
+        loss = C.c_double()
+        self._check(self.lib.mdft_wsnet_loss_grad(self._h, n[0], n[1], n[2], _ptr(d_in), _ptr(d_target),
+                                                  _ptr(d_mask), _ptr(d_gin), C.byref(loss)))
+        return loss.value
+
+    def adam(self, lr: float) -> None:
+        self._check(self.lib.mdft_wsnet_adam(self._h, float(lr)))
+
+    def save(self, path: str, **meta) -> None:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.savez(path, params=self.get_params(), channels=np.array(self.channels), dilations=np.array(self.dilations),
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path: str = DEFAULT_WEIGHTS, device: int = 0) -> "WarmStartNet":
+        with np.load(path, allow_pickle=False) as z:
+            net = cls(tuple(z["channels"]), tuple(z["dilations"]), device=device)
+            net.set_params(z["params"])
+        return net
+
+
+def _ptr(b):
+    if b is None:
+        return None
+    return b.ptr if isinstance(b, DeviceBuffer) else b
+
+
+def features(eng: Engine) -> DeviceBuffer:
+    """[8][N] FP32 features of the engine's vext."""
+    out = DeviceBuffer(eng, 4 * 8 * eng.N)
+    eng._check(_lib().mdft_ws_features(eng._h, out.ptr))
+    return out
+
+
+def gamma(eng: Engine, xi: DeviceBuffer, want_field: bool = False):
+    """m <= 1 projections [4][N] (FP32) of gamma = c (*) drho for xi; with
+    want_field also the field gamma(r, Omega) [N][no] (FP64)."""
+    g = DeviceBuffer(eng, 8 * eng.N * eng.no)
+    p4 = DeviceBuffer(eng, 4 * 4 * eng.N)
+    eng._check(_lib().mdft_ws_gamma(eng._h, xi.ptr, g.ptr, p4.ptr))
+    if want_field:
+        return p4, g
+    g.free()
+    return p4
+
+
+def guess(eng: Engine, pred: DeviceBuffer, xi: DeviceBuffer) -> None:
+    eng._check(_lib().mdft_ws_guess(eng._h, pred.ptr, xi.ptr))
+
+
+def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer) -> None:
+    """xi <- the network's guess for the engine's vext."""
+    f = features(eng)
+    pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
+    net.forward(eng.n, f, pred)
+    guess(eng, pred, xi)
+    f.free()
+    pred.free()
+
+
+def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4, precision: str = "fp32",
+                      maxeval: int = 600, log=None):
+    """Solve `nsolutes` seeded solutes of workload `w` (grid, box, mmax, c(k))
+    with the plain method from the reference cold start; returns the engine and
+    per-solute (features, targets, evaluations, F) with device-resident fields."""
+    from . import workloads
+    from .minimize import minimize
+    eng = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, precision)
+    eng.set_ck(ck_table)
+    xi = eng.alloc_field()
+    g = eng.alloc_field()
+    out = []
+    for i in range(nsolutes):
+        t0 = time.perf_counter()
+        sol = workloads.make_solute(w, seed + i)
+        eng.compute_vext(sol, w.solvent)
+        feat = features(eng)
+        xi.upload(workloads.guess_xi(eng.get_vext(), w.kT))
+        res = minimize(eng, xi, g=g, m=5, gtol=gtol, maxeval=maxeval, fnoise=1e-6)
+        tgt = gamma(eng, xi)
+        out.append((feat, tgt, res.evaluations, res.F))
+        if log:
+            log(f"  solute {i}: {len(sol.q)} atoms, F={res.F:.3f}, evaluations={res.evaluations}, "
+                f"status={res.status}, {time.perf_counter() - t0:.2f}s")
+    return eng, out
+
+
+def train(net: WarmStartNet, n, data, epochs: int, lr: float = 1e-3, seed: int = 0, log=None):
+    """Adam over the samples in a seeded random order; returns per-epoch mean loss."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    for ep in range(epochs):
+        order = rng.permutation(len(data))
+        tot = 0.0
+        for i in order:
+            feat, tgt = data[i][0], data[i][1]
+            tot += net.loss_grad(n, feat, tgt)
+            net.adam(lr)
+        hist.append(tot / len(data))
+        if log:
+            log(f"  epoch {ep}: loss {hist[-1]:.5f}")
+    return hist

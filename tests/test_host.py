@@ -51,3 +51,13 @@ def test_vext_orientation_independent_lj_only():
     # a neutral LJ sphere acts on the oxygen site only, which sits at the molecule origin
     assert np.allclose(v, v[:, :1])
     assert v.max() <= 694.7 + 1e-9
+
+
+def test_library_exports_every_warm_start_symbol():
+    from mdft.warmstart import WSNET_EXPORTS
+    lib = load_library()
+    txt = open(os.path.join(ROOT, "include", "mdft_wsnet.h")).read()
+    names = sorted(set(re.findall(r"\b(mdft_[a-z0-9_]+)\s*\(", txt)))
+    assert set(names) == set(WSNET_EXPORTS)
+    for n in names:
+        assert hasattr(lib, n), n
