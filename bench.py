@@ -100,7 +100,7 @@ def main():
     ap.add_argument("--gtol", type=float, default=1e-6)
     ap.add_argument("--maxeval", type=int, default=1500)
     ap.add_argument("--mode", choices=["converge", "eval"], default="converge")
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp64",
                     help="evaluation precision for the rate/phase measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--weights", default=os.path.join(PKG, "mdft", "data", "warmstart_solute20.npz"),
@@ -179,38 +179,40 @@ def main():
         if args.precision != "fp64":
             raise SystemExit("bench.py: converge mode measures the FP64 reference itself; use --precision fp64")
         band = 0.05
+        from mdft import warmstart as wsm
         # ---- converged FP64 reference: cold start to |dF|/|dF0| < gtol
-        xi.upload(xi0)
+        wsm.cold_start(eng, xi)
         eng.synchronize()
         t0 = time.perf_counter()
         res = minimize(eng, xi, g=df, m=5, gtol=args.gtol, maxeval=args.maxeval, fnoise=1e-7)
         eng.synchronize()
         t_conv = time.perf_counter() - t0
         dg_ref = res.F
-        # ---- plain FP64 path, timed to the band
-        xi.upload(xi0)
+        # ---- plain FP64 path (cold start on the device), timed to the band
         eng.synchronize()
         t0 = time.perf_counter()
+        wsm.cold_start(eng, xi)
         res_p = minimize(eng, xi, g=df, m=5, gtol=0.0, maxeval=args.maxeval, fnoise=1e-7,
                          ftarget=dg_ref, fband=band)
         eng.synchronize()
         t_plain = time.perf_counter() - t0
         # ---- fast path: learned warm start + FP32 evaluations to the band,
         # confirmed by one FP64 evaluation of the final iterate
-        from mdft import warmstart as wsm
         e32 = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, "fp32")
         e32.set_ck(ckt)
         e32.compute_vext(sol, w.solvent)
         net = wsm.WarmStartNet.load(args.weights) if os.path.exists(args.weights) else None
+        scratch = e32.alloc_field() if net is not None else None
+        info = {}
 
         def fast(use_net):
             e32.synchronize()
             eng.synchronize()
             t0 = time.perf_counter()
             if use_net:
-                wsm.warm_start(e32, net, xi)
+                info.update(wsm.warm_start(e32, net, xi, fallback=scratch))
             else:
-                xi.upload(xi0)
+                wsm.cold_start(e32, xi)
             r = minimize(e32, xi, g=df, m=5, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref, fband=band)
             c, _ = eng.energy_and_gradient(xi, None)
             eng.synchronize()
@@ -239,7 +241,7 @@ def main():
             "plain_evaluations": res_p.evaluations, "plain_status": res_p.status,
             "fp32_cold_evaluations": res_c32.evaluations,
             "fast_evaluations": res_f.evaluations, "fast_iterations": res_f.iterations,
-            "fast_status": res_f.status,
+            "fast_status": res_f.status, "warm_start": info,
         })
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = cpu_baseline(w, res_p.evaluations)

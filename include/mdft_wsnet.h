@@ -63,7 +63,10 @@ int mdft_ws_features(struct mdft_ctx *ctx, float *d_feat);
  * minimum.  d_proj4 [4][N] (FP32, may be NULL): its projections c_k(r). */
 int mdft_ws_gamma(struct mdft_ctx *ctx, const double *d_xi, double *d_gamma, float *d_proj4);
 
-/* d_xi [N][no] = exp((-beta vext + sum_k pred_k(r) B_k(Omega)) / 2), exponent clipped to [-200, 30]. */
+/* d_xi [N][no] = exp((-beta vext + sum_k pred_k(r) B_k(Omega)) / 2), exponent
+ * clipped to [-200, 7].  With d_pred NULL: the reference's cold start
+ * (src/module_density.f90:267-284), xi = 0 where vext >= (-ln eps - 0.05) kT,
+ * else 1, computed on the device. */
 int mdft_ws_guess(struct mdft_ctx *ctx, const float *d_pred, double *d_xi);
 
 #ifdef __cplusplus

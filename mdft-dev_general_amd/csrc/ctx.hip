@@ -212,7 +212,16 @@ int ready(mdft_ctx *c) {
 }
 
 size_t proj_elem_bytes(const mdft_ctx *c) {
-    return c->d.precision == MDFT_PREC_FP32 ? sizeof(float2) : sizeof(double2);
+    return c->d.precision == MDFT_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+}
+
+// final pass (proj2angl + F + dF) of the context's precision
+hipError_t launch_final(mdft_ctx *c, const double *xi, double *df, int mode, int *nb) {
+    switch (c->d.precision) {
+        case MDFT_PREC_FP32: return launch_final_f32(c, static_cast<const float2 *>(c->d_proj), xi, df, mode, nb);
+        case MDFT_PREC_BF16: return launch_final_bf16(c, static_cast<const float2 *>(c->d_proj), xi, df, mode, nb);
+        default: return launch_final_f64(c, static_cast<const double2 *>(c->d_proj), xi, df, mode, nb);
+    }
 }
 
 hipError_t ensure_stage(mdft_ctx *c, double **buf) {
@@ -222,11 +231,13 @@ hipError_t ensure_stage(mdft_ctx *c, double **buf) {
 
 // Runs the excess-term projection pipeline up to `stage` (1..4) on device xi.
 int run_excess_pipeline(mdft_ctx *c, const double *xi, int stage, bool timers) {
-    const bool f32 = c->d.precision == MDFT_PREC_FP32;
+    const bool f32 = c->d.precision != MDFT_PREC_FP64;
+    const bool bf16 = c->d.precision == MDFT_PREC_BF16;
     double2 *p64 = static_cast<double2 *>(c->d_proj);
     float2 *p32 = static_cast<float2 *>(c->d_proj);
     if (timers) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, f32 ? launch_angl2proj_f32(c, xi, p32) : launch_angl2proj_f64(c, xi, p64));
+    HIPCHK(c, bf16 ? launch_angl2proj_bf16(c, xi, p32)
+                   : (f32 ? launch_angl2proj_f32(c, xi, p32) : launch_angl2proj_f64(c, xi, p64)));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (stage >= 2) HIPCHK(c, f32 ? launch_fft3d_f32(c, p32, +1) : launch_fft3d_f64(c, p64, +1));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -276,12 +287,12 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
         g_create_error = "invalid grid/solvent description";
         return MDFT_EINVAL;
     }
-    if (d.precision != MDFT_PREC_FP64 && d.precision != MDFT_PREC_FP32) {
-        g_create_error = "precision must be MDFT_PREC_FP64 or MDFT_PREC_FP32 (MDFT_PREC_BF16 is not available)";
+    if (d.precision != MDFT_PREC_FP64 && d.precision != MDFT_PREC_FP32 && d.precision != MDFT_PREC_BF16) {
+        g_create_error = "precision must be MDFT_PREC_FP64, MDFT_PREC_FP32 or MDFT_PREC_BF16";
         return MDFT_EINVAL;
     }
-    if (d.precision == MDFT_PREC_FP32 && (d.mrso != 2 || d.mmax > 5)) {
-        g_create_error = "MDFT_PREC_FP32 needs molecular symmetry order 2 and mmax <= 5";
+    if (d.precision != MDFT_PREC_FP64 && (d.mrso != 2 || d.mmax > 5)) {
+        g_create_error = "MDFT_PREC_FP32 and MDFT_PREC_BF16 need molecular symmetry order 2 and mmax <= 5";
         return MDFT_EINVAL;
     }
     if ((long long)d.nx * d.ny * d.nz >= (1LL << 31)) {
@@ -393,6 +404,9 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
         int lim = 0;
         if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, d.device) == hipSuccess && lim > 0)
             c->lds_limit = (size_t)lim;
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d.device) == hipSuccess && ncu > 0)
+            c->num_cus = ncu;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bad(MDFT_EHIP, "hipStreamCreate failed");
@@ -417,7 +431,11 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
     ok = ok && hipMalloc(&c->d_sums, sizeof(double) * 8) == hipSuccess;
     ok = ok && hipHostMalloc(&c->h_sums, sizeof(double) * 8, hipHostMallocDefault) == hipSuccess;
     if (!ok) return bad(MDFT_ENOMEM, "device allocation failed");
-    if (d.precision == MDFT_PREC_FP32) {
+    if (d.precision == MDFT_PREC_BF16) {
+        const int rc = bf16_tables(c, a2p, p2a, fmv);
+        if (rc != MDFT_OK) return bad(rc, "BF16 transform tables");
+    }
+    if (d.precision != MDFT_PREC_FP64) {
         std::vector<float> ftab;
         std::vector<int> lanerow;
         if (oz32_tables(c, ftab, lanerow) != MDFT_OK) return bad(MDFT_EINVAL, "too many projections per lane");
@@ -458,7 +476,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_oz32_ftab, c->d_oz32_lanerow,
-                    c->d_wsB, c->d_wsBw};
+                    c->d_wsB, c->d_wsBw, c->d_bf16_M, c->d_bf16_Q};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);
@@ -620,10 +638,7 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     rc = run_excess_pipeline(c, xd, 4, timers);
     if (rc) return rc;
     int nb = 0;
-    if (c->d.precision == MDFT_PREC_FP32)
-        HIPCHK(c, launch_final_f32(c, static_cast<const float2 *>(c->d_proj), xd, dd, mode, &nb));
-    else
-        HIPCHK(c, launch_final_f64(c, static_cast<const double2 *>(c->d_proj), xd, dd, mode, &nb));
+    HIPCHK(c, launch_final(c, xd, dd, mode, &nb));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     rc = read_sums(c, nb, sums);
     if (rc) return rc;
@@ -848,17 +863,14 @@ int mdft_ws_gamma(mdft_ctx *c, const double *d_xi, double *d_gamma, float *d_pro
     if ((rc = ensure_ws_basis(c))) return rc;
     if ((rc = run_excess_pipeline(c, d_xi, 4, false))) return rc;
     int nb = 0;
-    if (c->d.precision == MDFT_PREC_FP32)
-        HIPCHK(c, launch_final_f32(c, static_cast<const float2 *>(c->d_proj), d_xi, d_gamma, 2, &nb));
-    else
-        HIPCHK(c, launch_final_f64(c, static_cast<const double2 *>(c->d_proj), d_xi, d_gamma, 2, &nb));
+    HIPCHK(c, launch_final(c, d_xi, d_gamma, 2, &nb));
     if (d_proj4) HIPCHK(c, launch_ws_project(c, d_gamma, d_proj4, 1));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }
 
 int mdft_ws_guess(mdft_ctx *c, const float *d_pred, double *d_xi) {
-    if (!c || !d_pred || !d_xi) return MDFT_EINVAL;
+    if (!c || !d_xi) return MDFT_EINVAL;
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
     int rc = ensure_ws_basis(c);
     if (rc) return rc;

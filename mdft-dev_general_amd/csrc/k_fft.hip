@@ -190,6 +190,93 @@ __global__ __launch_bounds__(kThreads) void fft_axis(V *__restrict__ data, int n
     }
 }
 
+
+// ---- line lengths known at compile time: constant divisors and trip counts,
+// twiddles staged in LDS once per workgroup.
+constexpr int pick_radix(int rem) {
+    return (rem % 8 == 0 && rem > 8) ? 8 : (rem % 4 == 0 ? 4 : (rem % 2 == 0 ? 2 : (rem % 3 == 0 ? 3 : 5)));
+}
+
+template <int R, int N, int Ns, int PB, typename V>
+__device__ inline void stage_c(const V *src, V *dst, const V *twl, decltype(V::x) s) {
+    constexpr int nb = N / R, ld = N + 1, tstep = N / (Ns * R);
+    for (int task = threadIdx.x; task < PB * nb; task += kThreads) {
+        const int row = task / nb, j = task - row * nb;
+        const V *in = src + row * ld;
+        V *out = dst + row * ld;
+        const int k = j % Ns;
+        V v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            V x = in[j + r * nb];
+            if (r > 0 && Ns > 1) {
+                const V w = twl[(k * r * tstep) % N];
+                x = cmul(x, mk<V>(w.x, w.y * s));
+            }
+            v[r] = x;
+        }
+        Dft<R, V>::run(v, s);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) out[base + r * Ns] = v[r];
+    }
+}
+
+template <int N, int Ns, int REM, int PB, typename V>
+__device__ inline const V *stages_c(V *src, V *dst, const V *twl, decltype(V::x) s) {
+    if constexpr (REM == 1) {
+        return src;
+    } else {
+        constexpr int R = pick_radix(REM);
+        stage_c<R, N, Ns, PB, V>(src, dst, twl, s);
+        __syncthreads();
+        return stages_c<N, Ns * R, REM / R, PB, V>(dst, src, twl, s);
+    }
+}
+
+template <int N, int PB, typename V>
+__global__ __launch_bounds__(kThreads) void fft_axis_c(V *__restrict__ data, int np, LineGeom g, const double2 *tw,
+                                                      double sgn_d, double scale_d) {
+    using T = decltype(V::x);
+    extern __shared__ __align__(16) unsigned char fft_smem[];
+    V *A = reinterpret_cast<V *>(fft_smem), *B = A + PB * (N + 1), *twl = B + PB * (N + 1);
+    const T sgn = (T)sgn_d, scale = (T)scale_d;
+    for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = mk<V>((T)tw[i].x, (T)tw[i].y);
+    const int line = blockIdx.x;
+    const int a = line % g.na, b = line / g.na;
+    const long long base = (long long)a * g.sa + (long long)b * g.sb;
+    const int p0 = blockIdx.y * PB;
+    const int pb = (np - p0) < PB ? (np - p0) : PB;
+    for (int idx = threadIdx.x; idx < N * PB; idx += kThreads) {
+        const int p = idx % PB, e = idx / PB;
+        if (p < pb) A[p * (N + 1) + e] = data[(base + (long long)e * g.es) * np + p0 + p];
+    }
+    __syncthreads();
+    const V *res = stages_c<N, 1, N, PB, V>(A, B, twl, sgn);
+    for (int idx = threadIdx.x; idx < N * PB; idx += kThreads) {
+        const int p = idx % PB, e = idx / PB;
+        if (p < pb) {
+            V v = res[p * (N + 1) + e];
+            v.x *= scale;
+            v.y *= scale;
+            data[(base + (long long)e * g.es) * np + p0 + p] = v;
+        }
+    }
+}
+
+template <int PB, typename V>
+bool launch_axis_c(int n, dim3 grid, hipStream_t st, V *proj, int np, const LineGeom &g, const double2 *tw,
+                   double sign, double scale) {
+    const size_t lds = sizeof(V) * (2 * (size_t)PB * (n + 1) + n);
+    switch (n) {
+#define FFT_C(NN) \
+    case NN: hipLaunchKernelGGL((fft_axis_c<NN, PB, V>), grid, dim3(kThreads), lds, st, proj, np, g, tw, sign, scale); return true;
+        FFT_C(16) FFT_C(32) FFT_C(48) FFT_C(64) FFT_C(96) FFT_C(128) FFT_C(192) FFT_C(256)
+#undef FFT_C
+        default: return false;
+    }
+}
+
 template <typename V>
 hipError_t launch_fft3d(mdft_ctx *c, V *proj, int sign) {
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, np = c->np;
@@ -208,6 +295,11 @@ hipError_t launch_fft3d(mdft_ctx *c, V *proj, int sign) {
         dim3 grid((unsigned)lines, (unsigned)((np + PB - 1) / PB));
         const double scale = (ax == 2) ? scale_last : 1.0;
         const size_t lds = sizeof(V) * 2 * (size_t)PB * (g.n + 1);
+        if (PB == PBmax && launch_axis_c<PBmax, V>(g.n, grid, c->stream, proj, np, g, c->fft[ax].tw, (double)sign, scale)) {
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            continue;
+        }
         if (PB == PBmax)
             hipLaunchKernelGGL((fft_axis<PBmax, V>), grid, dim3(kThreads), lds, c->stream, proj, np, g, c->fft[ax],
                                (double)sign, scale);

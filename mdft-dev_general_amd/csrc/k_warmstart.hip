@@ -6,6 +6,8 @@
 // cos t) -- the m <= 1 projections for molecular symmetry order 2.  The
 // quadrature weights and 1 / sum_io w_io B_k^2 are folded into the table Bw
 // on the host, so a projection is a plain dot product over orientations.
+#include <cfloat>
+
 #include "mdft_internal.h"
 
 namespace {
@@ -38,18 +40,25 @@ __global__ __launch_bounds__(kThreads) void ws_project(const double *__restrict_
     }
 }
 
-// xi = exp((-beta vext + sum_k pred_k B_k) / 2), exponent clipped to [-200, 30]
+// xi = exp((-beta vext + sum_k pred_k B_k) / 2), exponent clipped to [-200, 7];
+// without pred the reference cold start (src/module_density.f90:267-284):
+// xi = 0 where vext >= (-ln(DBL_EPSILON) - 0.05) kT, 1 elsewhere
 __global__ __launch_bounds__(kThreads) void ws_guess(const double *__restrict__ vext, const float *__restrict__ pred,
                                                     const double *__restrict__ B, double *__restrict__ xi, size_t N,
                                                     int no, double beta) {
     const size_t total = N * (size_t)no;
+    const double vmax = -log(DBL_EPSILON) - 0.05;
     for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads) {
+        if (!pred) {
+            xi[g] = (beta * vext[g] >= vmax) ? 0.0 : 1.0;
+            continue;
+        }
         const size_t p = g / no;
         const int io = (int)(g - p * no);
         double lg = -beta * vext[g];
 #pragma unroll
         for (int k = 0; k < 4; k++) lg += (double)pred[(size_t)k * N + p] * B[(size_t)k * no + io];
-        lg = fmin(fmax(lg, -200.0), 30.0);
+        lg = fmin(fmax(lg, -200.0), 7.0);
         xi[g] = exp(0.5 * lg);
     }
 }

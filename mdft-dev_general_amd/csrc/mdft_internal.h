@@ -72,6 +72,8 @@ struct mdft_ctx {
     int *d_oz32_lanerow = nullptr; // [MDFT_OZ32_ROWS_PER_LANE][64] OZ rows per lane
     double *d_wsB = nullptr;      // [4][no] low angular basis of the warm start
     double *d_wsBw = nullptr;     // [4][no] the same times w_io / sum_io w_io B_k^2
+    void *d_bf16_M = nullptr;     // angl2proj matrix, BF16 MFMA fragment order
+    void *d_bf16_Q = nullptr;     // proj2angl matrix, BF16 MFMA fragment order
     PairSchedule pairs;
     // fields
     double *d_vext = nullptr;     // [N][no]
@@ -87,6 +89,7 @@ struct mdft_ctx {
     bool have_vext = false, have_ck = false;
     hipStream_t stream = nullptr;
     size_t lds_limit = 65536;      // max dynamic LDS per workgroup on this device
+    int num_cus = 0;               // compute units of the device
     hipEvent_t ev[8]{};
     bool timers_ready = false;
     double phase_ms[8]{};
@@ -112,6 +115,10 @@ hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
 hipError_t launch_ws_project(mdft_ctx *c, const double *f, float *out, int mode);
 hipError_t launch_ws_guess(mdft_ctx *c, const float *pred, double *xi);
+int bf16_tables(mdft_ctx *c, const std::vector<double> &a2p, const std::vector<double> &p2a,
+                const std::vector<double> &fm);
+hipError_t launch_angl2proj_bf16(mdft_ctx *c, const double *xi, float2 *proj);
+hipError_t launch_final_bf16(mdft_ctx *c, const float2 *proj, const double *xi, double *df, int mode, int *nblocks_out);
 int fft_make_plan(int n, int *radix);
 hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out);
 hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out);

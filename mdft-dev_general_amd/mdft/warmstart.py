@@ -179,18 +179,36 @@ def gamma(eng: Engine, xi: DeviceBuffer, want_field: bool = False):
     return p4
 
 
-def guess(eng: Engine, pred: DeviceBuffer, xi: DeviceBuffer) -> None:
-    eng._check(_lib().mdft_ws_guess(eng._h, pred.ptr, xi.ptr))
+def guess(eng: Engine, pred: DeviceBuffer | None, xi: DeviceBuffer) -> None:
+    eng._check(_lib().mdft_ws_guess(eng._h, _ptr(pred), xi.ptr))
 
 
-def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer) -> None:
-    """xi <- the network's guess for the engine's vext."""
+def cold_start(eng: Engine, xi: DeviceBuffer) -> None:
+    """xi <- the reference cold start (src/module_density.f90:267-284), on the device."""
+    guess(eng, None, xi)
+
+
+def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: DeviceBuffer | None = None) -> dict:
+    """xi <- the network's guess for the engine's vext.  With a scratch field
+    `fallback`, F is evaluated at the guess and at the cold start and the lower
+    one is kept: the cold start stays the fallback, so a poor prediction costs
+    one evaluation and never a worse start."""
     f = features(eng)
     pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
     net.forward(eng.n, f, pred)
     guess(eng, pred, xi)
     f.free()
     pred.free()
+    info = {"start": "learned"}
+    if fallback is not None:
+        fw = float(np.sum(eng.energy_and_gradient(xi, None)[0]))
+        cold_start(eng, fallback)
+        fc = float(np.sum(eng.energy_and_gradient(fallback, None)[0]))
+        info.update(F_learned=fw, F_cold=fc)
+        if not fw <= fc:
+            eng._check(eng.lib.mdft_vec_lincomb(eng._h, eng.N * eng.no, 1.0, fallback.ptr, 0.0, None, xi.ptr))
+            info["start"] = "cold (fallback)"
+    return info
 
 
 def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4, precision: str = "fp32",
@@ -210,7 +228,7 @@ def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4,
         sol = workloads.make_solute(w, seed + i)
         eng.compute_vext(sol, w.solvent)
         feat = features(eng)
-        xi.upload(workloads.guess_xi(eng.get_vext(), w.kT))
+        cold_start(eng, xi)
         res = minimize(eng, xi, g=g, m=5, gtol=gtol, maxeval=maxeval, fnoise=1e-6)
         tgt = gamma(eng, xi)
         out.append((feat, tgt, res.evaluations, res.F))

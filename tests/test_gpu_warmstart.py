@@ -128,8 +128,13 @@ def test_gamma_field_features_and_guess(oracle_mod):
     bp = DeviceBuffer(eng, pred.nbytes)
     bp.upload(pred)
     ws.guess(eng, bp, bx)
-    lg = np.clip(-bv + pred.T.astype(np.float64) @ B, -200.0, 30.0)
-    assert np.abs(bx.download() - np.exp(0.5 * lg).ravel()).max() < 1e-12 * np.exp(15)
+    lg = np.clip(-bv + pred.T.astype(np.float64) @ B, -200.0, 7.0)
+    assert np.abs(bx.download() - np.exp(0.5 * lg).ravel()).max() < 1e-12 * np.exp(3.5)
+    # device cold start = the host restatement of the reference's guess
+    vext[::7] = 700.0
+    eng.set_vext(vext)
+    ws.cold_start(eng, bx)
+    assert np.array_equal(bx.download(), workloads.guess_xi(vext, 2.4777))
 
 
 def test_euler_lagrange_at_the_minimum():
@@ -168,8 +173,9 @@ def _tiny_solute(rng, center):
 
 def test_trained_network_reduces_iterations():
     """Train on 16 solved solutes (32^3, 12 A box, mmax=3), then compare L-BFGS
-    evaluations to |dF|/|dF0| < 1e-5 from the cold start and from the
-    network's guess on 4 held-out solutes; both reach the same F."""
+    evaluations to |dF|/|dF0| < 1e-6 from the cold start and from the
+    network's guess (with the cold-start fallback, counted) on 4 held-out
+    solutes; both reach the same F."""
     from mdft import warmstart as ws
     from mdft.minimize import minimize
     w = workloads.Workload("ws_test", 3, (32, 32, 32), (12.0, 12.0, 12.0), 3, 512)
@@ -179,12 +185,12 @@ def test_trained_network_reduces_iterations():
     eng.set_ck(table)
     rng = np.random.default_rng(11)
     sols = [_tiny_solute(rng, np.array(w.length) / 2) for _ in range(20)]
-    xi, gb = eng.alloc_field(), eng.alloc_field()
+    xi, gb, tmp = eng.alloc_field(), eng.alloc_field(), eng.alloc_field()
     data = []
     for s in sols[:16]:
         eng.compute_vext(s, w.solvent)
         f = ws.features(eng)
-        xi.upload(workloads.guess_xi(eng.get_vext(), w.kT))
+        ws.cold_start(eng, xi)
         minimize(eng, xi, g=gb, m=5, gtol=1e-5, maxeval=800)
         data.append((f, ws.gamma(eng, xi)))
     net = ws.WarmStartNet(seed=1)
@@ -194,13 +200,14 @@ def test_trained_network_reduces_iterations():
     cold, warm = [], []
     for s in sols[16:]:
         eng.compute_vext(s, w.solvent)
-        xi.upload(workloads.guess_xi(eng.get_vext(), w.kT))
-        rc = minimize(eng, xi, g=gb, m=5, gtol=1e-5, maxeval=800)
-        ws.warm_start(eng, net, xi)
-        rw = minimize(eng, xi, g=gb, m=5, gtol=1e-5, maxeval=800)
-        say(f"  cold: {rc.evaluations} evaluations F={rc.F:.5f}; warm: {rw.evaluations} evaluations F={rw.F:.5f}")
+        ws.cold_start(eng, xi)
+        rc = minimize(eng, xi, g=gb, m=5, gtol=1e-6, maxeval=800)
+        info = ws.warm_start(eng, net, xi, fallback=tmp)
+        rw = minimize(eng, xi, g=gb, m=5, gtol=1e-6, maxeval=800)
+        say(f"  cold: {rc.evaluations} evaluations F={rc.F:.5f}; warm ({info['start']}, F0 {info['F_learned']:.3g} "
+            f"vs cold {info['F_cold']:.3g}): {rw.evaluations}+2 evaluations F={rw.F:.5f}")
         assert abs(rc.F - rw.F) < 0.05   # same minimum (kJ/mol)
         cold.append(rc.evaluations)
-        warm.append(rw.evaluations)
+        warm.append(rw.evaluations + 2)
     say(f"  mean evaluations cold {np.mean(cold):.1f}, warm {np.mean(warm):.1f}")
     assert np.mean(warm) < np.mean(cold)
