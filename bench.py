@@ -143,7 +143,7 @@ def main():
         eng.energy_and_gradient(xi, df, timers=True)
         ph.append(eng.phase_times())
     ph = np.median(np.array(ph), axis=0)
-    cbytes = 16 if args.precision == "fp64" else 8
+    cbytes = 16 if args.precision == "fp64" else 8  # projection field: complex FP64 or FP32
     pbytes = phase_bytes(eng.N, eng.no, eng.np_, cbytes)
     dom = max(pbytes, key=lambda k: ph[PHASES.index(k)])
     dom_ms = float(ph[PHASES.index(dom)])
@@ -157,7 +157,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 (FP64 reductions, ideal term, L-BFGS)",
+        "dtype": {"fp64": "f64", "fp32": "f32 (FP64 reductions, ideal term, L-BFGS)",
+                  "bf16": "bf16 MFMA angular transforms, f32 FFT/OZ (FP64 reductions, ideal term, L-BFGS)"}[args.precision],
         "data": "synthetic (seeded 20-atom solute, seeded c(k) with the convex self term, reference cold start)",
         "config": {"workload": w.name, "grid": list(w.n), "box_A": list(w.length), "mmax": w.mmax,
                    "orientations": eng.no, "projections": eng.np_, "natoms": int(len(sol.q))},
