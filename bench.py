@@ -99,7 +99,9 @@ def main():
     ap.add_argument("--workload", default="solute20_128")
     ap.add_argument("--gtol", type=float, default=1e-6)
     ap.add_argument("--maxeval", type=int, default=1500)
-    ap.add_argument("--mode", choices=["converge", "eval"], default="converge")
+    ap.add_argument("--mode", choices=["converge", "eval", "copy"], default="converge",
+                    help="copy: a plain device copy of 1 GiB (z = x through mdft_vec_lincomb), the "
+                         "reference kernel for calibrating rocprof's FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp64",
                     help="evaluation precision for the rate/phase measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -115,6 +117,22 @@ def main():
     from mdft.minimize import minimize
 
     w = workloads.WORKLOADS[args.workload]
+    if args.mode == "copy":
+        eng = Engine((8, 8, 8), (8.0, 8.0, 8.0), 1, 2, w.kT, w.n0, "fp64")
+        n = 1 << 27  # doubles: 1 GiB read + 1 GiB written per launch
+        from mdft.gpu import DeviceBuffer
+        x, z = DeviceBuffer(eng, 8 * n), DeviceBuffer(eng, 8 * n)
+        for _ in range(args.warmup + args.steps):
+            eng._check(eng.lib.mdft_vec_lincomb(eng._h, n, 1.0, x.ptr, 0.0, None, z.ptr))
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            eng._check(eng.lib.mdft_vec_lincomb(eng._h, n, 1.0, x.ptr, 0.0, None, z.ptr))
+        eng.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"mode": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "ms": dt * 1e3,
+                          "GBps": 16 * n / dt / 1e9}), flush=True)
+        return
     sol = w.solute()
     t_setup = time.perf_counter()
     ckt = w.ck_table()

@@ -476,7 +476,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_oz32_ftab, c->d_oz32_lanerow,
-                    c->d_wsB, c->d_wsBw, c->d_bf16_M, c->d_bf16_Q};
+                    c->d_bf16_M, c->d_bf16_Q};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);
@@ -822,49 +822,29 @@ int mdft_synchronize(mdft_ctx *c) {
 
 // ---- learned warm start: coupling to the context (no reference counterpart)
 
-static int ensure_ws_basis(mdft_ctx *c) {
-    if (c->d_wsB) return MDFT_OK;
-    const int no = c->no;
-    std::vector<double> B(4 * (size_t)no), Bw(4 * (size_t)no);
-    const double pi = acos(-1.0);
-    double nrm[4] = {0, 0, 0, 0};
-    for (int it = 0; it < c->ntheta; it++)
-        for (int iphi = 0; iphi < c->nphi; iphi++)
-            for (int ipsi = 0; ipsi < c->npsi; ipsi++) {
-                const int io = ipsi + c->npsi * (iphi + c->nphi * it);
-                const double th = c->theta[it], ph = 2.0 * pi * iphi / c->nphi;
-                const double b[4] = {1.0, sin(th) * cos(ph), sin(th) * sin(ph), cos(th)};
-                for (int k = 0; k < 4; k++) {
-                    B[(size_t)k * no + io] = b[k];
-                    nrm[k] += c->w[io] * b[k] * b[k];
-                }
-            }
-    for (int k = 0; k < 4; k++)
-        for (int io = 0; io < no; io++) Bw[(size_t)k * no + io] = c->w[io] * B[(size_t)k * no + io] / nrm[k];
-    HIPCHK(c, upload(c, &c->d_wsB, B.data(), B.size()));
-    HIPCHK(c, upload(c, &c->d_wsBw, Bw.data(), Bw.size()));
-    return MDFT_OK;
-}
-
 int mdft_ws_features(mdft_ctx *c, float *d_feat) {
     if (!c || !d_feat) return MDFT_EINVAL;
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
-    int rc = ensure_ws_basis(c);
-    if (rc) return rc;
-    HIPCHK(c, launch_ws_project(c, c->d_vext, d_feat, 0));
+    if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the warm start needs molecular symmetry order 2");
+    HIPCHK(c, launch_angl2proj_feat(c, c->d_vext));
+    HIPCHK(c, launch_proj_to_channels(c, d_feat, 1.0f));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }
 
-int mdft_ws_gamma(mdft_ctx *c, const double *d_xi, double *d_gamma, float *d_proj4) {
+int mdft_ws_gamma(mdft_ctx *c, const double *d_xi, double *d_gamma, float *d_target) {
     int rc = ready(c);
     if (rc) return rc;
-    if (!d_xi || !d_gamma) return fail(c, MDFT_EINVAL, "null argument");
-    if ((rc = ensure_ws_basis(c))) return rc;
+    if (!d_xi) return fail(c, MDFT_EINVAL, "null argument");
     if ((rc = run_excess_pipeline(c, d_xi, 4, false))) return rc;
-    int nb = 0;
-    HIPCHK(c, launch_final(c, d_xi, d_gamma, 2, &nb));
-    if (d_proj4) HIPCHK(c, launch_ws_project(c, d_gamma, d_proj4, 1));
+    // gamma = -(prefactor / kT) proj2angl(P4), prefactor of cproj_mrso.f90:728
+    const double pi = acos(-1.0);
+    const float scale = (float)(2.0 * 4.0 * pi * pi / c->d.mrso / c->d.n0);
+    if (d_target) HIPCHK(c, launch_proj_to_channels(c, d_target, scale));
+    if (d_gamma) {
+        int nb = 0;
+        HIPCHK(c, launch_final(c, d_xi, d_gamma, 2, &nb));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }
@@ -872,9 +852,13 @@ int mdft_ws_gamma(mdft_ctx *c, const double *d_xi, double *d_gamma, float *d_pro
 int mdft_ws_guess(mdft_ctx *c, const float *d_pred, double *d_xi) {
     if (!c || !d_xi) return MDFT_EINVAL;
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
-    int rc = ensure_ws_basis(c);
-    if (rc) return rc;
-    HIPCHK(c, launch_ws_guess(c, d_pred, d_xi));
+    if (!d_pred) {
+        HIPCHK(c, launch_cold_start(c, d_xi));
+    } else {
+        HIPCHK(c, launch_channels_to_proj(c, d_pred));
+        int nb = 0;
+        HIPCHK(c, launch_final(c, nullptr, d_xi, 3, &nb));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }

@@ -134,6 +134,7 @@ struct FinalConsts {
 // mode 0: df = ideal+ext gradient + excess gradient (energy_and_gradient)
 // mode 1: df += excess gradient (excess_energy plug-in)
 // mode 2: df = gamma(r, Omega) = -(prefactor / kT) proj2angl(proj), the OZ field in units of ln(rho/rho0)
+// mode 3: proj holds gamma's projections, df = exp((-beta vext + proj2angl(proj)) / 2) (xi not read)
 template <int PT>
 __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict__ proj, const double *__restrict__ xi,
                                                      const double *__restrict__ vext, double *__restrict__ df,
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict_
         const double wio = t.w[io];
         const double vexc = k.prefactor * wio * v;
         const size_t g = (r0 + pt) * (size_t)no + io;
-        const double x = xi[g];
+        const double x = (mode == 3) ? 0.0 : xi[g];
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
         if (mode == 0) {
@@ -224,8 +225,10 @@ __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict_
             if (df) df[g] = gid + 2.0 * rho0 * x * vexc;
         } else if (mode == 1) {
             if (df) df[g] += 2.0 * rho0 * x * vexc;
-        } else {  // mode 2: gamma(r, Omega) in units of ln(rho/rho0)
+        } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
             if (df) df[g] = -(k.prefactor / kT) * v;
+        } else {  // mode 3: proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
+            df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -299,9 +302,11 @@ struct Dims {
     static constexpr int NP = np_of(L);
 };
 
+// feat = 0: o = rho0 (xi^2 - 1) (the excess term); feat = 1: o = exp(-max(beta xi, -8) / 2) - 1
+// with xi the external potential (features of the learned warm start)
 template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
-                                                       size_t N, AngTables t, double rho0) {
+                                                       size_t N, AngTables t, double rho0, int feat, double beta) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -320,7 +325,7 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     const double *src = xi + r0 * NO;
     for (int i = tid; i < npts * NO; i += kThreads) {
         const double x = src[i];
-        o[i] = (T)(rho0 * (x * x - 1.0));
+        o[i] = feat ? (T)(exp(-0.5 * fmax(beta * x, -8.0)) - 1.0) : (T)(rho0 * (x * x - 1.0));
     }
     __syncthreads();
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
         const size_t g = (r0 + pt) * (size_t)NO + io;
-        const double x = xi[g];
+        const double x = (mode == 3) ? 0.0 : xi[g];
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
         if (mode == 0) {
@@ -468,8 +473,10 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
             if (df) df[g] = gid + 2.0 * rho0 * x * vexc;
         } else if (mode == 1) {
             if (df) df[g] += 2.0 * rho0 * x * vexc;
-        } else {  // mode 2: gamma(r, Omega) in units of ln(rho/rho0)
+        } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
             if (df) df[g] = -(k.prefactor / kT) * v;
+        } else {  // mode 3: proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
+            df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -498,11 +505,12 @@ hipError_t ensure_partials(mdft_ctx *c, size_t nblocks) {
 }  // namespace
 
 template <int PT, typename T>
-bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj) {
+bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int feat = 0) {
     const size_t nb = (c->N + PT - 1) / PT;
+    const double beta = 1.0 / c->d.kT;
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0); return true;
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0, feat, beta); return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;
@@ -613,4 +621,13 @@ hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, d
     if (c->d.mrso != 2 || !launch_final_t<kPTt, float>(c, proj, xi, df, mode, final_consts(c))) return hipErrorNotSupported;
     *nblocks_out = (int)nbt;
     return hipGetLastError();
+}
+
+// angl2proj of the warm-start feature field of vext (mrso = 2 kernels), into the context's projection buffer
+hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *vext) {
+    if (c->d.mrso != 2) return hipErrorNotSupported;
+    bool ok = (c->d.precision == MDFT_PREC_FP64)
+                  ? launch_a2p_t<kPTt, double>(c, vext, static_cast<double2 *>(c->d_proj), 1)
+                  : launch_a2p_t<kPTt, float>(c, vext, static_cast<float2 *>(c->d_proj), 1);
+    return ok ? hipGetLastError() : hipErrorNotSupported;
 }

@@ -1,5 +1,6 @@
-// k_net.hip -- the warm-start network: periodic dilated 3D convolutions with
-// ReLU, masked mean-squared loss, backward passes and Adam, all FP32.
+// k_net.hip -- the warm-start network: periodic dilated 3D convolutions
+// (3x3x3, up to 32 channels) and pointwise (1x1x1, up to 256 channels) layers
+// with ReLU, masked mean-squared loss, backward passes and Adam, all FP32.
 //
 // The reference has no learned component.  The network maps per-point
 // features of the external potential to the low angular projections of the
@@ -27,7 +28,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int KG = 8;            // output (forward) / input (data grad) channels per thread
-constexpr int kMaxC = 32;        // channels per layer
+constexpr int kMaxC3 = 32;       // channels of a 3x3x3 layer
+constexpr int kMaxC1 = 256;      // channels of a pointwise layer
 constexpr int kWChunk = 1024;    // points per workgroup in the weight gradient
 constexpr int kWSub = 128;       // points staged in LDS at a time
 
@@ -41,7 +43,12 @@ __device__ inline int wrap(int i, int n) {
 }
 
 // neighbour offsets of point p for dilation d: idx[t] for t = (tz*3+ty)*3+tx, sign s
+template <int TAPS>
 __device__ inline void taps(int p, Geo g, int d, int s, int *idx) {
+    if constexpr (TAPS == 1) {
+        idx[0] = p;
+        return;
+    }
     const int x = p % g.nx, y = (p / g.nx) % g.ny, z = p / (g.nx * g.ny);
     int xs[3], ys[3], zs[3];
 #pragma unroll
@@ -59,29 +66,31 @@ __device__ inline void taps(int p, Geo g, int d, int s, int *idx) {
 }
 
 // out[co][p] = b[co] + sum_ci sum_t W[co][ci][t] in[ci][p + (t-1) d], optional ReLU
+template <int TAPS>
 __global__ __launch_bounds__(kThreads) void conv_fwd(const float *__restrict__ in, const float *__restrict__ W,
                                                     const float *__restrict__ b, float *__restrict__ out, int cin,
                                                     int cout, Geo g, int d, int relu) {
-    __shared__ float ws[KG * kMaxC * 27];
+    extern __shared__ float ws[];  // [KG][cin][TAPS]
     const int N = g.nx * g.ny * g.nz;
     const int co0 = blockIdx.y * KG;
     const int kg = min(KG, cout - co0);
-    for (int i = threadIdx.x; i < kg * cin * 27; i += kThreads) ws[i] = W[(size_t)co0 * cin * 27 + i];
+    for (int i = threadIdx.x; i < kg * cin * TAPS; i += kThreads) ws[i] = W[(size_t)co0 * cin * TAPS + i];
     __syncthreads();
     const int p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= N) return;
-    int idx[27];
-    taps(p, g, d, 1, idx);
+    int idx[TAPS];
+    taps<TAPS>(p, g, d, 1, idx);
     float acc[KG];
 #pragma unroll
     for (int k = 0; k < KG; k++) acc[k] = 0.0f;
     for (int ci = 0; ci < cin; ci++) {
         const float *src = in + (size_t)ci * N;
 #pragma unroll
-        for (int t = 0; t < 27; t++) {
+        for (int t = 0; t < TAPS; t++) {
             const float v = src[idx[t]];
 #pragma unroll
-            for (int k = 0; k < KG; k++) acc[k] += ws[(k * cin + ci) * 27 + t] * v;
+            for (int k = 0; k < KG; k++)
+                if (k < kg) acc[k] += ws[(k * cin + ci) * TAPS + t] * v;
         }
     }
 #pragma unroll
@@ -101,31 +110,32 @@ __global__ __launch_bounds__(kThreads) void relu_bwd(float *__restrict__ gr, con
 }
 
 // din[ci][p] = sum_co sum_t W[co][ci][t] dz[co][p - (t-1) d]
+template <int TAPS>
 __global__ __launch_bounds__(kThreads) void conv_bwd_data(const float *__restrict__ dz, const float *__restrict__ W,
                                                          float *__restrict__ din, int cin, int cout, Geo g, int d) {
-    __shared__ float ws[KG * kMaxC * 27];
+    extern __shared__ float ws[];  // [cout][KG][TAPS]
     const int N = g.nx * g.ny * g.nz;
     const int ci0 = blockIdx.y * KG;
     const int kg = min(KG, cin - ci0);
-    for (int i = threadIdx.x; i < cout * kg * 27; i += kThreads) {
-        const int co = i / (kg * 27), r = i % (kg * 27), k = r / 27, t = r % 27;
-        ws[(co * KG + k) * 27 + t] = W[((size_t)co * cin + ci0 + k) * 27 + t];
+    for (int i = threadIdx.x; i < cout * KG * TAPS; i += kThreads) {
+        const int co = i / (KG * TAPS), r = i % (KG * TAPS), k = r / TAPS, t = r % TAPS;
+        ws[i] = (k < kg) ? W[((size_t)co * cin + ci0 + k) * TAPS + t] : 0.0f;
     }
     __syncthreads();
     const int p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= N) return;
-    int idx[27];
-    taps(p, g, d, -1, idx);
+    int idx[TAPS];
+    taps<TAPS>(p, g, d, -1, idx);
     float acc[KG];
 #pragma unroll
     for (int k = 0; k < KG; k++) acc[k] = 0.0f;
     for (int co = 0; co < cout; co++) {
         const float *src = dz + (size_t)co * N;
 #pragma unroll
-        for (int t = 0; t < 27; t++) {
+        for (int t = 0; t < TAPS; t++) {
             const float v = src[idx[t]];
 #pragma unroll
-            for (int k = 0; k < KG; k++) acc[k] += ws[(co * KG + k) * 27 + t] * v;
+            for (int k = 0; k < KG; k++) acc[k] += ws[(co * KG + k) * TAPS + t] * v;
         }
     }
 #pragma unroll
@@ -188,6 +198,51 @@ __global__ __launch_bounds__(1024) void conv_bwd_weight(const float *__restrict_
     }
 }
 
+// Pointwise layer: part[blk][co * cin + ci] = sum_p dz[co][p] a[ci][p],
+// part[blk][cout * cin + co] = sum_p dz[co][p]; one thread per input channel
+// (plus one for the bias), 32 output channels per workgroup (blockIdx.y).
+__global__ __launch_bounds__(320) void pw_bwd_weight(const float *__restrict__ dz, const float *__restrict__ a,
+                                                    float *__restrict__ part, int cin, int cout, int N) {
+    constexpr int CO = 32;
+    __shared__ float gs[CO * kWSub];
+    const int tid = threadIdx.x;
+    const int co0 = blockIdx.y * CO;
+    const bool wthread = tid < cin, bthread = tid == cin;
+    float acc[CO];
+#pragma unroll
+    for (int k = 0; k < CO; k++) acc[k] = 0.0f;
+    const int p0 = blockIdx.x * kWChunk;
+    const int pend = min(N, p0 + kWChunk);
+    for (int s0 = p0; s0 < pend; s0 += kWSub) {
+        const int ns = min(kWSub, pend - s0);
+        __syncthreads();
+        for (int i = tid; i < CO * kWSub; i += blockDim.x) {
+            const int k = i / kWSub, j = i % kWSub;
+            gs[i] = (co0 + k < cout && j < ns) ? dz[(size_t)(co0 + k) * N + s0 + j] : 0.0f;
+        }
+        __syncthreads();
+        if (wthread) {
+            const float *src = a + (size_t)tid * N + s0;
+            for (int j = 0; j < ns; j++) {
+                const float v = src[j];
+#pragma unroll
+                for (int k = 0; k < CO; k++) acc[k] += gs[k * kWSub + j] * v;
+            }
+        } else if (bthread) {
+            for (int j = 0; j < ns; j++)
+#pragma unroll
+                for (int k = 0; k < CO; k++) acc[k] += gs[k * kWSub + j];
+        }
+    }
+    float *dst = part + (size_t)blockIdx.x * (cout * cin + cout);
+#pragma unroll
+    for (int k = 0; k < CO; k++) {
+        if (co0 + k >= cout) break;
+        if (wthread) dst[(size_t)(co0 + k) * cin + tid] = acc[k];
+        else if (bthread) dst[(size_t)cout * cin + co0 + k] = acc[k];
+    }
+}
+
 // out[i] = sum_b part[b][i] (fixed order)
 __global__ __launch_bounds__(kThreads) void sum_parts(const float *__restrict__ part, int nblk, int n,
                                                      float *__restrict__ out) {
@@ -243,7 +298,7 @@ unsigned grid1(size_t n) {
 struct mdft_wsnet {
     int device = 0;
     int nl = 0;
-    std::vector<int> ch, dil;
+    std::vector<int> ch, dil, taps;  // taps: 27 (3x3x3) or 1 (pointwise)
     std::vector<size_t> woff, boff;  // offsets of W_l and b_l in the parameter vector
     size_t nparam = 0;
     float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;
@@ -305,10 +360,15 @@ int forward(mdft_wsnet *n, const float *in) {
     const size_t N = (size_t)g.nx * g.ny * g.nz;
     const float *src = in;
     for (int l = 1; l <= n->nl; l++) {
-        const int cin = n->ch[l - 1], cout = n->ch[l];
+        const int cin = n->ch[l - 1], cout = n->ch[l], T = n->taps[l - 1];
         dim3 grid((unsigned)((N + kThreads - 1) / kThreads), (unsigned)((cout + KG - 1) / KG));
-        hipLaunchKernelGGL(conv_fwd, grid, dim3(kThreads), 0, n->stream, src, n->params + n->woff[l - 1],
-                           n->params + n->boff[l - 1], n->act[l], cin, cout, g, n->dil[l - 1], l < n->nl ? 1 : 0);
+        const size_t lds = sizeof(float) * KG * cin * T;
+        if (T == 27)
+            hipLaunchKernelGGL(conv_fwd<27>, grid, dim3(kThreads), lds, n->stream, src, n->params + n->woff[l - 1],
+                               n->params + n->boff[l - 1], n->act[l], cin, cout, g, n->dil[l - 1], l < n->nl ? 1 : 0);
+        else
+            hipLaunchKernelGGL(conv_fwd<1>, grid, dim3(kThreads), lds, n->stream, src, n->params + n->woff[l - 1],
+                               n->params + n->boff[l - 1], n->act[l], cin, cout, g, 1, l < n->nl ? 1 : 0);
         WSCHK(n, hipGetLastError());
         src = n->act[l];
     }
@@ -325,23 +385,21 @@ void launch_wgrad(mdft_wsnet *n, const float *dz, const float *a, int cin, int c
 
 extern "C" {
 
-int mdft_wsnet_create(int nlayers, const int *channels, const int *dilations, unsigned long long seed, int device,
-                      mdft_wsnet **out) {
-    if (!out || !channels || !dilations || nlayers < 1) {
+int mdft_wsnet_create(int nlayers, const int *channels, const int *ksize, const int *dilations,
+                      unsigned long long seed, int device, mdft_wsnet **out) {
+    if (!out || !channels || !ksize || !dilations || nlayers < 1) {
         g_ws_create_error = "bad arguments";
         return MDFT_EINVAL;
     }
     *out = nullptr;
-    for (int l = 0; l <= nlayers; l++)
-        if (channels[l] < 1 || channels[l] > kMaxC) {
-            g_ws_create_error = "channels must be in 1..32";
+    for (int l = 0; l < nlayers; l++) {
+        const int cmax = ksize[l] == 3 ? kMaxC3 : kMaxC1;
+        if ((ksize[l] != 1 && ksize[l] != 3) || dilations[l] < 1 || channels[l] < 1 || channels[l] > cmax ||
+            channels[l + 1] < 1 || channels[l + 1] > cmax) {
+            g_ws_create_error = "layers: kernel size 3 (<= 32 channels) or 1 (<= 256 channels), dilation >= 1";
             return MDFT_EINVAL;
         }
-    for (int l = 0; l < nlayers; l++)
-        if (dilations[l] < 1) {
-            g_ws_create_error = "dilations must be >= 1";
-            return MDFT_EINVAL;
-        }
+    }
     if (hipSetDevice(device) != hipSuccess) {
         g_ws_create_error = "hipSetDevice failed";
         return MDFT_ENODEV;
@@ -351,10 +409,11 @@ int mdft_wsnet_create(int nlayers, const int *channels, const int *dilations, un
     n->nl = nlayers;
     n->ch.assign(channels, channels + nlayers + 1);
     n->dil.assign(dilations, dilations + nlayers);
+    for (int l = 0; l < nlayers; l++) n->taps.push_back(ksize[l] == 3 ? 27 : 1);
     size_t off = 0;
     for (int l = 0; l < nlayers; l++) {
         n->woff.push_back(off);
-        off += (size_t)n->ch[l + 1] * n->ch[l] * 27;
+        off += (size_t)n->ch[l + 1] * n->ch[l] * n->taps[l];
         n->boff.push_back(off);
         off += n->ch[l + 1];
     }
@@ -371,8 +430,8 @@ int mdft_wsnet_create(int nlayers, const int *channels, const int *dilations, un
         return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
     };
     for (int l = 0; l < nlayers; l++) {
-        const double sd = sqrt(2.0 / (27.0 * n->ch[l]));
-        for (size_t i = 0; i < (size_t)n->ch[l + 1] * n->ch[l] * 27; i++) {
+        const double sd = sqrt(2.0 / ((double)n->taps[l] * n->ch[l]));
+        for (size_t i = 0; i < (size_t)n->ch[l + 1] * n->ch[l] * n->taps[l]; i++) {
             const double u1 = next(), u2 = next();
             h[n->woff[l] + i] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2));
         }
@@ -475,7 +534,8 @@ int mdft_wsnet_loss_grad(mdft_wsnet *n, int nx, int ny, int nz, const float *d_i
     // weight-gradient partials
     const unsigned nblk = (unsigned)((N + kWChunk - 1) / kWChunk);
     size_t need = 0;
-    for (int l = 0; l < L; l++) need = std::max(need, (size_t)nblk * ((size_t)n->ch[l + 1] * n->ch[l] * 27 + n->ch[l + 1]));
+    for (int l = 0; l < L; l++)
+        need = std::max(need, (size_t)nblk * ((size_t)n->ch[l + 1] * n->ch[l] * n->taps[l] + n->ch[l + 1]));
     if (need > n->part_cap) {
         (void)hipFree(n->part);
         n->part = nullptr;
@@ -491,19 +551,31 @@ int mdft_wsnet_loss_grad(mdft_wsnet *n, int nx, int ny, int nz, const float *d_i
             WSCHK(n, hipGetLastError());
         }
         const float *a = (l == 1) ? d_in : n->act[l - 1];
-        if (cout <= 8) launch_wgrad<8>(n, gcur, a, cin, cout, g, d, nblk);
-        else if (cout <= 16) launch_wgrad<16>(n, gcur, a, cin, cout, g, d, nblk);
-        else launch_wgrad<32>(n, gcur, a, cin, cout, g, d, nblk);
+        const int T = n->taps[l - 1];
+        if (T == 27) {
+            if (cout <= 8) launch_wgrad<8>(n, gcur, a, cin, cout, g, d, nblk);
+            else if (cout <= 16) launch_wgrad<16>(n, gcur, a, cin, cout, g, d, nblk);
+            else launch_wgrad<32>(n, gcur, a, cin, cout, g, d, nblk);
+        } else {
+            const int thr = ((cin + 1 + 63) / 64) * 64;
+            hipLaunchKernelGGL(pw_bwd_weight, dim3(nblk, (unsigned)((cout + 31) / 32)), dim3(thr), 0, n->stream, gcur,
+                               a, n->part, cin, cout, N);
+        }
         WSCHK(n, hipGetLastError());
-        const int nw = cout * cin * 27 + cout;  // W_l then b_l, contiguous in the parameter vector
+        const int nw = cout * cin * T + cout;  // W_l then b_l, contiguous in the parameter vector
         hipLaunchKernelGGL(sum_parts, dim3((nw + kThreads - 1) / kThreads), dim3(kThreads), 0, n->stream, n->part,
                            (int)nblk, nw, n->grads + n->woff[l - 1]);
         WSCHK(n, hipGetLastError());
         if (l > 1 || d_gin) {
             float *dst = (l == 1) ? d_gin : gnext;
             dim3 grid((unsigned)((N + kThreads - 1) / kThreads), (unsigned)((cin + KG - 1) / KG));
-            hipLaunchKernelGGL(conv_bwd_data, grid, dim3(kThreads), 0, n->stream, gcur, n->params + n->woff[l - 1],
-                               dst, cin, cout, g, d);
+            const size_t lds = sizeof(float) * cout * KG * T;
+            if (T == 27)
+                hipLaunchKernelGGL(conv_bwd_data<27>, grid, dim3(kThreads), lds, n->stream, gcur,
+                                   n->params + n->woff[l - 1], dst, cin, cout, g, d);
+            else
+                hipLaunchKernelGGL(conv_bwd_data<1>, grid, dim3(kThreads), lds, n->stream, gcur,
+                                   n->params + n->woff[l - 1], dst, cin, cout, g, 1);
             WSCHK(n, hipGetLastError());
             std::swap(gcur, gnext);
         }

@@ -1,11 +1,13 @@
-// k_warmstart.hip -- per-point projections onto the low angular basis and the
-// density guess of the learned warm start.
+// k_warmstart.hip -- layout changes between the projection field and the
+// network's channel-major fields, and the reference cold start, for the
+// learned warm start.
 //
-// The reference has no learned component.  Basis B_k(Omega), k = 0..3: 1 and
-// the lab-frame components of the molecule's z axis (sin t cos p, sin t sin p,
-// cos t) -- the m <= 1 projections for molecular symmetry order 2.  The
-// quadrature weights and 1 / sum_io w_io B_k^2 are folded into the table Bw
-// on the host, so a projection is a plain dot product over orientations.
+// The reference has no learned component.  The network sees the projections
+// of a feature field of vext and predicts the projections of the OZ field
+// gamma; both live in the context's projection layout [N][np] complex (FP64 or
+// FP32) on the MDFT side and as 2 np real channels [2 np][N] (FP32, x fastest)
+// on the network side: channel 2 ip + 0 holds the real part of projection ip,
+// 2 ip + 1 the imaginary part.
 #include <cfloat>
 
 #include "mdft_internal.h"
@@ -13,72 +15,83 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPts = 16;  // grid points per workgroup (their [kPts][no] tile is staged in LDS)
+constexpr int kTile = 32;  // grid points per workgroup in the transposes
 
-// mode 0: features of vext (8 channels), mode 1: projections of a field (4 channels)
-__global__ __launch_bounds__(kThreads) void ws_project(const double *__restrict__ f, const double *__restrict__ Bw,
-                                                      float *__restrict__ out, size_t N, int no, double beta,
-                                                      int mode) {
-    extern __shared__ double tile[];  // [kPts][no]
-    const size_t p0 = (size_t)blockIdx.x * kPts;
-    const int npts = (int)((N - p0) < (size_t)kPts ? (N - p0) : (size_t)kPts);
-    for (int i = threadIdx.x; i < npts * no; i += kThreads) tile[i] = f[p0 * no + i];
+// out[2 ip + c][p] = scale * proj[p][ip].c  (via an LDS tile: both sides coalesced)
+template <typename V>
+__global__ __launch_bounds__(kThreads) void proj_to_channels(const V *__restrict__ proj, float *__restrict__ out,
+                                                            size_t N, int np, float scale) {
+    extern __shared__ float tile[];  // [kTile][2 np]
+    const int w2 = 2 * np;
+    const size_t p0 = (size_t)blockIdx.x * kTile;
+    const int npts = (int)((N - p0) < (size_t)kTile ? (N - p0) : (size_t)kTile);
+    const auto *src = reinterpret_cast<const decltype(V::x) *>(proj) + p0 * w2;
+    for (int i = threadIdx.x; i < npts * w2; i += kThreads) tile[i] = (float)src[i];
     __syncthreads();
-    const int nch = mode == 0 ? 8 : 4;
-    for (int o = threadIdx.x; o < npts * nch; o += kThreads) {
-        const int pt = o / nch, ch = o % nch, k = ch & 3;
-        const double *row = tile + pt * no, *b = Bw + (size_t)k * no;
-        double s = 0.0;
-        if (mode == 1) {
-            for (int io = 0; io < no; io++) s += row[io] * b[io];
-        } else if (ch < 4) {
-            for (int io = 0; io < no; io++) s += (exp(-0.5 * fmax(beta * row[io], -8.0)) - 1.0) * b[io];
-        } else {
-            for (int io = 0; io < no; io++) s += tanh(-beta * row[io] / 8.0) * b[io];
-        }
-        out[(size_t)ch * N + p0 + pt] = (float)s;
+    for (int i = threadIdx.x; i < kTile * w2; i += kThreads) {
+        const int ch = i / kTile, pt = i % kTile;
+        if (pt < npts) out[(size_t)ch * N + p0 + pt] = scale * tile[pt * w2 + ch];
     }
 }
 
-// xi = exp((-beta vext + sum_k pred_k B_k) / 2), exponent clipped to [-200, 7];
-// without pred the reference cold start (src/module_density.f90:267-284):
-// xi = 0 where vext >= (-ln(DBL_EPSILON) - 0.05) kT, 1 elsewhere
-__global__ __launch_bounds__(kThreads) void ws_guess(const double *__restrict__ vext, const float *__restrict__ pred,
-                                                    const double *__restrict__ B, double *__restrict__ xi, size_t N,
-                                                    int no, double beta) {
-    const size_t total = N * (size_t)no;
-    const double vmax = -log(DBL_EPSILON) - 0.05;
-    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads) {
-        if (!pred) {
-            xi[g] = (beta * vext[g] >= vmax) ? 0.0 : 1.0;
-            continue;
-        }
-        const size_t p = g / no;
-        const int io = (int)(g - p * no);
-        double lg = -beta * vext[g];
-#pragma unroll
-        for (int k = 0; k < 4; k++) lg += (double)pred[(size_t)k * N + p] * B[(size_t)k * no + io];
-        lg = fmin(fmax(lg, -200.0), 7.0);
-        xi[g] = exp(0.5 * lg);
+// proj[p][ip] = (in[2 ip][p], in[2 ip + 1][p])
+template <typename V>
+__global__ __launch_bounds__(kThreads) void channels_to_proj(const float *__restrict__ in, V *__restrict__ proj,
+                                                            size_t N, int np) {
+    extern __shared__ float tile[];  // [kTile][2 np]
+    using T = decltype(V::x);
+    const int w2 = 2 * np;
+    const size_t p0 = (size_t)blockIdx.x * kTile;
+    const int npts = (int)((N - p0) < (size_t)kTile ? (N - p0) : (size_t)kTile);
+    for (int i = threadIdx.x; i < kTile * w2; i += kThreads) {
+        const int ch = i / kTile, pt = i % kTile;
+        if (pt < npts) tile[pt * w2 + ch] = in[(size_t)ch * N + p0 + pt];
     }
+    __syncthreads();
+    T *dst = reinterpret_cast<T *>(proj) + p0 * w2;
+    for (int i = threadIdx.x; i < npts * w2; i += kThreads) dst[i] = (T)tile[i];
+}
+
+// reference cold start (src/module_density.f90:267-284):
+// xi = 0 where vext >= (-ln(DBL_EPSILON) - 0.05) kT, 1 elsewhere
+__global__ __launch_bounds__(kThreads) void cold_start(const double *__restrict__ vext, double *__restrict__ xi,
+                                                      size_t total, double beta) {
+    const double vmax = -log(DBL_EPSILON) - 0.05;
+    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads)
+        xi[g] = (beta * vext[g] >= vmax) ? 0.0 : 1.0;
 }
 
 }  // namespace
 
-hipError_t launch_ws_project(mdft_ctx *c, const double *f, float *out, int mode) {
-    const size_t nb = (c->N + kPts - 1) / kPts;
-    const size_t lds = sizeof(double) * kPts * c->no;
-    if (lds > c->lds_limit) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(ws_project, dim3((unsigned)nb), dim3(kThreads), lds, c->stream, f, c->d_wsBw, out, c->N, c->no,
-                       1.0 / c->d.kT, mode);
+hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale) {
+    const size_t nb = (c->N + kTile - 1) / kTile;
+    const size_t lds = sizeof(float) * kTile * 2 * c->np;
+    if (c->d.precision == MDFT_PREC_FP64)
+        hipLaunchKernelGGL(proj_to_channels<double2>, dim3((unsigned)nb), dim3(kThreads), lds, c->stream,
+                           static_cast<const double2 *>(c->d_proj), out, c->N, c->np, scale);
+    else
+        hipLaunchKernelGGL(proj_to_channels<float2>, dim3((unsigned)nb), dim3(kThreads), lds, c->stream,
+                           static_cast<const float2 *>(c->d_proj), out, c->N, c->np, scale);
     return hipGetLastError();
 }
 
-hipError_t launch_ws_guess(mdft_ctx *c, const float *pred, double *xi) {
+hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in) {
+    const size_t nb = (c->N + kTile - 1) / kTile;
+    const size_t lds = sizeof(float) * kTile * 2 * c->np;
+    if (c->d.precision == MDFT_PREC_FP64)
+        hipLaunchKernelGGL(channels_to_proj<double2>, dim3((unsigned)nb), dim3(kThreads), lds, c->stream, in,
+                           static_cast<double2 *>(c->d_proj), c->N, c->np);
+    else
+        hipLaunchKernelGGL(channels_to_proj<float2>, dim3((unsigned)nb), dim3(kThreads), lds, c->stream, in,
+                           static_cast<float2 *>(c->d_proj), c->N, c->np);
+    return hipGetLastError();
+}
+
+hipError_t launch_cold_start(mdft_ctx *c, double *xi) {
     const size_t total = c->N * (size_t)c->no;
     size_t nb = (total + kThreads - 1) / kThreads;
     if (nb > 65536) nb = 65536;
-    hipLaunchKernelGGL(ws_guess, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, c->d_vext, pred, c->d_wsB, xi, c->N,
-                       c->no, 1.0 / c->d.kT);
+    hipLaunchKernelGGL(cold_start, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, c->d_vext, xi, total,
+                       1.0 / c->d.kT);
     return hipGetLastError();
 }

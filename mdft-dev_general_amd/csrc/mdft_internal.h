@@ -70,8 +70,6 @@ struct mdft_ctx {
     double2 *d_ck = nullptr;      // [nq][npc]
     float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
     int *d_oz32_lanerow = nullptr; // [MDFT_OZ32_ROWS_PER_LANE][64] OZ rows per lane
-    double *d_wsB = nullptr;      // [4][no] low angular basis of the warm start
-    double *d_wsBw = nullptr;     // [4][no] the same times w_io / sum_io w_io B_k^2
     void *d_bf16_M = nullptr;     // angl2proj matrix, BF16 MFMA fragment order
     void *d_bf16_Q = nullptr;     // proj2angl matrix, BF16 MFMA fragment order
     PairSchedule pairs;
@@ -113,8 +111,10 @@ hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
 hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
-hipError_t launch_ws_project(mdft_ctx *c, const double *f, float *out, int mode);
-hipError_t launch_ws_guess(mdft_ctx *c, const float *pred, double *xi);
+hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *vext);
+hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale);
+hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in);
+hipError_t launch_cold_start(mdft_ctx *c, double *xi);
 int bf16_tables(mdft_ctx *c, const std::vector<double> &a2p, const std::vector<double> &p2a,
                 const std::vector<double> &fm);
 hipError_t launch_angl2proj_bf16(mdft_ctx *c, const double *xi, float2 *proj);

@@ -6,8 +6,9 @@ separation 1.2 A, sigma ~ U[2.5, 4.0] A, eps ~ U[0.2, 1.0] kJ/mol, charges
 N(0, 0.4) shifted to zero net charge), each solved by the plain method (reference
 cold start, FP32 evaluations, L-BFGS to |dF|/|dF0| < 1e-5) on a 20 A box at
 64^3 -- the grid spacing of the 128^3 / 40 A workload (0.3125 A) -- with mmax = 5
-and the same c(k) table.  Inputs are the vext features, targets the m <= 1
-projections of the converged OZ field.  Adam, lr 1e-3 halved every 10 epochs.
+and the same c(k) table.  Inputs are the projections of the vext feature
+field, targets the projections of the converged OZ field.  Adam, lr 1e-3
+halved every 10 epochs.
 
     python -m mdft.train_warmstart --train 192 --val 16 --epochs 40 --out DIR
 """
@@ -49,7 +50,7 @@ def main(argv=None):
     eng, data = ws.make_training_set(w, args.train + args.val, args.seed, table, gtol=1e-5, log=say)
     t_data = time.perf_counter() - t0
     train, val = data[:args.train], data[args.train:]
-    net = ws.WarmStartNet(seed=args.seed)
+    net = ws.WarmStartNet.for_engine(eng, seed=args.seed)
     say(f"[train_warmstart] {net.n_params} parameters; training {args.epochs} epochs on {len(train)} solutes")
     t1 = time.perf_counter()
     hist, vhist = [], []

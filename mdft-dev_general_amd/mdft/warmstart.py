@@ -2,16 +2,17 @@
 
 The reference has no learned component; the minimiser it drives
 (src/energy_minimization.f90) starts from the cold guess of
-src/module_density.f90:267-284.  Here a periodic dilated 3D convolutional
-network (hand-written HIP forward/backward/Adam kernels, FP32) maps per-point
-features of the external potential to the m <= 1 angular projections of the
-converged OZ field gamma = c (*) drho, and the minimiser starts from
+src/module_density.f90:267-284.  Here a periodic 3D convolutional network
+(hand-written HIP forward/backward/Adam kernels, FP32) maps the angular
+projections of a feature field of the external potential to the angular
+projections of the converged OZ field gamma = c (*) drho, and the minimiser
+starts from
 
-    xi = exp((-beta vext + sum_k pred_k(r) B_k(Omega)) / 2),
+    xi = exp((-beta vext + gamma_pred(r, Omega)) / 2),
 
-the Euler-Lagrange form of the minimum with gamma truncated to m <= 1.  The
-minimisation afterwards is the plain one: the network changes the number of
-iterations, not the answer.
+the Euler-Lagrange form of the minimum.  The minimisation afterwards is the
+plain one: the network changes the number of iterations, not the answer, and
+the cold start stays the fallback (warm_start(..., fallback=...)).
 
 Training data are converged solutions of the plain method on small boxes with
 the grid spacing of the target workload (the network is translation invariant
@@ -27,8 +28,15 @@ import numpy as np
 
 from .gpu import DeviceBuffer, Engine, load_library
 
-CHANNELS = (8, 32, 32, 32, 32, 4)
-DILATIONS = (1, 2, 4, 8, 1)
+HIDDEN = 32
+
+
+def architecture(np_: int, hidden: int = HIDDEN):
+    """(channels, kernel sizes, dilations): pointwise 2np -> hidden, four 3x3x3
+    layers with dilations 1, 2, 4, 8 (receptive field 31 voxels), pointwise
+    hidden -> 2np."""
+    c = 2 * np_
+    return (c, hidden, hidden, hidden, hidden, hidden, c), (1, 3, 3, 3, 3, 1), (1, 1, 2, 4, 8, 1)
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_WEIGHTS = os.path.join(DATA, "warmstart_solute20.npz")
 
@@ -46,7 +54,7 @@ def _lib():
     lib = load_library()
     if not _bound:
         vp, ip = C.c_void_p, C.POINTER(C.c_int)
-        lib.mdft_wsnet_create.argtypes = [C.c_int, ip, ip, C.c_ulonglong, C.c_int, C.POINTER(vp)]
+        lib.mdft_wsnet_create.argtypes = [C.c_int, ip, ip, ip, C.c_ulonglong, C.c_int, C.POINTER(vp)]
         lib.mdft_wsnet_destroy.argtypes = [vp]
         lib.mdft_wsnet_destroy.restype = None
         lib.mdft_wsnet_last_error.argtypes = [vp]
@@ -72,14 +80,16 @@ def _lib():
 class WarmStartNet:
     """The network: create with random (seeded) weights or load saved ones."""
 
-    def __init__(self, channels=CHANNELS, dilations=DILATIONS, seed: int = 0, device: int = 0):
+    def __init__(self, channels, ksize, dilations, seed: int = 0, device: int = 0):
         self.lib = _lib()
         self.channels = tuple(int(c) for c in channels)
+        self.ksize = tuple(int(k) for k in ksize)
         self.dilations = tuple(int(d) for d in dilations)
         ch = (C.c_int * len(self.channels))(*self.channels)
+        ks = (C.c_int * len(self.ksize))(*self.ksize)
         dl = (C.c_int * len(self.dilations))(*self.dilations)
         h = C.c_void_p()
-        rc = self.lib.mdft_wsnet_create(len(self.dilations), ch, dl, int(seed), int(device), C.byref(h))
+        rc = self.lib.mdft_wsnet_create(len(self.dilations), ch, ks, dl, int(seed), int(device), C.byref(h))
         if rc != 0:
             raise RuntimeError(f"mdft_wsnet_create: {self.lib.mdft_wsnet_last_error(None).decode()}")
         self._h = h
@@ -119,12 +129,12 @@ class WarmStartNet:
         return out
 
     def layers(self, p=None):
-        """Split a parameter vector into [(W [co][ci][3][3][3], b [co])]."""
+        """Split a parameter vector into [(W [co][ci][k][k][k], b [co])]."""
         p = self.get_params() if p is None else p
         out, off = [], 0
-        for ci, co in zip(self.channels[:-1], self.channels[1:]):
-            w = p[off:off + co * ci * 27].reshape(co, ci, 3, 3, 3)
-            off += co * ci * 27
+        for ci, co, k in zip(self.channels[:-1], self.channels[1:], self.ksize):
+            w = p[off:off + co * ci * k ** 3].reshape(co, ci, k, k, k)
+            off += co * ci * k ** 3
             out.append((w, p[off:off + co]))
             off += co
         return out
@@ -143,15 +153,19 @@ class WarmStartNet:
 
     def save(self, path: str, **meta) -> None:
         os.makedirs(os.path.dirname(path), exist_ok=True)
-        np.savez(path, params=self.get_params(), channels=np.array(self.channels), dilations=np.array(self.dilations),
-                 **{k: np.asarray(v) for k, v in meta.items()})
+        np.savez(path, params=self.get_params(), channels=np.array(self.channels), ksize=np.array(self.ksize),
+                 dilations=np.array(self.dilations), **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
     def load(cls, path: str = DEFAULT_WEIGHTS, device: int = 0) -> "WarmStartNet":
         with np.load(path, allow_pickle=False) as z:
-            net = cls(tuple(z["channels"]), tuple(z["dilations"]), device=device)
+            net = cls(tuple(z["channels"]), tuple(z["ksize"]), tuple(z["dilations"]), device=device)
             net.set_params(z["params"])
         return net
+
+    @classmethod
+    def for_engine(cls, eng: Engine, seed: int = 0, hidden: int = HIDDEN) -> "WarmStartNet":
+        return cls(*architecture(eng.np_, hidden), seed=seed)
 
 
 def _ptr(b):
@@ -161,22 +175,19 @@ def _ptr(b):
 
 
 def features(eng: Engine) -> DeviceBuffer:
-    """[8][N] FP32 features of the engine's vext."""
-    out = DeviceBuffer(eng, 4 * 8 * eng.N)
+    """[2 np][N] FP32: projections of the feature field of the engine's vext."""
+    out = DeviceBuffer(eng, 4 * 2 * eng.np_ * eng.N)
     eng._check(_lib().mdft_ws_features(eng._h, out.ptr))
     return out
 
 
 def gamma(eng: Engine, xi: DeviceBuffer, want_field: bool = False):
-    """m <= 1 projections [4][N] (FP32) of gamma = c (*) drho for xi; with
+    """Projections [2 np][N] (FP32) of gamma = c (*) drho for xi; with
     want_field also the field gamma(r, Omega) [N][no] (FP64)."""
-    g = DeviceBuffer(eng, 8 * eng.N * eng.no)
-    p4 = DeviceBuffer(eng, 4 * 4 * eng.N)
-    eng._check(_lib().mdft_ws_gamma(eng._h, xi.ptr, g.ptr, p4.ptr))
-    if want_field:
-        return p4, g
-    g.free()
-    return p4
+    tgt = DeviceBuffer(eng, 4 * 2 * eng.np_ * eng.N)
+    g = DeviceBuffer(eng, 8 * eng.N * eng.no) if want_field else None
+    eng._check(_lib().mdft_ws_gamma(eng._h, xi.ptr, _ptr(g), tgt.ptr))
+    return (tgt, g) if want_field else tgt
 
 
 def guess(eng: Engine, pred: DeviceBuffer | None, xi: DeviceBuffer) -> None:
@@ -195,6 +206,7 @@ def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: Devic
     one evaluation and never a worse start."""
     f = features(eng)
     pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
+    assert net.channels[-1] == 2 * eng.np_ and net.channels[0] == 2 * eng.np_
     net.forward(eng.n, f, pred)
     guess(eng, pred, xi)
     f.free()
@@ -239,6 +251,7 @@ def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4,
 
 
 def train(net: WarmStartNet, n, data, epochs: int, lr: float = 1e-3, seed: int = 0, log=None):
+    """data: [(features, targets, ...)] device fields of grid n."""
     """Adam over the samples in a seeded random order; returns per-epoch mean loss."""
     rng = np.random.default_rng(seed)
     hist = []

@@ -5,52 +5,54 @@ The reference has no learned component; this restates the network that the
 HIP kernels implement (mdft-dev_general_amd/csrc/k_net.hip):
 
     a_0 = x                                   [C0][nz][ny][nx]
-    z_l = conv3d(a_{l-1}; W_l, dilation d_l) + b_l   (3x3x3 taps, periodic)
+    z_l = conv3d(a_{l-1}; W_l, dilation d_l) + b_l   (k x k x k taps, k in {1, 3}, periodic)
     a_l = relu(z_l)   for l < L,   a_L = z_L  (linear output)
-    loss = sum(mask * (a_L - t)^2) / sum(mask)
+    loss = sum(mask * (a_L - t)^2) / (sum(mask) C_L)
 
-Weights W_l[co][ci][tz][ty][tx], tap offsets (t - 1) * d_l along each axis,
-periodic wrap (the simulation box is periodic).  Pinned in the tests against
-torch's conv3d with circular padding and autograd on the CPU.
+Weights W_l[co][ci][tz][ty][tx], tap offsets (t - (k-1)/2) * d_l along each
+axis, periodic wrap (the simulation box is periodic).  Pinned in the tests
+against torch's conv3d with circular padding and autograd on the CPU.
 """
 from __future__ import annotations
 
 import numpy as np
 
 
+def _taps(k):
+    h = (k - 1) // 2
+    return [(tz, ty, tx) for tz in range(k) for ty in range(k) for tx in range(k)], h
+
+
 def conv3d(x, w, b, dil):
-    """x [Ci][nz][ny][nx], w [Co][Ci][3][3][3] -> [Co][nz][ny][nx] (periodic)."""
+    """x [Ci][nz][ny][nx], w [Co][Ci][k][k][k] -> [Co][nz][ny][nx] (periodic)."""
     co = w.shape[0]
     out = np.zeros((co,) + x.shape[1:], dtype=np.float64)
-    for tz in range(3):
-        for ty in range(3):
-            for tx in range(3):
-                # out[p] += w * x[p + (t-1) d]
-                xs = np.roll(x, shift=(-(tz - 1) * dil, -(ty - 1) * dil, -(tx - 1) * dil), axis=(1, 2, 3))
-                out += np.einsum("oi,izyx->ozyx", w[:, :, tz, ty, tx], xs)
+    taps, h = _taps(w.shape[2])
+    for tz, ty, tx in taps:
+        # out[p] += w * x[p + (t-h) d]
+        xs = np.roll(x, shift=(-(tz - h) * dil, -(ty - h) * dil, -(tx - h) * dil), axis=(1, 2, 3))
+        out += np.einsum("oi,izyx->ozyx", w[:, :, tz, ty, tx], xs)
     return out + b[:, None, None, None]
 
 
 def conv3d_bwd_data(g, w, dil):
-    """dL/dx given dL/dz = g [Co][...]: x[p] feeds z[p - (t-1) d]."""
+    """dL/dx given dL/dz = g [Co][...]: x[p] feeds z[p - (t-h) d]."""
     ci = w.shape[1]
     out = np.zeros((ci,) + g.shape[1:], dtype=np.float64)
-    for tz in range(3):
-        for ty in range(3):
-            for tx in range(3):
-                gs = np.roll(g, shift=((tz - 1) * dil, (ty - 1) * dil, (tx - 1) * dil), axis=(1, 2, 3))
-                out += np.einsum("oi,ozyx->izyx", w[:, :, tz, ty, tx], gs)
+    taps, h = _taps(w.shape[2])
+    for tz, ty, tx in taps:
+        gs = np.roll(g, shift=((tz - h) * dil, (ty - h) * dil, (tx - h) * dil), axis=(1, 2, 3))
+        out += np.einsum("oi,ozyx->izyx", w[:, :, tz, ty, tx], gs)
     return out
 
 
-def conv3d_bwd_weight(g, x, dil):
+def conv3d_bwd_weight(g, x, dil, k=3):
     co, ci = g.shape[0], x.shape[0]
-    dw = np.zeros((co, ci, 3, 3, 3))
-    for tz in range(3):
-        for ty in range(3):
-            for tx in range(3):
-                xs = np.roll(x, shift=(-(tz - 1) * dil, -(ty - 1) * dil, -(tx - 1) * dil), axis=(1, 2, 3))
-                dw[:, :, tz, ty, tx] = np.einsum("ozyx,izyx->oi", g, xs)
+    dw = np.zeros((co, ci, k, k, k))
+    taps, h = _taps(k)
+    for tz, ty, tx in taps:
+        xs = np.roll(x, shift=(-(tz - h) * dil, -(ty - h) * dil, -(tx - h) * dil), axis=(1, 2, 3))
+        dw[:, :, tz, ty, tx] = np.einsum("ozyx,izyx->oi", g, xs)
     return dw, g.sum(axis=(1, 2, 3))
 
 
@@ -71,7 +73,7 @@ class Net:
         acts = self.forward(x)
         y = acts[-1]
         m = np.ones_like(y) if mask is None else np.broadcast_to(mask, y.shape)
-        cnt = m.sum()
+        cnt = m.sum()   # mask broadcast over channels: sum(mask) * C_L
         r = y - t
         loss = float((m * r * r).sum() / cnt)
         g = 2.0 * m * r / cnt
@@ -79,7 +81,7 @@ class Net:
         for li in range(len(self.W) - 1, -1, -1):
             if li < len(self.W) - 1:
                 g = g * (acts[li + 1] > 0.0)
-            gW[li], gb[li] = conv3d_bwd_weight(g, acts[li], self.d[li])
+            gW[li], gb[li] = conv3d_bwd_weight(g, acts[li], self.d[li], self.W[li].shape[2])
             g = conv3d_bwd_data(g, self.W[li], self.d[li])
         return loss, gW, gb, g
 

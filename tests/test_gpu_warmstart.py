@@ -33,13 +33,13 @@ def test_network_kernels_match_oracle():
     say("[wsnet] forward, loss gradient, input gradient and Adam vs the FP64 restatement")
     n = (10, 9, 8)
     N = n[0] * n[1] * n[2]
-    chans, dils = (3, 12, 9, 2), (1, 2, 3)
-    net = WarmStartNet(chans, dils, seed=5)
+    chans, ks, dils = (5, 20, 12, 9, 40, 2), (1, 3, 3, 1, 1), (1, 1, 2, 1, 1)
+    net = WarmStartNet(chans, ks, dils, seed=5)
     rng = np.random.default_rng(0)
     p = rng.normal(0, 0.2, net.n_params).astype(np.float32)
     net.set_params(p)
     eng = _engine((4, 4, 4), (8.0, 8.0, 8.0), 1)   # only for device buffers
-    x = rng.normal(size=(3, n[2], n[1], n[0])).astype(np.float32)
+    x = rng.normal(size=(5, n[2], n[1], n[0])).astype(np.float32)
     t = rng.normal(size=(2, n[2], n[1], n[0])).astype(np.float32)
     mask = (rng.uniform(size=(n[2], n[1], n[0])) > 0.25).astype(np.float32)
     dx, dt, dm = DeviceBuffer(eng, x.nbytes), DeviceBuffer(eng, t.nbytes), DeviceBuffer(eng, mask.nbytes)
@@ -83,58 +83,67 @@ def test_network_kernels_match_oracle():
 
 def test_gamma_field_features_and_guess(oracle_mod):
     from mdft import warmstart as ws
+    from mdft.gpu import DeviceBuffer
     from mdft.grid import angular_grid
     n, L, mmax = (9, 8, 10), (9.0, 8.0, 10.0), 3
-    say(f"[ws coupling] gamma field, projections, features, guess on {n} mmax={mmax}")
+    say(f"[ws coupling] gamma field, gamma projections, features, guess on {n} mmax={mmax}")
     t = ck.synthetic_ck(mmax, 256, 0.05, seed=7, n0=0.0333)
     eng = _engine(n, L, mmax)
     eng.set_ck(t)
     o = oracle_mod.Oracle(n, L, mmax, 2, 2.4777, 0.0333)
     o.set_ck(t)
     rng = np.random.default_rng(3)
-    N, no = eng.N, eng.no
+    N, no, npj = eng.N, eng.no, eng.np_
+    kT = 2.4777
     vext = rng.uniform(-8.0, 12.0, N * no)
     eng.set_vext(vext)
     xi = 1.0 + 0.3 * rng.uniform(-1, 1, N * no)   # no zeros: gamma is read off dF_exc / xi
     bx = eng.alloc_field()
     bx.upload(xi)
-    p4, gfield = ws.gamma(eng, bx, want_field=True)
+    tgt, gfield = ws.gamma(eng, bx, want_field=True)
     gam = gfield.download()
     _, dexc = o.excess(xi)
     g = angular_grid(mmax, 2)
     w = np.tile(g["w"], N)
-    gam_ref = -dexc / (2.0 * o.rho0 * xi * w * 2.4777)
+    gam_ref = -dexc / (2.0 * o.rho0 * xi * w * kT)
     err = np.abs(gam - gam_ref).max() / np.abs(gam_ref).max()
     say(f"  gamma field vs -dF_exc / (2 rho0 xi w kT): max rel err {err:.2e}")
     assert err < 1e-9
-    # basis and projections
-    th, ph = g["theta"], g["phi"]
-    B = np.stack([np.ones_like(th), np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
-    nrm = (g["w"] * B * B).sum(axis=1)
-    assert np.allclose(B @ (g["w"][:, None] * B.T), np.diag(nrm), atol=1e-12)   # orthogonal on the quadrature
-    Bw = g["w"] * B / nrm[:, None]
-    proj = gam.reshape(N, no) @ Bw.T
-    got = p4.download(np.float32).reshape(4, N).T
-    assert np.abs(got - proj).max() < 1e-6 * max(1.0, np.abs(proj).max())
-    f = ws.features(eng).download(np.float32).reshape(8, N).T
-    bv = vext.reshape(N, no) / 2.4777
-    f1 = (np.exp(-0.5 * np.maximum(bv, -8.0)) - 1.0) @ Bw.T
-    f2 = np.tanh(-bv / 8.0) @ Bw.T
-    assert np.abs(f[:, :4] - f1).max() < 1e-5 * max(1.0, np.abs(f1).max())
-    assert np.abs(f[:, 4:] - f2).max() < 1e-6
-    # guess: xi = exp((-beta v + pred . B) / 2)
-    from mdft.gpu import DeviceBuffer
-    pred = rng.normal(0, 0.5, (4, N)).astype(np.float32)
+    # projections of gamma = (2 (4 pi^2) / (mrso n0)) * the stage-4 projections
+    scale = 2.0 * 4.0 * np.pi ** 2 / 2 / 0.0333
+    P4 = o.excess_stage(xi, 4)                       # [N][np] complex
+    ref = np.empty((2 * npj, N))
+    ref[0::2], ref[1::2] = scale * P4.real.T, scale * P4.imag.T
+    got = tgt.download(np.float32).reshape(2 * npj, N)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say(f"  gamma projections: max rel err {err:.2e}")
+    assert err < 1e-6
+    # rebuilding gamma from its projections gives the gamma field back
+    rebuilt = np.concatenate([o.proj2angl(P4[i] * scale) for i in range(N)])
+    assert np.abs(rebuilt - gam).max() < 1e-9 * np.abs(gam).max()
+    # features: projections of exp(-max(beta v, -8) / 2) - 1
+    f = ws.features(eng).download(np.float32).reshape(2 * npj, N)
+    fo = (np.exp(-0.5 * np.maximum(vext / kT, -8.0)) - 1.0).reshape(N, no)
+    fp = np.stack([o.angl2proj(fo[i]) for i in range(N)])
+    err = max(np.abs(f[0::2] - fp.real.T).max(), np.abs(f[1::2] - fp.imag.T).max()) / np.abs(fp).max()
+    say(f"  features: max rel err {err:.2e}")
+    assert err < 1e-6
+    # guess from predicted projections: xi = exp(clip(-beta v + proj2angl(pred), -200, 7) / 2)
+    pred = (rng.normal(0, 0.3, (2 * npj, N)) * (np.abs(ref).max() / 10)).astype(np.float32)
     bp = DeviceBuffer(eng, pred.nbytes)
     bp.upload(pred)
     ws.guess(eng, bp, bx)
-    lg = np.clip(-bv + pred.T.astype(np.float64) @ B, -200.0, 7.0)
-    assert np.abs(bx.download() - np.exp(0.5 * lg).ravel()).max() < 1e-12 * np.exp(3.5)
+    pc = pred[0::2].T.astype(np.float64) + 1j * pred[1::2].T.astype(np.float64)
+    gp = np.concatenate([o.proj2angl(pc[i]) for i in range(N)])
+    lg = np.clip(-vext / kT + gp, -200.0, 7.0)
+    err = np.abs(bx.download() - np.exp(0.5 * lg)).max() / np.exp(3.5)
+    say(f"  guess: max err / e^3.5 {err:.2e}")
+    assert err < 1e-6
     # device cold start = the host restatement of the reference's guess
     vext[::7] = 700.0
     eng.set_vext(vext)
     ws.cold_start(eng, bx)
-    assert np.array_equal(bx.download(), workloads.guess_xi(vext, 2.4777))
+    assert np.array_equal(bx.download(), workloads.guess_xi(vext, kT))
 
 
 def test_euler_lagrange_at_the_minimum():
@@ -193,7 +202,7 @@ def test_trained_network_reduces_iterations():
         ws.cold_start(eng, xi)
         minimize(eng, xi, g=gb, m=5, gtol=1e-5, maxeval=800)
         data.append((f, ws.gamma(eng, xi)))
-    net = ws.WarmStartNet(seed=1)
+    net = ws.WarmStartNet.for_engine(eng, seed=1)
     hist = ws.train(net, w.n, data, epochs=30, lr=2e-3, seed=0)
     say(f"  loss: first epoch {hist[0]:.4f}, last {hist[-1]:.4f}")
     assert hist[-1] < 0.5 * hist[0]

@@ -14,7 +14,8 @@ def _torch_net(weights, biases, dils, x):
     Ws = [torch.tensor(w, dtype=torch.float64, requires_grad=True) for w in weights]
     bs = [torch.tensor(b, dtype=torch.float64, requires_grad=True) for b in biases]
     for li, (w, b, d) in enumerate(zip(Ws, bs, dils)):
-        a = F.conv3d(F.pad(a, (d,) * 6, mode="circular"), w, b, dilation=d)
+        pad = d * (w.shape[2] // 2)
+        a = F.conv3d(F.pad(a, (pad,) * 6, mode="circular") if pad else a, w, b, dilation=d)
         if li < len(Ws) - 1:
             a = torch.relu(a)
     return a[0], Ws, bs
@@ -23,10 +24,11 @@ def _torch_net(weights, biases, dils, x):
 @pytest.fixture(scope="module")
 def small_net():
     rng = np.random.default_rng(0)
-    chans = [3, 5, 4, 2]
-    dils = [1, 2, 3]
-    weights = [rng.normal(0, 0.3, (chans[i + 1], chans[i], 3, 3, 3)) for i in range(3)]
-    biases = [rng.normal(0, 0.1, chans[i + 1]) for i in range(3)]
+    chans = [3, 5, 4, 6, 2]
+    ks = [1, 3, 3, 1]
+    dils = [1, 2, 3, 1]
+    weights = [rng.normal(0, 0.3, (chans[i + 1], chans[i], k, k, k)) for i, k in enumerate(ks)]
+    biases = [rng.normal(0, 0.1, chans[i + 1]) for i in range(4)]
     x = rng.normal(size=(3, 7, 6, 9))
     t = rng.normal(size=(2, 7, 6, 9))
     mask = (rng.uniform(size=(1, 7, 6, 9)) > 0.3).astype(np.float64)
