@@ -137,6 +137,10 @@ int mdft_vec_axpy_dot(mdft_ctx *ctx, size_t n, double a, const double *x, double
                       double *out);
 /* z = a*x + b*y (y nullable: z = a*x); z may alias x or y */
 int mdft_vec_lincomb(mdft_ctx *ctx, size_t n, double a, const double *x, double b, const double *y, double *z);
+/* One pass over x and k <= 16 device vectors: out[j] = x.vecs[j] (out on the host). */
+int mdft_vec_multi_dot(mdft_ctx *ctx, size_t n, int k, const double *const *vecs, const double *x, double *out);
+/* One pass: z = sum_j coef[j] vecs[j], k <= 16 (z may be one of vecs). */
+int mdft_vec_combine(mdft_ctx *ctx, size_t n, int k, const double *coef, const double *const *vecs, double *z);
 
 /* Device memory owned by the caller, for device-resident fields
  * (MDFT_DEVICE_PTRS) and the device L-BFGS vectors. */

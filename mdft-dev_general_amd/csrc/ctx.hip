@@ -475,7 +475,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
     void *bufs[] = {c->d_w,   c->d_a2p,  c->d_p2a,  c->d_fm,   c->d_cph,      c->d_sph,        c->d_cps,
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
-                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_oz32_ftab, c->d_oz32_lanerow,
+                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
                     c->d_bf16_M, c->d_bf16_Q};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -778,6 +778,22 @@ int mdft_vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *
 int mdft_vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z) {
     if (!c || !x || !z) return MDFT_EINVAL;
     HIPCHK(c, vec_lincomb(c, n, a, x, b, y, z));
+    return MDFT_OK;
+}
+
+int mdft_vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out) {
+    if (!c || !vecs || !x || !out || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    for (int j = 0; j < k; j++)
+        if (!vecs[j]) return fail(c, MDFT_EINVAL, "null vector");
+    HIPCHK(c, vec_multi_dot(c, n, k, vecs, x, out));
+    return MDFT_OK;
+}
+
+int mdft_vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z) {
+    if (!c || !coef || !vecs || !z || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    for (int j = 0; j < k; j++)
+        if (!vecs[j]) return fail(c, MDFT_EINVAL, "null vector");
+    HIPCHK(c, vec_combine(c, n, k, coef, vecs, z));
     return MDFT_OK;
 }
 

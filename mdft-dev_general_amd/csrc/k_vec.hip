@@ -99,6 +99,69 @@ __global__ __launch_bounds__(kThreads) void widen(size_t n, const float2 *__rest
     }
 }
 
+// Up to kMaxVec vectors in one pass (vector-free L-BFGS): multi-dot and linear combination.
+struct VecSet {
+    const double *p[MDFT_VEC_MAX];
+    double c[MDFT_VEC_MAX];
+    int k;
+};
+
+// part[blk][j] = sum_i x[i] v_j[i]
+__global__ __launch_bounds__(kThreads) void multi_dot_partial(size_t n, const double *__restrict__ x, VecSet v,
+                                                             double *__restrict__ part) {
+    double acc[MDFT_VEC_MAX];
+#pragma unroll
+    for (int j = 0; j < MDFT_VEC_MAX; j++) acc[j] = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const double xv = x[i];
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) acc[j] += xv * v.p[j][i];
+    }
+    __shared__ double red[MDFT_VEC_MAX][kThreads / 64];
+#pragma unroll
+    for (int j = 0; j < MDFT_VEC_MAX; j++) {
+        if (j >= v.k) break;
+        const double s = wave_sum(acc[j]);
+        if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < v.k) {
+        double s = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) s += red[threadIdx.x][w];
+        part[(size_t)blockIdx.x * MDFT_VEC_MAX + threadIdx.x] = s;
+    }
+}
+
+// out[j] = sum_b part[b][j], fixed order
+__global__ __launch_bounds__(kThreads) void multi_dot_final(const double *__restrict__ part, int nb, int k,
+                                                           double *__restrict__ out) {
+    __shared__ double red[kThreads];
+    for (int j = 0; j < k; j++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kThreads) s += part[(size_t)b * MDFT_VEC_MAX + j];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = kThreads / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j] = red[0];
+        __syncthreads();
+    }
+}
+
+// z = sum_j c_j v_j   (z may be one of the v_j: each element is read before it is written)
+__global__ __launch_bounds__(kThreads) void combine(size_t n, VecSet v, double *z) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) s += v.c[j] * v.p[j][i];
+        z[i] = s;
+    }
+}
+
 unsigned blocks_for(size_t n) {
     size_t b = (n / 2 + kThreads - 1) / kThreads;
     if (b < 1) b = 1;
@@ -154,4 +217,36 @@ hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double 
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n) {
     hipLaunchKernelGGL(widen, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
     return hipGetLastError();
+}
+
+hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out) {
+    if (!c->d_mpart) {
+        hipError_t e = hipMalloc(&c->d_mpart, sizeof(double) * ((size_t)kMaxBlocks * MDFT_VEC_MAX + MDFT_VEC_MAX));
+        if (e != hipSuccess) return e;
+    }
+    VecSet v{};
+    for (int j = 0; j < k; j++) v.p[j] = vecs[j];
+    v.k = k;
+    const unsigned nb = blocks_for(n);
+    hipLaunchKernelGGL(multi_dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    double *dout = c->d_mpart + (size_t)kMaxBlocks * MDFT_VEC_MAX;
+    hipLaunchKernelGGL(multi_dot_final, dim3(1), dim3(kThreads), 0, c->stream, c->d_mpart, (int)nb, k, dout);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(out, dout, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z) {
+    VecSet v{};
+    for (int j = 0; j < k; j++) {
+        v.p[j] = vecs[j];
+        v.c[j] = coef[j];
+    }
+    v.k = k;
+    hipLaunchKernelGGL(combine, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
 }

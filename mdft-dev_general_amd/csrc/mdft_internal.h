@@ -10,6 +10,7 @@
 #include "mdft_gpu.h"
 
 #define MDFT_MAX_MMAX 6
+#define MDFT_VEC_MAX 16  // vectors per multi-dot / combine pass
 #define MDFT_OZ32_ROWS_PER_LANE 4  // OZ rows per lane in the FP32 q-space kernel (np <= 256)
 #define MDFT_FFT_MAX_STAGES 12
 
@@ -82,6 +83,7 @@ struct mdft_ctx {
     double *d_slices = nullptr;   // [3 * 512] first-pass sums of d_partials
     double *d_sums = nullptr;     // [8] final reductions
     double *d_vpart = nullptr;    // vector-op block partials
+    double *d_mpart = nullptr;    // multi-dot block partials [blocks][MDFT_VEC_MAX] + result
     double *h_sums = nullptr;     // pinned mirror of d_sums
     size_t partials_cap = 0;
     bool have_vext = false, have_ck = false;
@@ -123,5 +125,7 @@ int fft_make_plan(int n, int *radix);
 hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out);
 hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out);
 hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z);
+hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out);
+hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z);
 hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,
                        const double *d_site_off, double rcut, double qfact, double clip);

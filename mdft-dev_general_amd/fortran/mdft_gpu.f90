@@ -116,11 +116,28 @@ module mdft_gpu
             integer(c_size_t), value :: n
             real(c_double), value :: a, b
         end function
+        integer(c_int) function mdft_vec_multi_dot(ctx, n, k, vecs, x, out) bind(C, name="mdft_vec_multi_dot")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, x
+            integer(c_size_t), value :: n
+            integer(c_int), value :: k
+            type(c_ptr), intent(in) :: vecs(*)
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function mdft_vec_combine(ctx, n, k, coef, vecs, z) bind(C, name="mdft_vec_combine")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, z
+            integer(c_size_t), value :: n
+            integer(c_int), value :: k
+            real(c_double), intent(in) :: coef(*)
+            type(c_ptr), intent(in) :: vecs(*)
+        end function
     end interface
 
     public :: mdft_ctx_create, mdft_ctx_destroy, mdft_get_info, mdft_set_ck, mdft_compute_vext, mdft_set_vext
     public :: mdft_energy_and_gradient, mdft_excess_energy, mdft_device_alloc, mdft_device_free
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
+    public :: mdft_vec_multi_dot, mdft_vec_combine
     public :: check, last_error_text
 
     ! ---- module state, the analogue of the reference's solvent(1)%xi and grid globals

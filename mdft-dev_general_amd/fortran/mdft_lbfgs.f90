@@ -3,21 +3,26 @@
 ! Replaces the minimisation loop of src/energy_minimization.f90:232-290 and the
 ! reverse-communication L-BFGS-B of src/module_lbfgs_nocedal_mdft.f90 (used
 ! there unconstrained, with memory m = 1) for the device path: all length-n
-! vectors (xi, the gradient, the search direction, the m correction pairs)
-! stay in HBM and are combined with the library's FP64 vector operations; the
-! scalars of the two-loop recursion and of the line search are FP64 on the
-! host.  The line search enforces the strong Wolfe conditions with cubic
+! vectors stay in HBM.  The recursion is the vector-free form of L-BFGS (Chen,
+! Wang, Zhou & Gao, NIPS 2014): the correction pairs are kept implicitly as the
+! search directions d_i (s_i = -t_i d_i) and the gradients g_i (y_i = g_{i+1} - g_i),
+! their FP64 Gram matrix is kept on the host, and the two-loop recursion runs on
+! coefficient vectors of length <= 2m+3.  Per iteration the device does one
+! linear combination of <= 2m+1 vectors (the new direction), one step update
+! and one multi-dot of the new gradient with the <= 2m+3 live vectors -- about
+! 28 passes over n instead of the ~66 of the two-loop recursion on vectors.  The line search enforces the strong Wolfe conditions with cubic
 ! interpolation inside a bracket (the role of dcsrch,
 ! src/module_lbfgs_nocedal_mdft.f90:1653).  Stopping rule: |dF|/|dF_0| < gtol,
 ! or a relative decrease of F below ftol, or maxeval evaluations.
 module mdft_lbfgs
     use iso_c_binding
-    use mdft_gpu, only: mdft_energy_and_gradient, mdft_device_alloc, mdft_device_free, mdft_vec_dot, &
-                        mdft_vec_axpy_dot, mdft_vec_lincomb, mdft_get_info, MDFT_DEVICE_PTRS
+    use mdft_gpu, only: mdft_energy_and_gradient, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
+                        mdft_vec_multi_dot, mdft_vec_combine, MDFT_DEVICE_PTRS
     implicit none
     private
     integer, parameter :: dp = c_double
-    integer, parameter :: mmem_max = 32
+    integer, parameter :: mmem_max = 6           ! 2 m + 3 live vectors per multi-dot <= 16
+    integer, parameter :: nslot = 2 * mmem_max + 3
     public :: mdft_minimize
 
 contains
@@ -43,11 +48,16 @@ contains
         real(c_double), value :: gtol, ftol, fnoise, ftarget, fband
         real(c_double), intent(out) :: stats(16)
         real(c_double), intent(out) :: trace(3, *)
-        type(c_ptr) :: s(mmem_max), y(mmem_max), d, xn, gn, tmp, xi_user, g_user, xn_own, gn_own
-        real(dp) :: rho(mmem_max), alpha(mmem_max)
-        real(dp) :: parts(3), pn(3), f, fn, gnorm, g0, gd, gdn, t, dotv, beta, ys, yy, gam
-        real(dp) :: f_prev
-        integer :: k, i, j, nstored, newest, neval, iter, status, slot, mm
+        ! vector slots: 1..mm+1 hold directions, mm+2..2mm+3 gradients (slot mm+2 starts as the caller's g)
+        type(c_ptr) :: vec(nslot), x, xn, tmp, xn_own, pv(16)
+        logical :: owned(nslot)
+        real(dp) :: gram(nslot, nslot), tstep(nslot)
+        integer :: ds(mmem_max + 1), gs(mmem_max + 2)   ! window: direction slots of pairs 1..k, gradient slots 1..k+1
+        real(dp) :: q(nslot), r(nslot), sv(nslot), yv(nslot), alpha(mmem_max), rho(mmem_max)
+        real(dp) :: coef(16), dots(16)
+        integer :: lst(16)
+        real(dp) :: parts(3), pn(3), f, fn, gnorm, g0, gd, gdn, t, beta, sy, yy, gam, f_prev
+        integer :: i, j, k, mm, nb, nl, fd, fg, gcur, neval, iter, status
         integer(c_size_t) :: bytes
         logical :: ok
 
@@ -55,29 +65,39 @@ contains
         stats = 0
         mm = max(1, min(int(m), mmem_max))
         bytes = 8_c_size_t * n
-        do i = 1, mm
-            rc = mdft_device_alloc(ctx, bytes, s(i)); if (rc /= 0) return
-            rc = mdft_device_alloc(ctx, bytes, y(i)); if (rc /= 0) return
+        owned = .false.
+        xn_own = c_null_ptr
+        do i = 1, 2 * mm + 3
+            vec(i) = c_null_ptr
         end do
-        rc = mdft_device_alloc(ctx, bytes, d); if (rc /= 0) return
-        rc = mdft_device_alloc(ctx, bytes, xn); if (rc /= 0) return
-        rc = mdft_device_alloc(ctx, bytes, gn); if (rc /= 0) return
-        xi_user = xi
-        g_user = g
-        xn_own = xn
-        gn_own = gn
+        do i = 1, 2 * mm + 3
+            if (i == mm + 2) then
+                vec(i) = g
+            else
+                rc = mdft_device_alloc(ctx, bytes, vec(i)); if (rc /= 0) goto 900
+                owned(i) = .true.
+            end if
+        end do
+        rc = mdft_device_alloc(ctx, bytes, xn_own); if (rc /= 0) goto 900
+        x = xi
+        xn = xn_own
+        gram = 0.0_dp
+        tstep = 0.0_dp
 
         neval = 0
         iter = 0
         status = 2
-        rc = mdft_energy_and_gradient(ctx, xi, parts, g, MDFT_DEVICE_PTRS); if (rc /= 0) goto 900
+        gcur = mm + 2
+        rc = mdft_energy_and_gradient(ctx, x, parts, vec(gcur), MDFT_DEVICE_PTRS); if (rc /= 0) goto 900
         neval = 1
         f = sum(parts)
-        rc = mdft_vec_dot(ctx, n, g, g, dotv); if (rc /= 0) goto 900
-        g0 = sqrt(dotv)
+        pv(1) = vec(gcur)
+        rc = mdft_vec_multi_dot(ctx, n, 1, pv, vec(gcur), dots); if (rc /= 0) goto 900
+        gram(gcur, gcur) = dots(1)
+        g0 = sqrt(dots(1))
         gnorm = g0
-        nstored = 0
-        newest = 0
+        k = 0
+        gs(1) = gcur
         gam = 1.0_dp
         f_prev = huge(1.0_dp)
         if (g0 == 0.0_dp) then
@@ -90,77 +110,87 @@ contains
                 status = 0
                 exit
             end if
-            ! ---- two-loop recursion: d = H g (search direction is -d)
-            rc = mdft_vec_lincomb(ctx, n, 1.0_dp, g, 0.0_dp, c_null_ptr, d); if (rc /= 0) goto 900
-            if (nstored > 0) then
-                rc = mdft_vec_dot(ctx, n, s(newest), d, dotv); if (rc /= 0) goto 900
-                do j = 0, nstored - 1
-                    slot = modulo(newest - 1 - j, mm) + 1
-                    alpha(slot) = rho(slot) * dotv
-                    if (j < nstored - 1) then
-                        i = modulo(slot - 2, mm) + 1
-                        rc = mdft_vec_axpy_dot(ctx, n, -alpha(slot), y(slot), d, s(i), dotv)
-                    else
-                        rc = mdft_vec_axpy_dot(ctx, n, -alpha(slot), y(slot), d, c_null_ptr, dotv)
-                    end if
-                    if (rc /= 0) goto 900
-                end do
-                rc = mdft_vec_lincomb(ctx, n, gam, d, 0.0_dp, c_null_ptr, d); if (rc /= 0) goto 900
-                slot = modulo(newest - nstored, mm) + 1
-                rc = mdft_vec_dot(ctx, n, y(slot), d, dotv); if (rc /= 0) goto 900
-                do j = nstored - 1, 0, -1
-                    slot = modulo(newest - 1 - j, mm) + 1
-                    beta = rho(slot) * dotv
-                    if (j > 0) then
-                        i = modulo(slot, mm) + 1
-                        rc = mdft_vec_axpy_dot(ctx, n, alpha(slot) - beta, s(slot), d, y(i), dotv)
-                    else
-                        rc = mdft_vec_axpy_dot(ctx, n, alpha(slot) - beta, s(slot), d, c_null_ptr, dotv)
-                    end if
-                    if (rc /= 0) goto 900
-                end do
-                t = 1.0_dp
-            else
-                t = 1.0_dp / gnorm       ! first step: unit-length move along -g
+            ! ---- two-loop recursion on coefficients: r = H g (search direction is -r)
+            q = 0.0_dp
+            q(gcur) = 1.0_dp
+            do i = k, 1, -1
+                call pair(i, sv, yv)
+                rho(i) = 1.0_dp / gdot(sv, yv)
+                alpha(i) = rho(i) * gdot(sv, q)
+                q = q - alpha(i) * yv
+            end do
+            gam = 1.0_dp
+            if (k > 0) then
+                call pair(k, sv, yv)
+                gam = gdot(sv, yv) / gdot(yv, yv)
             end if
-            rc = mdft_vec_dot(ctx, n, g, d, gd); if (rc /= 0) goto 900
-            gd = -gd                     ! directional derivative along -d
-            if (gd >= 0.0_dp) then       ! not a descent direction: restart from steepest descent
-                nstored = 0
-                rc = mdft_vec_lincomb(ctx, n, 1.0_dp, g, 0.0_dp, c_null_ptr, d); if (rc /= 0) goto 900
+            r = gam * q
+            do i = 1, k
+                call pair(i, sv, yv)
+                beta = rho(i) * gdot(yv, r)
+                r = r + (alpha(i) - beta) * sv
+            end do
+            t = 1.0_dp
+            if (k == 0) t = 1.0_dp / gnorm       ! first step: unit-length move along -g
+            gd = -gdot(unit(gcur), r)            ! directional derivative along -d
+            if (gd >= 0.0_dp) then               ! not a descent direction: restart from steepest descent
+                k = 0
+                gs(1) = gcur
+                r = unit(gcur)
                 gd = -gnorm * gnorm
                 t = 1.0_dp / gnorm
             end if
+            ! ---- the new direction d = sum_j r_j b_j into a free direction slot
+            fd = free_direction()
+            nb = 0
+            do j = 1, 2 * mm + 3
+                if (r(j) /= 0.0_dp) then
+                    nb = nb + 1
+                    pv(nb) = vec(j)
+                    coef(nb) = r(j)
+                end if
+            end do
+            rc = mdft_vec_combine(ctx, n, nb, coef, pv, vec(fd)); if (rc /= 0) goto 900
+            do j = 1, 2 * mm + 3
+                if (j == fd) cycle
+                gram(fd, j) = dot_product(r, gram(:, j))
+                gram(j, fd) = gram(fd, j)
+            end do
+            gram(fd, fd) = gdot(r, r)
+            fg = free_gradient()
             call line_search(ok)
             if (rc /= 0) goto 900
             if (.not. ok) then
-                if (nstored > 0) then    ! forget the memory once and retry along -g
-                    nstored = 0
+                if (k > 0) then                  ! forget the memory once and retry along -g
+                    k = 0
+                    gs(1) = gcur
                     cycle
                 end if
                 status = 3
                 exit
             end if
-            ! ---- update the correction pairs: s = xn - x = -t d, y = gn - g
-            newest = modulo(newest, mm) + 1
-            rc = mdft_vec_lincomb(ctx, n, -t, d, 0.0_dp, c_null_ptr, s(newest)); if (rc /= 0) goto 900
-            rc = mdft_vec_lincomb(ctx, n, 1.0_dp, gn, -1.0_dp, g, y(newest)); if (rc /= 0) goto 900
-            rc = mdft_vec_dot(ctx, n, y(newest), s(newest), ys); if (rc /= 0) goto 900
-            rc = mdft_vec_dot(ctx, n, y(newest), y(newest), yy); if (rc /= 0) goto 900
-            if (ys > 1.0e-300_dp) then
-                rho(newest) = 1.0_dp / ys
-                gam = ys / yy
-                nstored = min(nstored + 1, mm)
+            ! ---- accept: pair (s = -t d, y = g_new - g)
+            sy = -t * (gram(fd, fg) - gram(fd, gcur))
+            if (sy > 1.0e-300_dp) then
+                tstep(fd) = t
+                k = k + 1
+                ds(k) = fd
+                gs(k + 1) = fg
+                if (k > mm) then                 ! drop the oldest pair
+                    ds(1:k - 1) = ds(2:k)
+                    gs(1:k) = gs(2:k + 1)
+                    k = k - 1
+                end if
             else
-                newest = modulo(newest - 2, mm) + 1
+                k = 0
+                gs(1) = fg
             end if
-            tmp = xi; xi = xn; xn = tmp
-            tmp = g; g = gn; gn = tmp
+            gcur = fg
+            tmp = x; x = xn; xn = tmp
             f_prev = f
             f = fn
             parts = pn
-            rc = mdft_vec_dot(ctx, n, g, g, dotv); if (rc /= 0) goto 900
-            gnorm = sqrt(dotv)
+            gnorm = sqrt(gram(gcur, gcur))
             iter = iter + 1
             if (iter <= ntrace) then
                 trace(1, iter) = f
@@ -188,11 +218,89 @@ contains
         stats(9) = g0
 900     continue
         ! the caller's xi/g buffers must hold the result
-        if (rc == 0 .and. .not. c_associated(xi, xi_user)) rc = mdft_vec_lincomb(ctx, n, 1.0_dp, xi, 0.0_dp, c_null_ptr, xi_user)
-        if (rc == 0 .and. .not. c_associated(g, g_user)) rc = mdft_vec_lincomb(ctx, n, 1.0_dp, g, 0.0_dp, c_null_ptr, g_user)
+        if (rc == 0 .and. .not. c_associated(x, xi)) rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, 0.0_dp, c_null_ptr, xi)
+        if (rc == 0 .and. .not. c_associated(vec(gcur), g)) &
+            rc = mdft_vec_lincomb(ctx, n, 1.0_dp, vec(gcur), 0.0_dp, c_null_ptr, g)
         call release()
 
     contains
+
+        ! a^T G b over the slot coefficients
+        real(dp) function gdot(a, b)
+            real(dp), intent(in) :: a(nslot), b(nslot)
+            gdot = dot_product(a, matmul(gram, b))
+        end function
+
+        function unit(j) result(e)
+            integer, intent(in) :: j
+            real(dp) :: e(nslot)
+            e = 0.0_dp
+            e(j) = 1.0_dp
+        end function
+
+        ! coefficients of s_i = -t_i d_i and y_i = g_{i+1} - g_i
+        subroutine pair(i, a, b)
+            integer, intent(in) :: i
+            real(dp), intent(out) :: a(nslot), b(nslot)
+            a = 0.0_dp
+            b = 0.0_dp
+            a(ds(i)) = -tstep(ds(i))
+            b(gs(i + 1)) = b(gs(i + 1)) + 1.0_dp
+            b(gs(i)) = b(gs(i)) - 1.0_dp
+        end subroutine
+
+        integer function free_direction()
+            integer :: a
+            do a = 1, mm + 1
+                if (.not. any(ds(1:k) == a)) then
+                    free_direction = a
+                    return
+                end if
+            end do
+            free_direction = mm + 1
+        end function
+
+        integer function free_gradient()
+            integer :: a
+            do a = mm + 2, 2 * mm + 3
+                if (.not. any(gs(1:k + 1) == a)) then
+                    free_gradient = a
+                    return
+                end if
+            end do
+            free_gradient = 2 * mm + 3
+        end function
+
+        ! evaluate at xn = x - t d into slot fg and its Gram row (one multi-dot with the live vectors)
+        subroutine trial()
+            integer :: a
+            rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, -t, vec(fd), xn); if (rc /= 0) return
+            rc = mdft_energy_and_gradient(ctx, xn, pn, vec(fg), MDFT_DEVICE_PTRS); if (rc /= 0) return
+            neval = neval + 1
+            fn = sum(pn)
+            nl = 0
+            do a = 1, k
+                nl = nl + 1
+                lst(nl) = ds(a)
+            end do
+            nl = nl + 1
+            lst(nl) = fd
+            do a = 1, k + 1
+                nl = nl + 1
+                lst(nl) = gs(a)
+            end do
+            nl = nl + 1
+            lst(nl) = fg
+            do a = 1, nl
+                pv(a) = vec(lst(a))
+            end do
+            rc = mdft_vec_multi_dot(ctx, n, nl, pv, vec(fg), dots); if (rc /= 0) return
+            do a = 1, nl
+                gram(fg, lst(a)) = dots(a)
+                gram(lst(a), fg) = dots(a)
+            end do
+            gdn = -gram(fg, fd)
+        end subroutine
 
         subroutine line_search(ok)
             logical, intent(out) :: ok
@@ -211,12 +319,7 @@ contains
             bracketed = .false.
             do it = 1, 30
                 if (neval >= maxeval) return
-                rc = mdft_vec_lincomb(ctx, n, 1.0_dp, xi, -t, d, xn); if (rc /= 0) return
-                rc = mdft_energy_and_gradient(ctx, xn, pn, gn, MDFT_DEVICE_PTRS); if (rc /= 0) return
-                neval = neval + 1
-                fn = sum(pn)
-                rc = mdft_vec_dot(ctx, n, gn, d, gdn); if (rc /= 0) return
-                gdn = -gdn
+                call trial(); if (rc /= 0) return
                 if (fn > f + c1 * t * gd + epsf .or. (bracketed .and. fn >= flo + epsf)) then
                     thi = t; fhi = fn; ghi = gdn
                     bracketed = .true.
@@ -248,10 +351,7 @@ contains
             ! accept the best Armijo point found, if any
             if (tlo > 0.0_dp .and. flo < f) then
                 t = tlo
-                rc = mdft_vec_lincomb(ctx, n, 1.0_dp, xi, -t, d, xn); if (rc /= 0) return
-                rc = mdft_energy_and_gradient(ctx, xn, pn, gn, MDFT_DEVICE_PTRS); if (rc /= 0) return
-                neval = neval + 1
-                fn = sum(pn)
+                call trial(); if (rc /= 0) return
                 ok = .true.
             end if
         end subroutine
@@ -270,15 +370,12 @@ contains
         end function
 
         subroutine release()
-            integer :: q
+            integer :: a
             integer(c_int) :: r2
-            do q = 1, mm
-                r2 = mdft_device_free(ctx, s(q))
-                r2 = mdft_device_free(ctx, y(q))
+            do a = 1, 2 * mm + 3
+                if (owned(a)) r2 = mdft_device_free(ctx, vec(a))
             end do
-            r2 = mdft_device_free(ctx, d)
-            r2 = mdft_device_free(ctx, gn_own)
-            r2 = mdft_device_free(ctx, xn_own)
+            if (c_associated(xn_own)) r2 = mdft_device_free(ctx, xn_own)
         end subroutine
 
     end function mdft_minimize

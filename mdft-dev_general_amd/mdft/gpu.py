@@ -27,7 +27,8 @@ MDFT_EXPORTS = (
     "mdft_set_vext", "mdft_set_ck", "mdft_energy_and_gradient", "mdft_excess_energy",
     "mdft_ideal_external", "mdft_excess_stage", "mdft_phase_times", "mdft_device_alloc",
     "mdft_device_free", "mdft_memcpy_h2d", "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext",
-    "mdft_get_vext", "mdft_vec_dot", "mdft_vec_axpy_dot", "mdft_vec_lincomb",
+    "mdft_get_vext", "mdft_vec_dot", "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot",
+    "mdft_vec_combine",
 )
 
 
@@ -83,11 +84,13 @@ def load_library(path: str = LIBPATH):
     lib.mdft_vec_dot.argtypes = [vp, C.c_size_t, vp, vp, _dp]
     lib.mdft_vec_axpy_dot.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp, _dp]
     lib.mdft_vec_lincomb.argtypes = [vp, C.c_size_t, C.c_double, vp, C.c_double, vp, vp]
+    lib.mdft_vec_multi_dot.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(vp), vp, _dp]
+    lib.mdft_vec_combine.argtypes = [vp, C.c_size_t, C.c_int, _dp, C.POINTER(vp), vp]
     for name in ("mdft_get_info", "mdft_get_quadrature", "mdft_set_vext", "mdft_set_ck",
                  "mdft_energy_and_gradient", "mdft_excess_energy", "mdft_ideal_external", "mdft_excess_stage",
                  "mdft_phase_times", "mdft_device_alloc", "mdft_device_free", "mdft_memcpy_h2d",
                  "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext", "mdft_get_vext", "mdft_vec_dot",
-                 "mdft_vec_axpy_dot", "mdft_vec_lincomb"):
+                 "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib

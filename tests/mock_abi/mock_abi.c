@@ -132,3 +132,27 @@ int mdft_vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b,
     for (size_t i = 0; i < n; i++) z[i] = a * x[i] + (y ? b * y[i] : 0.0);
     return 0;
 }
+
+int mdft_vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out)
+{
+    (void)c;
+    if (k < 1 || k > 16) return -1;
+    for (int j = 0; j < k; j++) {
+        double s = 0.0;
+        for (size_t i = 0; i < n; i++) s += x[i] * vecs[j][i];
+        out[j] = s;
+    }
+    return 0;
+}
+
+int mdft_vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z)
+{
+    (void)c;
+    if (k < 1 || k > 16) return -1;
+    for (size_t i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < k; j++) s += coef[j] * vecs[j][i];
+        z[i] = s;
+    }
+    return 0;
+}

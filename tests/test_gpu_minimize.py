@@ -29,7 +29,7 @@ def test_device_vext_matches_host():
 
 
 def test_vector_ops():
-    from mdft.gpu import Engine
+    from mdft.gpu import DeviceBuffer, Engine
     eng = Engine((8, 8, 8), (8.0, 8.0, 8.0), 2, 2, 2.4, 0.0333)
     rng = np.random.default_rng(0)
     n = eng.N * eng.no
@@ -44,6 +44,22 @@ def test_vector_ops():
     assert np.allclose(db.download(), b + 0.5 * a) and abs(out[0] - a @ (b + 0.5 * a)) < 1e-9
     eng._check(eng.lib.mdft_vec_lincomb(eng._h, n, 2.0, da.ptr, -1.0, db.ptr, db.ptr))
     assert np.allclose(db.download(), 2 * a - (b + 0.5 * a))
+    # multi-dot and linear combination over k vectors in one pass
+    k = 13
+    vs = [rng.standard_normal(n) for _ in range(k)]
+    bufs = []
+    for v in vs:
+        bb = DeviceBuffer(eng, 8 * n)
+        bb.upload(v)
+        bufs.append(bb)
+    ptrs = (C.c_void_p * k)(*[bb.ptr.value for bb in bufs])
+    dots = np.zeros(k)
+    eng._check(eng.lib.mdft_vec_multi_dot(eng._h, n, k, ptrs, da.ptr, dots.ctypes.data_as(C.POINTER(C.c_double))))
+    ref = np.array([a @ v for v in vs])
+    assert np.allclose(dots, ref, rtol=1e-10, atol=1e-9)
+    coef = rng.standard_normal(k)
+    eng._check(eng.lib.mdft_vec_combine(eng._h, n, k, coef.ctypes.data_as(C.POINTER(C.c_double)), ptrs, db.ptr))
+    assert np.allclose(db.download(), sum(c * v for c, v in zip(coef, vs)), rtol=1e-12, atol=1e-12)
 
 
 def test_lbfgs_reaches_the_cpu_minimum(oracle_mod):
