@@ -65,11 +65,21 @@ int mdft_ws_features(struct mdft_ctx *ctx, float *d_feat);
  * may be NULL) receives the projections of gamma. */
 int mdft_ws_gamma(struct mdft_ctx *ctx, const double *d_xi, double *d_gamma, float *d_target);
 
-/* d_xi [N][no] = exp((-beta vext + gamma) / 2), gamma rebuilt from its predicted
- * projections d_pred [2 np][N], exponent clipped to [-200, 7].  With d_pred
- * NULL: the reference's cold start (src/module_density.f90:267-284), xi = 0
- * where vext >= (-ln eps - 0.05) kT, else 1.  Both on the device. */
-int mdft_ws_guess(struct mdft_ctx *ctx, const float *d_pred, double *d_xi);
+/* d_target [2 np][N] (FP32): projections of rho/rho0 - 1 for the density xi (device). */
+int mdft_ws_density(struct mdft_ctx *ctx, const double *d_xi, float *d_target);
+
+enum { MDFT_WS_GAMMA = 0, MDFT_WS_DENSITY = 1 };
+
+/* Starting density from predicted projections d_pred [2 np][N]:
+ * kind MDFT_WS_DENSITY: d_pred are projections of rho/rho0 - 1, d_xi = sqrt(max(rho/rho0, 0));
+ * kind MDFT_WS_GAMMA: d_pred are projections of gamma, d_xi = exp((-beta vext + gamma) / 2),
+ * exponent clipped to [-200, 7].
+ * With d_pred NULL: the reference's cold start (src/module_density.f90:267-284),
+ * xi = 0 where vext >= (-ln eps - 0.05) kT, else 1.  All on the device. */
+int mdft_ws_guess(struct mdft_ctx *ctx, const float *d_pred, int kind, double *d_xi);
+
+/* d_out = sqrt((1 - lam) xa^2 + lam xb^2): xi of the density mixture (1 - lam) rho_a + lam rho_b. */
+int mdft_ws_mix(struct mdft_ctx *ctx, double lam, const double *d_xa, const double *d_xb, double *d_out);
 
 #ifdef __cplusplus
 }

@@ -865,16 +865,34 @@ int mdft_ws_gamma(mdft_ctx *c, const double *d_xi, double *d_gamma, float *d_tar
     return MDFT_OK;
 }
 
-int mdft_ws_guess(mdft_ctx *c, const float *d_pred, double *d_xi) {
+int mdft_ws_density(mdft_ctx *c, const double *d_xi, float *d_target) {
+    int rc = ready(c);
+    if (rc) return rc;
+    if (!d_xi || !d_target) return fail(c, MDFT_EINVAL, "null argument");
+    if ((rc = run_excess_pipeline(c, d_xi, 1, false))) return rc;  // projections of rho0 (xi^2 - 1)
+    HIPCHK(c, launch_proj_to_channels(c, d_target, (float)(1.0 / c->rho0)));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+int mdft_ws_guess(mdft_ctx *c, const float *d_pred, int kind, double *d_xi) {
     if (!c || !d_xi) return MDFT_EINVAL;
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
     if (!d_pred) {
         HIPCHK(c, launch_cold_start(c, d_xi));
     } else {
+        if (kind != MDFT_WS_GAMMA && kind != MDFT_WS_DENSITY) return fail(c, MDFT_EINVAL, "unknown guess kind");
         HIPCHK(c, launch_channels_to_proj(c, d_pred));
         int nb = 0;
-        HIPCHK(c, launch_final(c, nullptr, d_xi, 3, &nb));
+        HIPCHK(c, launch_final(c, nullptr, d_xi, kind == MDFT_WS_GAMMA ? 3 : 4, &nb));
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+int mdft_ws_mix(mdft_ctx *c, double lam, const double *d_xa, const double *d_xb, double *d_out) {
+    if (!c || !d_xa || !d_xb || !d_out) return MDFT_EINVAL;
+    HIPCHK(c, launch_density_mix(c, c->N * (size_t)c->no, lam, d_xa, d_xb, d_out));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }

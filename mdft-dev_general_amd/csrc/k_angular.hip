@@ -135,6 +135,7 @@ struct FinalConsts {
 // mode 1: df += excess gradient (excess_energy plug-in)
 // mode 2: df = gamma(r, Omega) = -(prefactor / kT) proj2angl(proj), the OZ field in units of ln(rho/rho0)
 // mode 3: proj holds gamma's projections, df = exp((-beta vext + proj2angl(proj)) / 2) (xi not read)
+// mode 4: proj holds the projections of rho/rho0 - 1, df = sqrt(max(1 + proj2angl(proj), 0)) (xi not read)
 template <int PT>
 __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict__ proj, const double *__restrict__ xi,
                                                      const double *__restrict__ vext, double *__restrict__ df,
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict_
         const double wio = t.w[io];
         const double vexc = k.prefactor * wio * v;
         const size_t g = (r0 + pt) * (size_t)no + io;
-        const double x = (mode == 3) ? 0.0 : xi[g];
+        const double x = (mode >= 3) ? 0.0 : xi[g];
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
         if (mode == 0) {
@@ -227,8 +228,10 @@ __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict_
             if (df) df[g] += 2.0 * rho0 * x * vexc;
         } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
             if (df) df[g] = -(k.prefactor / kT) * v;
-        } else {  // mode 3: proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
+        } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
             df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
+        } else {  // mode 4: proj holds the projections of rho/rho0 - 1; df = sqrt(max(rho/rho0, 0))
+            df[g] = sqrt(fmax(1.0 + v, 0.0));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -456,7 +459,7 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
         const size_t g = (r0 + pt) * (size_t)NO + io;
-        const double x = (mode == 3) ? 0.0 : xi[g];
+        const double x = (mode >= 3) ? 0.0 : xi[g];
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
         if (mode == 0) {
@@ -475,8 +478,10 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
             if (df) df[g] += 2.0 * rho0 * x * vexc;
         } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
             if (df) df[g] = -(k.prefactor / kT) * v;
-        } else {  // mode 3: proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
+        } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
             df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
+        } else {  // mode 4: proj holds the projections of rho/rho0 - 1; df = sqrt(max(rho/rho0, 0))
+            df[g] = sqrt(fmax(1.0 + v, 0.0));
         }
     }
     block_reduce3(fid, fext, fexc, partials);

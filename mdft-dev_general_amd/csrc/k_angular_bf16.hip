@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void final_bf16(const float *__restrict__
                 const double v = (double)acc[t][i];
                 const double vexc = k.prefactor * wio * v;
                 const size_t g = pr * NO + io;
-                const double x = (mode == 3) ? 0.0 : xi[g];
+                const double x = (mode >= 3) ? 0.0 : xi[g];
                 const double rho = x * x * rho0;
                 fexc += (rho - rho0) * vexc;
                 if (mode == 0) {
@@ -194,8 +194,10 @@ __global__ __launch_bounds__(kThreads) void final_bf16(const float *__restrict__
                     if (df) df[g] += 2.0 * rho0 * x * vexc;
                 } else if (mode == 2) {
                     if (df) df[g] = -(k.prefactor / kT) * v;
-                } else {
+                } else if (mode == 3) {
                     df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
+                } else {
+                    df[g] = sqrt(fmax(1.0 + v, 0.0));
                 }
             }
         }

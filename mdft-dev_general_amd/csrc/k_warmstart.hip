@@ -3,11 +3,11 @@
 // learned warm start.
 //
 // The reference has no learned component.  The network sees the projections
-// of a feature field of vext and predicts the projections of the OZ field
-// gamma; both live in the context's projection layout [N][np] complex (FP64 or
-// FP32) on the MDFT side and as 2 np real channels [2 np][N] (FP32, x fastest)
-// on the network side: channel 2 ip + 0 holds the real part of projection ip,
-// 2 ip + 1 the imaginary part.
+// of a feature field of vext and predicts the projections of the converged
+// density (rho/rho0 - 1) or of the OZ field gamma; both live in the context's
+// projection layout [N][np] complex (FP64 or FP32) on the MDFT side and as
+// 2 np real channels [2 np][N] (FP32, x fastest) on the network side: channel
+// 2 ip + 0 holds the real part of projection ip, 2 ip + 1 the imaginary part.
 #include <cfloat>
 
 #include "mdft_internal.h"
@@ -61,7 +61,24 @@ __global__ __launch_bounds__(kThreads) void cold_start(const double *__restrict_
         xi[g] = (beta * vext[g] >= vmax) ? 0.0 : 1.0;
 }
 
+// out = sqrt((1 - lam) xa^2 + lam xb^2): the density mixture (1 - lam) rho_a + lam rho_b in xi
+__global__ __launch_bounds__(kThreads) void density_mix(size_t total, double lam, const double *__restrict__ xa,
+                                                       const double *__restrict__ xb, double *__restrict__ out) {
+    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads) {
+        const double a = xa[g], b = xb[g];
+        out[g] = sqrt((1.0 - lam) * a * a + lam * b * b);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_density_mix(mdft_ctx *c, size_t total, double lam, const double *xa, const double *xb,
+                              double *out) {
+    size_t nb = (total + kThreads - 1) / kThreads;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(density_mix, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, total, lam, xa, xb, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale) {
     const size_t nb = (c->N + kTile - 1) / kTile;

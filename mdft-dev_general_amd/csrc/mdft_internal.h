@@ -117,6 +117,8 @@ hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *vext);
 hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale);
 hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in);
 hipError_t launch_cold_start(mdft_ctx *c, double *xi);
+hipError_t launch_density_mix(mdft_ctx *c, size_t total, double lam, const double *xa, const double *xb,
+                              double *out);
 int bf16_tables(mdft_ctx *c, const std::vector<double> &a2p, const std::vector<double> &p2a,
                 const std::vector<double> &fm);
 hipError_t launch_angl2proj_bf16(mdft_ctx *c, const double *xi, float2 *proj);

@@ -5,14 +5,16 @@ The reference has no learned component; the minimiser it drives
 src/module_density.f90:267-284.  Here a periodic 3D convolutional network
 (hand-written HIP forward/backward/Adam kernels, FP32) maps the angular
 projections of a feature field of the external potential to the angular
-projections of the converged OZ field gamma = c (*) drho, and the minimiser
-starts from
+projections of the converged density, rho/rho0 - 1, and the minimiser starts
+from xi = sqrt(max(rho_pred/rho0, 0)) -- or from the best of a few density
+mixtures (1 - lam) rho_cold + lam rho_pred, the cold start included, judged by
+F.  The minimisation afterwards is the plain one: the network changes the
+number of iterations, not the answer, and the cold start stays the fallback.
 
-    xi = exp((-beta vext + gamma_pred(r, Omega)) / 2),
-
-the Euler-Lagrange form of the minimum.  The minimisation afterwards is the
-plain one: the network changes the number of iterations, not the answer, and
-the cold start stays the fallback (warm_start(..., fallback=...)).
+(A prediction of the OZ field gamma with the Euler-Lagrange start
+xi = exp((-beta vext + gamma)/2) is also available, kind MDFT_WS_GAMMA; its
+errors are exponentiated at high-density sites, which makes it the weaker
+choice for a start.)
 
 Training data are converged solutions of the plain method on small boxes with
 the grid spacing of the target workload (the network is translation invariant
@@ -43,8 +45,10 @@ DEFAULT_WEIGHTS = os.path.join(DATA, "warmstart_solute20.npz")
 WSNET_EXPORTS = (
     "mdft_wsnet_create", "mdft_wsnet_destroy", "mdft_wsnet_last_error", "mdft_wsnet_num_params",
     "mdft_wsnet_get_params", "mdft_wsnet_set_params", "mdft_wsnet_get_grads", "mdft_wsnet_forward",
-    "mdft_wsnet_loss_grad", "mdft_wsnet_adam", "mdft_ws_features", "mdft_ws_gamma", "mdft_ws_guess",
+    "mdft_wsnet_loss_grad", "mdft_wsnet_adam", "mdft_ws_features", "mdft_ws_gamma", "mdft_ws_density",
+    "mdft_ws_guess", "mdft_ws_mix",
 )
+WS_GAMMA, WS_DENSITY = 0, 1
 
 _bound = False
 
@@ -69,7 +73,9 @@ def _lib():
         lib.mdft_wsnet_adam.argtypes = [vp, C.c_float]
         lib.mdft_ws_features.argtypes = [vp, vp]
         lib.mdft_ws_gamma.argtypes = [vp, vp, vp, vp]
-        lib.mdft_ws_guess.argtypes = [vp, vp, vp]
+        lib.mdft_ws_density.argtypes = [vp, vp, vp]
+        lib.mdft_ws_guess.argtypes = [vp, vp, C.c_int, vp]
+        lib.mdft_ws_mix.argtypes = [vp, C.c_double, vp, vp, vp]
         for n in WSNET_EXPORTS:
             if n not in ("mdft_wsnet_destroy", "mdft_wsnet_last_error", "mdft_wsnet_num_params"):
                 getattr(lib, n).restype = C.c_int
@@ -190,8 +196,15 @@ def gamma(eng: Engine, xi: DeviceBuffer, want_field: bool = False):
     return (tgt, g) if want_field else tgt
 
 
-def guess(eng: Engine, pred: DeviceBuffer | None, xi: DeviceBuffer) -> None:
-    eng._check(_lib().mdft_ws_guess(eng._h, _ptr(pred), xi.ptr))
+def density(eng: Engine, xi: DeviceBuffer) -> DeviceBuffer:
+    """Projections [2 np][N] (FP32) of rho/rho0 - 1 for xi."""
+    tgt = DeviceBuffer(eng, 4 * 2 * eng.np_ * eng.N)
+    eng._check(_lib().mdft_ws_density(eng._h, xi.ptr, tgt.ptr))
+    return tgt
+
+
+def guess(eng: Engine, pred: DeviceBuffer | None, xi: DeviceBuffer, kind: int = WS_DENSITY) -> None:
+    eng._check(_lib().mdft_ws_guess(eng._h, _ptr(pred), int(kind), xi.ptr))
 
 
 def cold_start(eng: Engine, xi: DeviceBuffer) -> None:
@@ -199,27 +212,49 @@ def cold_start(eng: Engine, xi: DeviceBuffer) -> None:
     guess(eng, None, xi)
 
 
-def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: DeviceBuffer | None = None) -> dict:
-    """xi <- the network's guess for the engine's vext.  With a scratch field
-    `fallback`, F is evaluated at the guess and at the cold start and the lower
-    one is kept: the cold start stays the fallback, so a poor prediction costs
-    one evaluation and never a worse start."""
+def mix(eng: Engine, lam: float, xa: DeviceBuffer, xb: DeviceBuffer, out: DeviceBuffer) -> None:
+    eng._check(_lib().mdft_ws_mix(eng._h, float(lam), xa.ptr, xb.ptr, out.ptr))
+
+
+def _F(eng, xi):
+    return float(np.sum(eng.energy_and_gradient(xi, None)[0]))
+
+
+def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: DeviceBuffer | None = None,
+               kind: int = WS_DENSITY, lams=(0.5,)) -> dict:
+    """xi <- the network's starting density for the engine's vext.  With a
+    scratch field `fallback`, F is evaluated at the prediction (lam = 1), at
+    the cold start (lam = 0) and at the density mixtures `lams`, and the lowest
+    is kept: a poor prediction costs len(lams) + 2 F evaluations and never a
+    worse start than the cold one."""
+    assert net.channels[-1] == 2 * eng.np_ and net.channels[0] == 2 * eng.np_
     f = features(eng)
     pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
-    assert net.channels[-1] == 2 * eng.np_ and net.channels[0] == 2 * eng.np_
     net.forward(eng.n, f, pred)
-    guess(eng, pred, xi)
     f.free()
-    pred.free()
-    info = {"start": "learned"}
+    guess(eng, pred, xi, kind)
+    info = {"start": "learned", "kind": "density" if kind == WS_DENSITY else "gamma"}
     if fallback is not None:
-        fw = float(np.sum(eng.energy_and_gradient(xi, None)[0]))
+        trials = {1.0: _F(eng, xi)}
         cold_start(eng, fallback)
-        fc = float(np.sum(eng.energy_and_gradient(fallback, None)[0]))
-        info.update(F_learned=fw, F_cold=fc)
-        if not fw <= fc:
+        trials[0.0] = _F(eng, fallback)
+        for lam in lams:
+            if kind != WS_DENSITY:
+                break
+            guess(eng, pred, xi, kind)
+            mix(eng, lam, fallback, xi, xi)
+            trials[float(lam)] = _F(eng, xi)
+        best = min(trials, key=lambda k: trials[k] if np.isfinite(trials[k]) else np.inf)
+        info.update(F={str(k): v for k, v in trials.items()}, lam=best, evaluations=len(trials))
+        if best == 0.0:
             eng._check(eng.lib.mdft_vec_lincomb(eng._h, eng.N * eng.no, 1.0, fallback.ptr, 0.0, None, xi.ptr))
             info["start"] = "cold (fallback)"
+        else:
+            guess(eng, pred, xi, kind)
+            if best != 1.0:
+                mix(eng, best, fallback, xi, xi)
+                info["start"] = f"mixture lam={best}"
+    pred.free()
     return info
 
 
@@ -227,7 +262,7 @@ def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4,
                       maxeval: int = 600, log=None):
     """Solve `nsolutes` seeded solutes of workload `w` (grid, box, mmax, c(k))
     with the plain method from the reference cold start; returns the engine and
-    per-solute (features, targets, evaluations, F) with device-resident fields."""
+    per-solute (features, density targets, evaluations, F) with device-resident fields."""
     from . import workloads
     from .minimize import minimize
     eng = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, precision)
@@ -242,7 +277,7 @@ def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4,
         feat = features(eng)
         cold_start(eng, xi)
         res = minimize(eng, xi, g=g, m=5, gtol=gtol, maxeval=maxeval, fnoise=1e-6)
-        tgt = gamma(eng, xi)
+        tgt = density(eng, xi)
         out.append((feat, tgt, res.evaluations, res.F))
         if log:
             log(f"  solute {i}: {len(sol.q)} atoms, F={res.F:.3f}, evaluations={res.evaluations}, "
