@@ -65,13 +65,16 @@ int mdft_ws_features(struct mdft_ctx *ctx, float *d_feat);
  * may be NULL) receives the projections of gamma. */
 int mdft_ws_gamma(struct mdft_ctx *ctx, const double *d_xi, double *d_gamma, float *d_target);
 
-/* d_target [2 np][N] (FP32): projections of rho/rho0 - 1 for the density xi (device). */
+/* d_target [2 np][N] (FP32): projections of rho/rho0 - c for the density xi
+ * (device), c the reference cold start (0 where beta vext >= -ln eps - 0.05,
+ * else 1): the part of the density the cold start misses. */
 int mdft_ws_density(struct mdft_ctx *ctx, const double *d_xi, float *d_target);
 
 enum { MDFT_WS_GAMMA = 0, MDFT_WS_DENSITY = 1 };
 
 /* Starting density from predicted projections d_pred [2 np][N]:
- * kind MDFT_WS_DENSITY: d_pred are projections of rho/rho0 - 1, d_xi = sqrt(max(rho/rho0, 0));
+ * kind MDFT_WS_DENSITY: d_pred are projections of rho/rho0 - c (see mdft_ws_density),
+ *   d_xi = 0 where c = 0, else sqrt(max(1 + residual, 0));
  * kind MDFT_WS_GAMMA: d_pred are projections of gamma, d_xi = exp((-beta vext + gamma) / 2),
  * exponent clipped to [-200, 7].
  * With d_pred NULL: the reference's cold start (src/module_density.f90:267-284),

@@ -842,7 +842,7 @@ int mdft_ws_features(mdft_ctx *c, float *d_feat) {
     if (!c || !d_feat) return MDFT_EINVAL;
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
     if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the warm start needs molecular symmetry order 2");
-    HIPCHK(c, launch_angl2proj_feat(c, c->d_vext));
+    HIPCHK(c, launch_angl2proj_feat(c, c->d_vext, 1));
     HIPCHK(c, launch_proj_to_channels(c, d_feat, 1.0f));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
@@ -869,8 +869,10 @@ int mdft_ws_density(mdft_ctx *c, const double *d_xi, float *d_target) {
     int rc = ready(c);
     if (rc) return rc;
     if (!d_xi || !d_target) return fail(c, MDFT_EINVAL, "null argument");
-    if ((rc = run_excess_pipeline(c, d_xi, 1, false))) return rc;  // projections of rho0 (xi^2 - 1)
-    HIPCHK(c, launch_proj_to_channels(c, d_target, (float)(1.0 / c->rho0)));
+    if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the warm start needs molecular symmetry order 2");
+    if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
+    HIPCHK(c, launch_angl2proj_feat(c, d_xi, 2));  // projections of rho/rho0 - c
+    HIPCHK(c, launch_proj_to_channels(c, d_target, 1.0f));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }

@@ -135,7 +135,8 @@ struct FinalConsts {
 // mode 1: df += excess gradient (excess_energy plug-in)
 // mode 2: df = gamma(r, Omega) = -(prefactor / kT) proj2angl(proj), the OZ field in units of ln(rho/rho0)
 // mode 3: proj holds gamma's projections, df = exp((-beta vext + proj2angl(proj)) / 2) (xi not read)
-// mode 4: proj holds the projections of rho/rho0 - 1, df = sqrt(max(1 + proj2angl(proj), 0)) (xi not read)
+// mode 4: proj holds the projections of rho/rho0 - c, c the reference cold start (0 where
+// beta vext >= -ln(DBL_EPSILON) - 0.05, else 1): df = 0 where c = 0, else sqrt(max(1 + proj2angl(proj), 0))
 template <int PT>
 __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict__ proj, const double *__restrict__ xi,
                                                      const double *__restrict__ vext, double *__restrict__ df,
@@ -230,8 +231,9 @@ __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict_
             if (df) df[g] = -(k.prefactor / kT) * v;
         } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
             df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
-        } else {  // mode 4: proj holds the projections of rho/rho0 - 1; df = sqrt(max(rho/rho0, 0))
-            df[g] = sqrt(fmax(1.0 + v, 0.0));
+        } else {  // mode 4: proj holds the projections of rho/rho0 - c (c: the cold start);
+                  // df = sqrt(max(c + v, 0)), 0 where the cold start excludes the solvent
+            df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, 0.0));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -306,10 +308,13 @@ struct Dims {
 };
 
 // feat = 0: o = rho0 (xi^2 - 1) (the excess term); feat = 1: o = exp(-max(beta xi, -8) / 2) - 1
-// with xi the external potential (features of the learned warm start)
+// with xi the external potential (features of the learned warm start); feat = 2:
+// o = xi^2 - c with c the reference cold start from aux = vext (0 where
+// beta vext >= -ln(DBL_EPSILON) - 0.05, else 1): the warm start's density residual
 template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
-                                                       size_t N, AngTables t, double rho0, int feat, double beta) {
+                                                       size_t N, AngTables t, double rho0, int feat, double beta,
+                                                       const double *__restrict__ aux) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -328,7 +333,9 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     const double *src = xi + r0 * NO;
     for (int i = tid; i < npts * NO; i += kThreads) {
         const double x = src[i];
-        o[i] = feat ? (T)(exp(-0.5 * fmax(beta * x, -8.0)) - 1.0) : (T)(rho0 * (x * x - 1.0));
+        if (feat == 0) o[i] = (T)(rho0 * (x * x - 1.0));
+        else if (feat == 1) o[i] = (T)(exp(-0.5 * fmax(beta * x, -8.0)) - 1.0);
+        else o[i] = (T)(x * x - ((beta * aux[r0 * NO + i] >= -log(DBL_EPSILON) - 0.05) ? 0.0 : 1.0));
     }
     __syncthreads();
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
@@ -480,8 +487,9 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
             if (df) df[g] = -(k.prefactor / kT) * v;
         } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
             df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
-        } else {  // mode 4: proj holds the projections of rho/rho0 - 1; df = sqrt(max(rho/rho0, 0))
-            df[g] = sqrt(fmax(1.0 + v, 0.0));
+        } else {  // mode 4: proj holds the projections of rho/rho0 - c (c: the cold start);
+                  // df = sqrt(max(c + v, 0)), 0 where the cold start excludes the solvent
+            df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, 0.0));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -510,12 +518,12 @@ hipError_t ensure_partials(mdft_ctx *c, size_t nblocks) {
 }  // namespace
 
 template <int PT, typename T>
-bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int feat = 0) {
+bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int feat = 0, const double *aux = nullptr) {
     const size_t nb = (c->N + PT - 1) / PT;
     const double beta = 1.0 / c->d.kT;
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0, feat, beta); return true;
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0, feat, beta, aux); return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;
@@ -628,11 +636,12 @@ hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, d
     return hipGetLastError();
 }
 
-// angl2proj of the warm-start feature field of vext (mrso = 2 kernels), into the context's projection buffer
-hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *vext) {
+// angl2proj of a warm-start field into the context's projection buffer (mrso = 2 kernels):
+// feat 1: the feature field of src = vext; feat 2: the density residual of src = xi against the cold start
+hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat) {
     if (c->d.mrso != 2) return hipErrorNotSupported;
     bool ok = (c->d.precision == MDFT_PREC_FP64)
-                  ? launch_a2p_t<kPTt, double>(c, vext, static_cast<double2 *>(c->d_proj), 1)
-                  : launch_a2p_t<kPTt, float>(c, vext, static_cast<float2 *>(c->d_proj), 1);
+                  ? launch_a2p_t<kPTt, double>(c, src, static_cast<double2 *>(c->d_proj), feat, c->d_vext)
+                  : launch_a2p_t<kPTt, float>(c, src, static_cast<float2 *>(c->d_proj), feat, c->d_vext);
     return ok ? hipGetLastError() : hipErrorNotSupported;
 }

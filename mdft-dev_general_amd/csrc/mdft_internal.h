@@ -113,7 +113,7 @@ hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
 hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
-hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *vext);
+hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat);
 hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale);
 hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in);
 hipError_t launch_cold_start(mdft_ctx *c, double *xi);

@@ -5,10 +5,11 @@ The reference has no learned component; the minimiser it drives
 src/module_density.f90:267-284.  Here a periodic 3D convolutional network
 (hand-written HIP forward/backward/Adam kernels, FP32) maps the angular
 projections of a feature field of the external potential to the angular
-projections of the converged density, rho/rho0 - 1, and the minimiser starts
-from xi = sqrt(max(rho_pred/rho0, 0)) -- or from the best of a few density
-mixtures (1 - lam) rho_cold + lam rho_pred, the cold start included, judged by
-F.  The minimisation afterwards is the plain one: the network changes the
+projections of the part of the converged density the reference's cold start
+misses, rho/rho0 - c (c = 0 where the cold start excludes the solvent, 1
+elsewhere), and the minimiser starts from xi = sqrt(max(c + residual, 0))
+(0 where c = 0) -- or from the best of a few density mixtures
+(1 - lam) rho_cold + lam rho_pred, the cold start included, judged by F.  The minimisation afterwards is the plain one: the network changes the
 number of iterations, not the answer, and the cold start stays the fallback.
 
 (A prediction of the OZ field gamma with the Euler-Lagrange start
@@ -197,7 +198,7 @@ def gamma(eng: Engine, xi: DeviceBuffer, want_field: bool = False):
 
 
 def density(eng: Engine, xi: DeviceBuffer) -> DeviceBuffer:
-    """Projections [2 np][N] (FP32) of rho/rho0 - 1 for xi."""
+    """Projections [2 np][N] (FP32) of rho/rho0 - c for xi (c: the cold start)."""
     tgt = DeviceBuffer(eng, 4 * 2 * eng.np_ * eng.N)
     eng._check(_lib().mdft_ws_density(eng._h, xi.ptr, tgt.ptr))
     return tgt

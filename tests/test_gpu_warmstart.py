@@ -139,25 +139,28 @@ def test_gamma_field_features_and_guess(oracle_mod):
     err = np.abs(bx.download() - np.exp(0.5 * lg)).max() / np.exp(3.5)
     say(f"  gamma guess: max err / e^3.5 {err:.2e}")
     assert err < 1e-6
-    # density target: projections of rho/rho0 - 1 = the stage-1 projections / rho0
+    # density residual target: projections of xi^2 - c, c the cold start (0 where vext is excluding)
+    vext[::11] = 700.0
+    eng.set_vext(vext)
+    c = workloads.guess_xi(vext, kT)
     bx.upload(xi)
     dt = ws.density(eng, bx).download(np.float32).reshape(2 * npj, N)
-    P1 = o.excess_stage(xi, 1) / o.rho0
+    P1 = np.stack([o.angl2proj(r) for r in (xi * xi - c).reshape(N, no)])
     err = max(np.abs(dt[0::2] - P1.real.T).max(), np.abs(dt[1::2] - P1.imag.T).max()) / np.abs(P1).max()
-    say(f"  density projections: max rel err {err:.2e}")
+    say(f"  density residual projections: max rel err {err:.2e}")
     assert err < 1e-6
-    # density guess: xi = sqrt(max(1 + proj2angl(pred), 0))
+    # density guess: xi = 0 where c = 0, else sqrt(max(1 + proj2angl(pred), 0))
     pred2 = (rng.normal(0, 0.5, (2 * npj, N))).astype(np.float32)
     bp.upload(pred2)
     ws.guess(eng, bp, bx, ws.WS_DENSITY)
     pc = pred2[0::2].T.astype(np.float64) + 1j * pred2[1::2].T.astype(np.float64)
-    rp = 1.0 + np.concatenate([o.proj2angl(pc[i]) for i in range(N)])
-    assert np.abs(bx.download() - np.sqrt(np.maximum(rp, 0.0))).max() < 1e-9
+    rp = np.where(c > 0, np.maximum(1.0 + np.concatenate([o.proj2angl(pc[i]) for i in range(N)]), 0.0), 0.0)
+    assert np.abs(bx.download() - np.sqrt(rp)).max() < 1e-9
     # density mixture
     by = eng.alloc_field()
     by.upload(xi)
     ws.mix(eng, 0.3, by, bx, by)
-    assert np.allclose(by.download(), np.sqrt(0.7 * xi ** 2 + 0.3 * np.maximum(rp, 0.0)), rtol=1e-12, atol=1e-12)
+    assert np.allclose(by.download(), np.sqrt(0.7 * xi ** 2 + 0.3 * rp), rtol=1e-12, atol=1e-12)
     # device cold start = the host restatement of the reference's guess
     vext[::7] = 700.0
     eng.set_vext(vext)
