@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="solute20_128")
     ap.add_argument("--gtol", type=float, default=1e-6)
+    ap.add_argument("--memory", type=int, default=5, help="L-BFGS memory m (both paths)")
     ap.add_argument("--maxeval", type=int, default=1500)
     ap.add_argument("--mode", choices=["converge", "eval", "copy"], default="converge",
                     help="copy: a plain device copy of 1 GiB (z = x through mdft_vec_lincomb), the "
@@ -204,7 +205,7 @@ def main():
         eng.synchronize()
         t0 = time.perf_counter()
         wsm.cold_start(eng, xi)
-        res = minimize(eng, xi, g=df, m=5, gtol=args.gtol, maxeval=args.maxeval, fnoise=1e-7)
+        res = minimize(eng, xi, g=df, m=args.memory, gtol=args.gtol, maxeval=args.maxeval, fnoise=1e-7)
         eng.synchronize()
         t_conv = time.perf_counter() - t0
         dg_ref = res.F
@@ -212,7 +213,7 @@ def main():
         eng.synchronize()
         t0 = time.perf_counter()
         wsm.cold_start(eng, xi)
-        res_p = minimize(eng, xi, g=df, m=5, gtol=0.0, maxeval=args.maxeval, fnoise=1e-7,
+        res_p = minimize(eng, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-7,
                          ftarget=dg_ref, fband=band)
         eng.synchronize()
         t_plain = time.perf_counter() - t0
@@ -233,7 +234,8 @@ def main():
                 info.update(wsm.warm_start(e32, net, xi, fallback=scratch))
             else:
                 wsm.cold_start(e32, xi)
-            r = minimize(e32, xi, g=df, m=5, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref, fband=band)
+            r = minimize(e32, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref,
+                         fband=band)
             c, _ = eng.energy_and_gradient(xi, None)
             eng.synchronize()
             return time.perf_counter() - t0, r, float(np.sum(c))
@@ -262,7 +264,7 @@ def main():
             "plain_evaluations": res_p.evaluations, "plain_status": res_p.status,
             "fp32_cold_evaluations": res_c32.evaluations,
             "fast_evaluations": res_f.evaluations, "fast_iterations": res_f.iterations,
-            "fast_status": res_f.status, "warm_start": info,
+            "fast_status": res_f.status, "warm_start": info, "lbfgs_memory": args.memory,
         })
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = cpu_baseline(w, res_p.evaluations)
