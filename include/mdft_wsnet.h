@@ -11,7 +11,7 @@
  *   z_l = conv3d(a_{l-1}; W_l, kernel k_l in {1, 3}, dilation d_l) + b_l
  *         (periodic box; a 3x3x3 layer has <= 32 channels in and out, a
  *         pointwise layer <= 256),
- *   a_l = relu(z_l) for l < L, output a_L = z_L.
+ *   a_l = leaky_relu(z_l) = (z_l > 0 ? z_l : 0.1 z_l) for l < L, output a_L = z_L.
  * Parameters: for each layer W_l[co][ci][k][k][k] then b_l[co], one FP32 vector.
  * Loss: sum(mask (a_L - target)^2) / (sum(mask) C_L), mask per grid point.
  * All field pointers are device pointers; the caller makes them ready before

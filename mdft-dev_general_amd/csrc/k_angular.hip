@@ -542,7 +542,8 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
     }
 }
 
-constexpr int kPTt = 4;  // points per workgroup of the specialised kernels
+constexpr int kPTt = 4;   // points per workgroup of the specialised kernels, FP64
+constexpr int kPT32 = 8;  // the same for FP32 (half the LDS per point; amortises the table staging)
 
 hipError_t launch_angl2proj_f64(mdft_ctx *c, const double *xi, double2 *proj) {
     if (c->d.mrso == 2 && launch_a2p_t<kPTt, double>(c, xi, proj)) return hipGetLastError();
@@ -623,15 +624,15 @@ FinalConsts final_consts(mdft_ctx *c) {
 }
 
 hipError_t launch_angl2proj_f32(mdft_ctx *c, const double *xi, float2 *proj) {
-    if (c->d.mrso != 2 || !launch_a2p_t<kPTt, float>(c, xi, proj)) return hipErrorNotSupported;
+    if (c->d.mrso != 2 || !launch_a2p_t<kPT32, float>(c, xi, proj)) return hipErrorNotSupported;
     return hipGetLastError();
 }
 
 hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, double *df, int mode, int *nblocks_out) {
-    const size_t nbt = (c->N + kPTt - 1) / kPTt;
+    const size_t nbt = (c->N + kPT32 - 1) / kPT32;
     hipError_t e = ensure_partials(c, nbt);
     if (e != hipSuccess) return e;
-    if (c->d.mrso != 2 || !launch_final_t<kPTt, float>(c, proj, xi, df, mode, final_consts(c))) return hipErrorNotSupported;
+    if (c->d.mrso != 2 || !launch_final_t<kPT32, float>(c, proj, xi, df, mode, final_consts(c))) return hipErrorNotSupported;
     *nblocks_out = (int)nbt;
     return hipGetLastError();
 }
@@ -642,6 +643,6 @@ hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat) {
     if (c->d.mrso != 2) return hipErrorNotSupported;
     bool ok = (c->d.precision == MDFT_PREC_FP64)
                   ? launch_a2p_t<kPTt, double>(c, src, static_cast<double2 *>(c->d_proj), feat, c->d_vext)
-                  : launch_a2p_t<kPTt, float>(c, src, static_cast<float2 *>(c->d_proj), feat, c->d_vext);
+                  : launch_a2p_t<kPT32, float>(c, src, static_cast<float2 *>(c->d_proj), feat, c->d_vext);
     return ok ? hipGetLastError() : hipErrorNotSupported;
 }

@@ -1,6 +1,7 @@
 // k_net.hip -- the warm-start network: periodic dilated 3D convolutions
 // (3x3x3, up to 32 channels) and pointwise (1x1x1, up to 256 channels) layers
-// with ReLU, masked mean-squared loss, backward passes and Adam, all FP32.
+// with leaky ReLU (slope 0.1), masked mean-squared loss, backward passes and
+// Adam, all FP32.
 //
 // The reference has no learned component.  The network maps per-point
 // features of the external potential to the low angular projections of the
@@ -32,6 +33,7 @@ constexpr int kMaxC3 = 32;       // channels of a 3x3x3 layer
 constexpr int kMaxC1 = 256;      // channels of a pointwise layer
 constexpr int kWChunk = 1024;    // points per workgroup in the weight gradient
 constexpr int kWSub = 128;       // points staged in LDS at a time
+constexpr float kLeak = 0.1f;    // slope of the leaky ReLU for negative inputs
 
 struct Geo {
     int nx, ny, nz;
@@ -97,16 +99,16 @@ __global__ __launch_bounds__(kThreads) void conv_fwd(const float *__restrict__ i
     for (int k = 0; k < KG; k++) {
         if (k < kg) {
             float v = acc[k] + b[co0 + k];
-            if (relu) v = fmaxf(v, 0.0f);
+            if (relu && v < 0.0f) v *= kLeak;
             out[(size_t)(co0 + k) * N + p] = v;
         }
     }
 }
 
-// g *= (a > 0): gradient through the ReLU of a layer whose output is a
+// gradient through the leaky ReLU of a layer whose output is a: g *= (a > 0 ? 1 : kLeak)
 __global__ __launch_bounds__(kThreads) void relu_bwd(float *__restrict__ gr, const float *__restrict__ a, size_t n) {
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
-        if (!(a[i] > 0.0f)) gr[i] = 0.0f;
+        if (!(a[i] > 0.0f)) gr[i] *= kLeak;
 }
 
 // din[ci][p] = sum_co sum_t W[co][ci][t] dz[co][p - (t-1) d]

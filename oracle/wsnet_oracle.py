@@ -6,7 +6,7 @@ HIP kernels implement (mdft-dev_general_amd/csrc/k_net.hip):
 
     a_0 = x                                   [C0][nz][ny][nx]
     z_l = conv3d(a_{l-1}; W_l, dilation d_l) + b_l   (k x k x k taps, k in {1, 3}, periodic)
-    a_l = relu(z_l)   for l < L,   a_L = z_L  (linear output)
+    a_l = leaky_relu(z_l) = max(z_l, 0.1 z_l)   for l < L,   a_L = z_L  (linear output)
     loss = sum(mask * (a_L - t)^2) / (sum(mask) C_L)
 
 Weights W_l[co][ci][tz][ty][tx], tap offsets (t - (k-1)/2) * d_l along each
@@ -16,6 +16,8 @@ against torch's conv3d with circular padding and autograd on the CPU.
 from __future__ import annotations
 
 import numpy as np
+
+LEAK = 0.1  # slope of the leaky ReLU for negative inputs
 
 
 def _taps(k):
@@ -66,7 +68,7 @@ class Net:
         acts = [np.asarray(x, dtype=np.float64)]
         for li, (w, b, d) in enumerate(zip(self.W, self.b, self.d)):
             z = conv3d(acts[-1], w, b, d)
-            acts.append(np.maximum(z, 0.0) if li < len(self.W) - 1 else z)
+            acts.append(np.where(z > 0.0, z, LEAK * z) if li < len(self.W) - 1 else z)
         return acts
 
     def loss_and_grads(self, x, t, mask=None):
@@ -80,7 +82,7 @@ class Net:
         gW, gb = [None] * len(self.W), [None] * len(self.W)
         for li in range(len(self.W) - 1, -1, -1):
             if li < len(self.W) - 1:
-                g = g * (acts[li + 1] > 0.0)
+                g = g * np.where(acts[li + 1] > 0.0, 1.0, LEAK)
             gW[li], gb[li] = conv3d_bwd_weight(g, acts[li], self.d[li], self.W[li].shape[2])
             g = conv3d_bwd_data(g, self.W[li], self.d[li])
         return loss, gW, gb, g

@@ -225,7 +225,7 @@ def test_trained_network_reduces_iterations():
         minimize(eng, xi, g=gb, m=5, gtol=1e-5, maxeval=800)
         data.append((f, ws.density(eng, xi)))
     net = ws.WarmStartNet.for_engine(eng, seed=1)
-    hist = ws.train(net, w.n, data, epochs=30, lr=2e-3, seed=0)
+    hist = ws.train(net, w.n, data, epochs=60, lr=1e-3, seed=0)
     say(f"  loss: first epoch {hist[0]:.4f}, last {hist[-1]:.4f}")
     assert hist[-1] < 0.5 * hist[0]
     cold, warm = [], []

@@ -17,7 +17,7 @@ def _torch_net(weights, biases, dils, x):
         pad = d * (w.shape[2] // 2)
         a = F.conv3d(F.pad(a, (pad,) * 6, mode="circular") if pad else a, w, b, dilation=d)
         if li < len(Ws) - 1:
-            a = torch.relu(a)
+            a = F.leaky_relu(a, 0.1)
     return a[0], Ws, bs
 
 
