@@ -777,22 +777,26 @@ int mdft_vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *
 
 int mdft_vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z) {
     if (!c || !x || !z) return MDFT_EINVAL;
+    if (!aligned16(x) || !aligned16(z) || (y && !aligned16(y)))
+        return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
     HIPCHK(c, vec_lincomb(c, n, a, x, b, y, z));
     return MDFT_OK;
 }
 
 int mdft_vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out) {
     if (!c || !vecs || !x || !out || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    if (!aligned16(x)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
     for (int j = 0; j < k; j++)
-        if (!vecs[j]) return fail(c, MDFT_EINVAL, "null vector");
+        if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "null or unaligned vector");
     HIPCHK(c, vec_multi_dot(c, n, k, vecs, x, out));
     return MDFT_OK;
 }
 
 int mdft_vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z) {
     if (!c || !coef || !vecs || !z || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    if (!aligned16(z)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
     for (int j = 0; j < k; j++)
-        if (!vecs[j]) return fail(c, MDFT_EINVAL, "null vector");
+        if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "null or unaligned vector");
     HIPCHK(c, vec_combine(c, n, k, coef, vecs, z));
     return MDFT_OK;
 }

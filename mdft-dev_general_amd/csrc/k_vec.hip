@@ -85,11 +85,37 @@ __global__ __launch_bounds__(kThreads) void sum_partials(const double *__restric
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// z = a x + b y   (z may alias x or y)
+// z = a x + b y   (z may alias x or y); 16-byte accesses, two independent per trip
 __global__ __launch_bounds__(kThreads) void lincomb(size_t n, double a, const double *x, double b, const double *y,
                                                    double *z) {
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
-        z[i] = a * x[i] + (y ? b * y[i] : 0.0);
+    const size_t n2 = n / 2, stride = (size_t)gridDim.x * kThreads;
+    const double2 *x2 = reinterpret_cast<const double2 *>(x);
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    double2 *z2 = reinterpret_cast<double2 *>(z);
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += 2 * stride) {
+        const size_t j = i + stride;
+        const double2 u = x2[i];
+        double2 v = make_double2(a * u.x, a * u.y);
+        double2 w = make_double2(0.0, 0.0);
+        const bool two = j < n2;
+        if (two) {
+            const double2 u2 = x2[j];
+            w = make_double2(a * u2.x, a * u2.y);
+        }
+        if (y) {
+            const double2 t = y2[i];
+            v.x += b * t.x;
+            v.y += b * t.y;
+            if (two) {
+                const double2 t2 = y2[j];
+                w.x += b * t2.x;
+                w.y += b * t2.y;
+            }
+        }
+        z2[i] = v;
+        if (two) z2[j] = w;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) z[n - 1] = a * x[n - 1] + (y ? b * y[n - 1] : 0.0);
 }
 
 __global__ __launch_bounds__(kThreads) void widen(size_t n, const float2 *__restrict__ in, double2 *__restrict__ out) {
@@ -106,17 +132,27 @@ struct VecSet {
     int k;
 };
 
-// part[blk][j] = sum_i x[i] v_j[i]
+// part[blk][j] = sum_i x[i] v_j[i]   (16-byte accesses)
 __global__ __launch_bounds__(kThreads) void multi_dot_partial(size_t n, const double *__restrict__ x, VecSet v,
                                                              double *__restrict__ part) {
     double acc[MDFT_VEC_MAX];
 #pragma unroll
     for (int j = 0; j < MDFT_VEC_MAX; j++) acc[j] = 0.0;
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
-        const double xv = x[i];
+    const size_t n2 = n / 2;
+    const double2 *x2 = reinterpret_cast<const double2 *>(x);
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += (size_t)gridDim.x * kThreads) {
+        const double2 xv = x2[i];
 #pragma unroll
         for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) acc[j] += xv * v.p[j][i];
+            if (j < v.k) {
+                const double2 w = reinterpret_cast<const double2 *>(v.p[j])[i];
+                acc[j] += xv.x * w.x + xv.y * w.y;
+            }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) acc[j] += x[n - 1] * v.p[j][n - 1];
     }
     __shared__ double red[MDFT_VEC_MAX][kThreads / 64];
 #pragma unroll
@@ -153,12 +189,24 @@ __global__ __launch_bounds__(kThreads) void multi_dot_final(const double *__rest
 
 // z = sum_j c_j v_j   (z may be one of the v_j: each element is read before it is written)
 __global__ __launch_bounds__(kThreads) void combine(size_t n, VecSet v, double *z) {
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+    const size_t n2 = n / 2;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += (size_t)gridDim.x * kThreads) {
+        double sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) {
+                const double2 w = reinterpret_cast<const double2 *>(v.p[j])[i];
+                sx += v.c[j] * w.x;
+                sy += v.c[j] * w.y;
+            }
+        reinterpret_cast<double2 *>(z)[i] = make_double2(sx, sy);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
         double s = 0.0;
 #pragma unroll
         for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) s += v.c[j] * v.p[j][i];
-        z[i] = s;
+            if (j < v.k) s += v.c[j] * v.p[j][n - 1];
+        z[n - 1] = s;
     }
 }
 
