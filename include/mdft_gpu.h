@@ -128,6 +128,20 @@ int mdft_compute_vext(mdft_ctx *ctx, int natom, const double *atoms, int nsite, 
 /* Copies the context's vext(no,nx,ny,nz) out (host or, with MDFT_DEVICE_PTRS, device). */
 int mdft_get_vext(mdft_ctx *ctx, double *out, int flags);
 
+/* Solvation free energy corrections of src/module_pressure_correction.f90:7-156
+ * (bridge-free branch) for the density xi (host, or device with MDFT_DEVICE_PTRS):
+ * out[0] solvent molecules in the box, sum_r sum_io w rho0 xi^2 dv
+ * out[1] the same without solute, n0 V;  out[2] dN = out[1] - out[0]
+ * out[3] partial molar volume dN / n0 (A^3);  out[4] cavity volume, points with n(r)/n0 < 1e-3 (A^3)
+ * out[5] the functional of the empty box (xi = 0, kJ/mol);  out[6] bulk pressure out[5] / V (kJ/mol/A^3)
+ * out[7] PMV correction -dN/n0 * P (kJ/mol);  out[8] ideal-gas correction dN kT (kJ/mol).
+ * The reference's estimate is F + out[7] + the LJ tail correction.  Needs c(k). */
+int mdft_pressure_correction(mdft_ctx *ctx, const double *xi, int flags, double *out);
+/* Long-range LJ correction of src/module_lennardjones.f90:145-149 (kJ/mol): atoms and
+ * sites as for mdft_compute_vext, rcut the LJ cutoff (A). */
+int mdft_lj_tail_correction(mdft_ctx *ctx, int natom, const double *atoms, int nsite, const double *sites,
+                            double rcut, double *out);
+
 /* FP64 vector algebra on device vectors of length n (16-byte aligned device
  * pointers), for a device-resident minimiser: the dot/axpy/copy/scal loops of
  * src/module_blas_etc.f90:8-184 over n = no*nx*ny*nz. */

@@ -801,6 +801,71 @@ int mdft_vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const dou
     return MDFT_OK;
 }
 
+// src/module_pressure_correction.f90:7-156 (the bridge-free branch)
+int mdft_pressure_correction(mdft_ctx *c, const double *xi, int flags, double *out) {
+    int rc = ready(c);
+    if (rc) return rc;
+    if (!xi || !out) return fail(c, MDFT_EINVAL, "null argument");
+    const size_t nbytes = sizeof(double) * c->N * c->no;
+    const double *xd = xi;
+    if (!(flags & MDFT_DEVICE_PTRS)) {
+        HIPCHK(c, ensure_stage(c, &c->d_xi_stage));
+        HIPCHK(c, hipMemcpyAsync(c->d_xi_stage, xi, nbytes, hipMemcpyHostToDevice, c->stream));
+        xd = c->d_xi_stage;
+    }
+    // molecules in the box: n(r) = sum_io w rho0 xi^2 integrated over the box (:46-47, :74);
+    // cavity: points with n(r)/n0 < 1e-3 (:50-62)
+    int nb = 0;
+    double s3[3];
+    HIPCHK(c, launch_density_integrals(c, xd, 1.0e-3, &nb));
+    if ((rc = read_sums(c, nb, s3))) return rc;
+    const double volume = c->d.lx * c->d.ly * c->d.lz;
+    const double n_with = s3[0] * c->dv, n_without = c->d.n0 * volume;
+    const double dN = n_without - n_with;
+    // bulk grand potential: the functional of the empty box, xi = 0 (:115-119)
+    double *zero = nullptr;
+    HIPCHK(c, hipMalloc(&zero, nbytes));
+    hipError_t e = hipMemsetAsync(zero, 0, nbytes, c->stream);
+    double sums[3] = {0.0, 0.0, 0.0};
+    if (e == hipSuccess) rc = evaluate(c, zero, nullptr, MDFT_DEVICE_PTRS, 0, sums);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(zero);
+    if (e != hipSuccess) return fail(c, MDFT_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    if (rc) return rc;
+    const double f_empty = sums[0] + sums[1] + sums[2];
+    const double pbulk = f_empty / volume;  // kJ/mol/A^3 (:120)
+    out[0] = n_with;
+    out[1] = n_without;
+    out[2] = dN;
+    out[3] = dN / c->d.n0;          // partial molar volume, A^3 (:85)
+    out[4] = s3[1] * c->dv;         // cavity volume, A^3
+    out[5] = f_empty;
+    out[6] = pbulk;
+    out[7] = -dN / c->d.n0 * pbulk;  // PMV correction (:128)
+    out[8] = dN * c->d.kT;           // ideal-gas ("Pid") correction (:129)
+    return MDFT_OK;
+}
+
+// src/module_lennardjones.f90:145-149: for each solute atom and each solvent site with eps > 0,
+// -(16 pi / 3) eps_uv sigma_uv^6 n0 / rcut^3 with Lorentz-Berthelot mixing; pi as the reference writes it
+int mdft_lj_tail_correction(mdft_ctx *c, int natom, const double *atoms, int nsite, const double *sites, double rcut,
+                            double *out) {
+    if (!c || !atoms || !sites || !out || natom < 0 || nsite < 0 || !(rcut > 0.0)) return MDFT_EINVAL;
+    const double pi_ref = 3.14159;
+    double e = 0.0;
+    for (int u = 0; u < natom; u++)
+        for (int ss = 0; ss < nsite; ss++) {
+            const double eps_s = sites[6 * ss + 5], sig_s = sites[6 * ss + 4];
+            if (eps_s <= DBL_EPSILON) continue;
+            const double epsuv = sqrt(atoms[6 * u + 5] * eps_s);
+            const double siguv = 0.5 * (atoms[6 * u + 4] + sig_s);
+            const double s6 = siguv * siguv * siguv * siguv * siguv * siguv;
+            e -= (16.0 * pi_ref / 3.0) * epsuv * s6 * c->d.n0 / (rcut * rcut * rcut);
+        }
+    *out = e;
+    return MDFT_OK;
+}
+
 int mdft_phase_times(const mdft_ctx *c, double *ms, int n) {
     if (!c || !ms) return MDFT_EINVAL;
     const int k = std::min(n, c->nphase);

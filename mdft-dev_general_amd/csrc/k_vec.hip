@@ -210,6 +210,42 @@ __global__ __launch_bounds__(kThreads) void combine(size_t n, VecSet v, double *
     }
 }
 
+// Per grid point (one wave each): n(r) = rho0 sum_io w_io xi^2; block partials of
+// sum_r n(r) and of the number of points with n(r) / n0 < cav (cavity points).
+__global__ __launch_bounds__(kThreads) void density_integrals(const double *__restrict__ xi,
+                                                             const double *__restrict__ w, size_t N, int no,
+                                                             double rho0, double n0, double cav,
+                                                             double *__restrict__ partials) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double sum = 0.0, ncav = 0.0;
+    for (size_t p = (size_t)blockIdx.x * (kThreads / 64) + wv; p < N; p += (size_t)gridDim.x * (kThreads / 64)) {
+        const double *row = xi + p * no;
+        double s = 0.0;
+        for (int io = lane; io < no; io += 64) s += w[io] * row[io] * row[io];
+        s = wave_sum(s) * rho0;
+        if (lane == 0) {
+            sum += s;
+            ncav += (s / n0 < cav) ? 1.0 : 0.0;
+        }
+    }
+    __shared__ double red[2][kThreads / 64];
+    if (lane == 0) {
+        red[0][wv] = sum;
+        red[1][wv] = ncav;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < kThreads / 64; k++) {
+            a += red[0][k];
+            b += red[1][k];
+        }
+        partials[3 * blockIdx.x + 0] = a;
+        partials[3 * blockIdx.x + 1] = b;
+        partials[3 * blockIdx.x + 2] = 0.0;
+    }
+}
+
 unsigned blocks_for(size_t n) {
     size_t b = (n / 2 + kThreads - 1) / kThreads;
     if (b < 1) b = 1;
@@ -297,4 +333,20 @@ hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const d
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
+}
+
+hipError_t launch_density_integrals(mdft_ctx *c, const double *xi, double cav, int *nblocks_out) {
+    const unsigned nb = 2048;
+    if (c->partials_cap < 3 * (size_t)nb) {
+        (void)hipFree(c->d_partials);
+        c->d_partials = nullptr;
+        c->partials_cap = 0;
+        hipError_t e = hipMalloc(&c->d_partials, sizeof(double) * 3 * nb);
+        if (e != hipSuccess) return e;
+        c->partials_cap = 3 * (size_t)nb;
+    }
+    hipLaunchKernelGGL(density_integrals, dim3(nb), dim3(kThreads), 0, c->stream, xi, c->d_w, c->N, c->no, c->rho0,
+                       c->d.n0, cav, c->d_partials);
+    *nblocks_out = (int)nb;
+    return hipGetLastError();
 }

@@ -127,6 +127,7 @@ int fft_make_plan(int n, int *radix);
 hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out);
 hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out);
 hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z);
+hipError_t launch_density_integrals(mdft_ctx *c, const double *xi, double cav, int *nblocks_out);
 hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out);
 hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z);
 hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,

@@ -28,7 +28,7 @@ MDFT_EXPORTS = (
     "mdft_ideal_external", "mdft_excess_stage", "mdft_phase_times", "mdft_device_alloc",
     "mdft_device_free", "mdft_memcpy_h2d", "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext",
     "mdft_get_vext", "mdft_vec_dot", "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot",
-    "mdft_vec_combine",
+    "mdft_vec_combine", "mdft_pressure_correction", "mdft_lj_tail_correction",
 )
 
 
@@ -85,12 +85,15 @@ def load_library(path: str = LIBPATH):
     lib.mdft_vec_axpy_dot.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp, _dp]
     lib.mdft_vec_lincomb.argtypes = [vp, C.c_size_t, C.c_double, vp, C.c_double, vp, vp]
     lib.mdft_vec_multi_dot.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(vp), vp, _dp]
+    lib.mdft_pressure_correction.argtypes = [vp, vp, C.c_int, _dp]
+    lib.mdft_lj_tail_correction.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double, _dp]
     lib.mdft_vec_combine.argtypes = [vp, C.c_size_t, C.c_int, _dp, C.POINTER(vp), vp]
     for name in ("mdft_get_info", "mdft_get_quadrature", "mdft_set_vext", "mdft_set_ck",
                  "mdft_energy_and_gradient", "mdft_excess_energy", "mdft_ideal_external", "mdft_excess_stage",
                  "mdft_phase_times", "mdft_device_alloc", "mdft_device_free", "mdft_memcpy_h2d",
                  "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext", "mdft_get_vext", "mdft_vec_dot",
-                 "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine"):
+                 "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine",
+                 "mdft_pressure_correction", "mdft_lj_tail_correction"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib
@@ -198,6 +201,27 @@ class Engine:
         sites = np.ascontiguousarray([[*s.r, s.q, s.sig, s.eps] for s in solvent.sites], dtype=np.float64)
         self._check(self.lib.mdft_compute_vext(self._h, len(atoms), atoms.ctypes.data_as(_dp), len(sites),
                                                sites.ctypes.data_as(_dp), float(rcut)))
+
+    def pressure_correction(self, xi) -> dict:
+        """src/module_pressure_correction.f90:7-156 for xi (host array or DeviceBuffer)."""
+        out = np.zeros(9)
+        if isinstance(xi, DeviceBuffer):
+            self._check(self.lib.mdft_pressure_correction(self._h, xi.ptr, DEVICE_PTRS, _p(out)))
+        else:
+            xi = np.ascontiguousarray(xi, dtype=np.float64)
+            self._check(self.lib.mdft_pressure_correction(self._h, xi.ctypes.data, 0, _p(out)))
+        keys = ("n_with_solute", "n_without_solute", "delta_n", "pmv_A3", "cavity_volume_A3", "F_empty_box",
+                "P_bulk_kJmol_A3", "pmv_correction", "ideal_gas_correction")
+        return dict(zip(keys, (float(v) for v in out)))
+
+    def lj_tail_correction(self, solute, solvent, rcut: float = 10.0) -> float:
+        """src/module_lennardjones.f90:145-149 (kJ/mol)."""
+        atoms = np.ascontiguousarray(np.column_stack([solute.xyz, solute.q, solute.sig, solute.eps]), dtype=np.float64)
+        sites = np.ascontiguousarray([[*s.r, s.q, s.sig, s.eps] for s in solvent.sites], dtype=np.float64)
+        out = np.zeros(1)
+        self._check(self.lib.mdft_lj_tail_correction(self._h, len(atoms), _p(atoms), len(sites), _p(sites),
+                                                     float(rcut), _p(out)))
+        return float(out[0])
 
     def get_vext(self):
         out = np.empty(self.N * self.no)

@@ -105,3 +105,48 @@ def test_lbfgs_reaches_the_cpu_minimum(oracle_mod):
     d = np.abs(xs) - np.abs(r.x)
     say(f"  |xi| difference: rms {np.sqrt(np.mean(d * d)):.2e} max {np.abs(d).max():.2e}")
     assert np.sqrt(np.mean(d * d)) < 1e-3 and np.abs(d).max() < 2e-2
+
+
+def test_pressure_and_lj_tail_corrections(oracle_mod):
+    """src/module_pressure_correction.f90 and src/module_lennardjones.f90:145-149
+    restated: molecule counts and cavity from the orientation integral of
+    rho0 xi^2, the empty-box functional from the oracle at xi = 0, the LJ tail
+    from its closed form."""
+    from mdft.gpu import Engine
+    from mdft.grid import angular_grid
+    n, L, mmax = (9, 10, 8), (9.0, 10.0, 8.0), 2
+    say("[dG] pressure correction and LJ tail")
+    t = ck.synthetic_ck(mmax, 256, 0.05, seed=9, n0=0.0333)
+    eng = Engine(n, L, mmax, 2, 2.4777, 0.0333)
+    eng.set_ck(t)
+    o = oracle_mod.Oracle(n, L, mmax, 2, 2.4777, 0.0333)
+    o.set_ck(t)
+    rng = np.random.default_rng(5)
+    N, no = eng.N, eng.no
+    xi = 1.0 + 0.3 * rng.uniform(-1, 1, N * no)
+    xi.reshape(N, no)[::7] *= 0.01          # some cavity points
+    eng.set_vext(rng.uniform(-1, 1, N * no))
+    out = eng.pressure_correction(xi)
+    g = angular_grid(mmax, 2)
+    dens = (xi.reshape(N, no) ** 2 * g["w"]).sum(axis=1) * o.rho0
+    dv = 9.0 * 10.0 * 8.0 / N
+    vol = 9.0 * 10.0 * 8.0
+    assert abs(out["n_with_solute"] - dens.sum() * dv) < 1e-10 * dens.sum() * dv
+    assert abs(out["n_without_solute"] - 0.0333 * vol) < 1e-12
+    assert abs(out["cavity_volume_A3"] - (dens / 0.0333 < 1e-3).sum() * dv) < 1e-12
+    f0 = o.energy_and_gradient(np.zeros(N * no), np.zeros(N * no))[0].sum()
+    assert abs(out["F_empty_box"] - f0) < 1e-10 * abs(f0)
+    dN = 0.0333 * vol - dens.sum() * dv
+    assert abs(out["pmv_correction"] - (-dN / 0.0333 * f0 / vol)) < 1e-9 * abs(f0)
+    assert abs(out["ideal_gas_correction"] - dN * 2.4777) < 1e-9
+    sol = workloads.Solute(np.array([[1.0, 2.0, 3.0], [2.0, 2.0, 2.0]]), np.array([0.3, -0.3]),
+                           np.array([3.0, 3.5]), np.array([0.5, 0.8]))
+    solv = workloads.spce(0.0333)
+    ref = 0.0
+    for a in range(2):
+        for site in solv.sites:
+            if site.eps > 1e-300:
+                e = np.sqrt(sol.eps[a] * site.eps)
+                sg = 0.5 * (sol.sig[a] + site.sig)
+                ref -= (16.0 * 3.14159 / 3.0) * e * sg ** 6 * 0.0333 / 10.0 ** 3
+    assert abs(eng.lj_tail_correction(sol, solv, 10.0) - ref) < 1e-12 * abs(ref)
