@@ -245,6 +245,14 @@ def main():
             t_fast, res_f, dg_fast = fast(True)
         else:
             t_fast, res_f, dg_fast = t_cold32, res_c32, dg_c32
+        # reference-style solvation free energy of the fast path's final density
+        # (src/module_pressure_correction.f90: F + PMV correction + LJ tail), outside the timed region
+        pc = eng.pressure_correction(xi)
+        lj = eng.lj_tail_correction(sol, w.solvent)
+        line["dG_post_processing"] = {"F_kJmol": dg_fast, "pmv_correction": pc["pmv_correction"],
+                                      "lj_tail_correction": lj, "pmv_A3": pc["pmv_A3"],
+                                      "P_bulk_atm": pc["P_bulk_kJmol_A3"] * 16394.078514951,
+                                      "estimated_solvation_free_energy": dg_fast + pc["pmv_correction"] + lj}
         line.update({
             "value": t_fast, "unit": "s", "steps": res_f.evaluations,
             "ms_per_step": t_fast / max(res_f.evaluations, 1) * 1e3,

@@ -543,7 +543,7 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
 }
 
 constexpr int kPTt = 4;   // points per workgroup of the specialised kernels, FP64
-constexpr int kPT32 = 8;  // the same for FP32 (half the LDS per point; amortises the table staging)
+constexpr int kPT32 = 4;  // the same for FP32 (8 measured slower at 128^3, mmax = 5)
 
 hipError_t launch_angl2proj_f64(mdft_ctx *c, const double *xi, double2 *proj) {
     if (c->d.mrso == 2 && launch_a2p_t<kPTt, double>(c, xi, proj)) return hipGetLastError();
