@@ -223,7 +223,6 @@ def main():
         e32.set_ck(ckt)
         e32.compute_vext(sol, w.solvent)
         net = wsm.WarmStartNet.load(args.weights) if os.path.exists(args.weights) else None
-        scratch = e32.alloc_field() if net is not None else None
         info = {}
 
         def fast(use_net):
@@ -231,7 +230,7 @@ def main():
             eng.synchronize()
             t0 = time.perf_counter()
             if use_net:
-                info.update(wsm.warm_start(e32, net, xi, fallback=scratch))
+                info.update(wsm.warm_start(e32, net, xi))
             else:
                 wsm.cold_start(e32, xi)
             r = minimize(e32, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref,

@@ -47,6 +47,14 @@ int mdft_wsnet_forward(mdft_wsnet *net, int nx, int ny, int nz, const float *d_i
 int mdft_wsnet_loss_grad(mdft_wsnet *net, int nx, int ny, int nz, const float *d_in, const float *d_target,
                          const float *d_mask, float *d_gin, double *loss);
 
+/* Backward pass from a given gradient of the output, d_gout [C_L][N] (device):
+ * accumulates nothing -- the stored parameter gradients are overwritten, as by
+ * mdft_wsnet_loss_grad -- and writes dL/d(input) to d_gin when not NULL.  Must
+ * follow mdft_wsnet_forward on the same d_in and grid (it uses the activations
+ * that call kept). */
+int mdft_wsnet_backward(mdft_wsnet *net, int nx, int ny, int nz, const float *d_in, const float *d_gout,
+                        float *d_gin);
+
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with the stored gradients. */
 int mdft_wsnet_adam(mdft_wsnet *net, float lr);
 
@@ -56,8 +64,10 @@ int mdft_wsnet_adam(mdft_wsnet *net, float lr);
  * (the reference's projection order), 2 ip + 1 the imaginary part.
  * Molecular symmetry order 2 only. */
 
-/* d_feat [2 np][N]: projections of exp(-max(beta vext, -8) / 2) - 1. */
-int mdft_ws_features(struct mdft_ctx *ctx, float *d_feat);
+/* d_feat [2 np][N]: projections of clip(-beta vext / 2, -4, 4) / 4 (the bounded log of the
+ * square-root Boltzmann factor), channel k multiplied by h_scale[k] (host array of 2 np
+ * floats; NULL: no scaling) -- the network's input normalisation. */
+int mdft_ws_features(struct mdft_ctx *ctx, float *d_feat, const float *h_scale);
 
 /* For the density xi (device): d_gamma [N][no] (FP64, may be NULL) receives
  * gamma(r, Omega) = c (*) drho in units of ln(rho/rho0), i.e.
@@ -65,21 +75,30 @@ int mdft_ws_features(struct mdft_ctx *ctx, float *d_feat);
  * may be NULL) receives the projections of gamma. */
 int mdft_ws_gamma(struct mdft_ctx *ctx, const double *d_xi, double *d_gamma, float *d_target);
 
-/* d_target [2 np][N] (FP32): projections of rho/rho0 - c for the density xi
- * (device), c the reference cold start (0 where beta vext >= -ln eps - 0.05,
- * else 1): the part of the density the cold start misses. */
+/* d_target [2 np][N] (FP32): projections of rho/rho0 - 1 for the density xi
+ * (device): 0 in the bulk, -1 where the solvent is excluded. */
 int mdft_ws_density(struct mdft_ctx *ctx, const double *d_xi, float *d_target);
 
 enum { MDFT_WS_GAMMA = 0, MDFT_WS_DENSITY = 1 };
 
 /* Starting density from predicted projections d_pred [2 np][N]:
- * kind MDFT_WS_DENSITY: d_pred are projections of rho/rho0 - c (see mdft_ws_density),
- *   d_xi = 0 where c = 0, else sqrt(max(1 + residual, 0));
+ * kind MDFT_WS_DENSITY: d_pred are projections of rho/rho0 - 1 (see mdft_ws_density),
+ *   d_xi = 0 where the cold start (below) is 0, else sqrt(max(1 + pred, 0.01)) -- the
+ *   floor keeps every point the minimiser may fill away from xi = 0, where the
+ *   gradient vanishes;
  * kind MDFT_WS_GAMMA: d_pred are projections of gamma, d_xi = exp((-beta vext + gamma) / 2),
  * exponent clipped to [-200, 7].
  * With d_pred NULL: the reference's cold start (src/module_density.f90:267-284),
  * xi = 0 where vext >= (-ln eps - 0.05) kT, else 1.  All on the device. */
 int mdft_ws_guess(struct mdft_ctx *ctx, const float *d_pred, int kind, double *d_xi);
+
+/* Gradient of F at a density guess with respect to the prediction it came from:
+ * d_xi is mdft_ws_guess(d_pred, MDFT_WS_DENSITY) and d_df the gradient of
+ * mdft_energy_and_gradient at d_xi (device, [N][no], dF = (1/dv) dF/dxi);
+ * d_gpred [2 np][N] (FP32) receives dF/d(pred), zero through the floor and the
+ * excluded points.  With mdft_wsnet_backward this trains the network on F
+ * itself. */
+int mdft_ws_start_gradient(struct mdft_ctx *ctx, const double *d_xi, const double *d_df, float *d_gpred);
 
 /* d_out = sqrt((1 - lam) xa^2 + lam xb^2): xi of the density mixture (1 - lam) rho_a + lam rho_b. */
 int mdft_ws_mix(struct mdft_ctx *ctx, double lam, const double *d_xa, const double *d_xb, double *d_out);

@@ -476,7 +476,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
-                    c->d_bf16_M, c->d_bf16_Q};
+                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);
@@ -907,12 +907,17 @@ int mdft_synchronize(mdft_ctx *c) {
 
 // ---- learned warm start: coupling to the context (no reference counterpart)
 
-int mdft_ws_features(mdft_ctx *c, float *d_feat) {
+int mdft_ws_features(mdft_ctx *c, float *d_feat, const float *h_scale) {
     if (!c || !d_feat) return MDFT_EINVAL;
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
     if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the warm start needs molecular symmetry order 2");
+    if (h_scale) {
+        const size_t bytes = sizeof(float) * 2 * c->np;
+        if (!c->d_chscale) HIPCHK(c, hipMalloc(&c->d_chscale, bytes));
+        HIPCHK(c, hipMemcpyAsync(c->d_chscale, h_scale, bytes, hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(c, launch_angl2proj_feat(c, c->d_vext, 1));
-    HIPCHK(c, launch_proj_to_channels(c, d_feat, 1.0f));
+    HIPCHK(c, launch_proj_to_channels(c, d_feat, 1.0f, h_scale ? c->d_chscale : nullptr));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }
@@ -940,7 +945,7 @@ int mdft_ws_density(mdft_ctx *c, const double *d_xi, float *d_target) {
     if (!d_xi || !d_target) return fail(c, MDFT_EINVAL, "null argument");
     if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the warm start needs molecular symmetry order 2");
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
-    HIPCHK(c, launch_angl2proj_feat(c, d_xi, 2));  // projections of rho/rho0 - c
+    HIPCHK(c, launch_angl2proj_feat(c, d_xi, 2));  // projections of rho/rho0 - 1
     HIPCHK(c, launch_proj_to_channels(c, d_target, 1.0f));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
@@ -957,6 +962,34 @@ int mdft_ws_guess(mdft_ctx *c, const float *d_pred, int kind, double *d_xi) {
         int nb = 0;
         HIPCHK(c, launch_final(c, nullptr, d_xi, kind == MDFT_WS_GAMMA ? 3 : 4, &nb));
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+int mdft_ws_start_gradient(mdft_ctx *c, const double *d_xi, const double *d_df, float *d_gpred) {
+    if (!c || !d_xi || !d_df || !d_gpred) return MDFT_EINVAL;
+    if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
+    if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the warm start needs molecular symmetry order 2");
+    if (!c->d_adjscale) {
+        // proj2angl^T = D angl2proj W^-1 (W: the quadrature weights), D = 8 pi^2 / mrso on the
+        // channels of mu = 0, twice that for mu > 0 (each stands for its mu < 0 partner too),
+        // and 0 on the imaginary channel of (m, mup = 0, mu = 0), which a real field cannot carry
+        const double s = 8.0 * acos(-1.0) * acos(-1.0) / c->d.mrso;
+        std::vector<float> h(2 * (size_t)c->np);
+        size_t ip = 0;
+        for (int m = 0; m <= c->L; m++)
+            for (int mup = -m; mup <= m; mup++)
+                for (int mu = 0; mu <= m; mu += c->d.mrso, ip++) {
+                    const double f = (mu > 0) ? 2.0 * s : s;
+                    h[2 * ip] = (float)f;
+                    h[2 * ip + 1] = (mup == 0 && mu == 0) ? 0.0f : (float)f;
+                }
+        if (ip != (size_t)c->np) return fail(c, MDFT_EINVAL, "projection count mismatch");
+        HIPCHK(c, hipMalloc(&c->d_adjscale, sizeof(float) * h.size()));
+        HIPCHK(c, hipMemcpy(c->d_adjscale, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, launch_angl2proj_feat(c, d_xi, 3, d_df));
+    HIPCHK(c, launch_proj_to_channels(c, d_gpred, 1.0f, c->d_adjscale));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDFT_OK;
 }

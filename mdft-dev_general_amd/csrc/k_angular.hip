@@ -135,8 +135,9 @@ struct FinalConsts {
 // mode 1: df += excess gradient (excess_energy plug-in)
 // mode 2: df = gamma(r, Omega) = -(prefactor / kT) proj2angl(proj), the OZ field in units of ln(rho/rho0)
 // mode 3: proj holds gamma's projections, df = exp((-beta vext + proj2angl(proj)) / 2) (xi not read)
-// mode 4: proj holds the projections of rho/rho0 - c, c the reference cold start (0 where
-// beta vext >= -ln(DBL_EPSILON) - 0.05, else 1): df = 0 where c = 0, else sqrt(max(1 + proj2angl(proj), 0))
+// mode 4: proj holds the projections of rho/rho0 - 1: df = 0 where the reference cold start
+// excludes the solvent (beta vext >= -ln(DBL_EPSILON) - 0.05), else
+// sqrt(max(1 + proj2angl(proj), MDFT_WS_DENSITY_FLOOR))
 template <int PT>
 __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict__ proj, const double *__restrict__ xi,
                                                      const double *__restrict__ vext, double *__restrict__ df,
@@ -231,9 +232,9 @@ __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict_
             if (df) df[g] = -(k.prefactor / kT) * v;
         } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
             df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
-        } else {  // mode 4: proj holds the projections of rho/rho0 - c (c: the cold start);
-                  // df = sqrt(max(c + v, 0)), 0 where the cold start excludes the solvent
-            df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, 0.0));
+        } else {  // mode 4: proj holds the projections of rho/rho0 - 1;
+                  // df = sqrt(max(1 + v, floor)), 0 where the cold start excludes the solvent
+            df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -307,13 +308,17 @@ struct Dims {
     static constexpr int NP = np_of(L);
 };
 
-// feat = 0: o = rho0 (xi^2 - 1) (the excess term); feat = 1: o = exp(-max(beta xi, -8) / 2) - 1
-// with xi the external potential (features of the learned warm start); feat = 2:
-// o = xi^2 - c with c the reference cold start from aux = vext (0 where
-// beta vext >= -ln(DBL_EPSILON) - 0.05, else 1): the warm start's density residual
+// feat = 0: o = rho0 (xi^2 - 1) (the excess term); feat = 1: o = clip(-beta xi / 2, -4, 4) / 4
+// with xi the external potential -- the log of the square-root Boltzmann factor, bounded
+// (features of the learned warm start); feat = 2: o = xi^2 - 1 = rho/rho0 - 1 (the
+// warm start's density target; 0 in the bulk, -1 where the solvent is excluded); feat = 3:
+// o = k0 aux / (xi w(io)) where xi^2 > MDFT_WS_DENSITY_FLOOR, else 0, with aux = dF at the
+// density guess xi and k0 = dv / 2: dF/dr / w for r = rho/rho0, whose angl2proj is the
+// adjoint of proj2angl up to a per-projection factor (the warm start's training gradient).
+// k0 is rho0 for feat 0.
 template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
-                                                       size_t N, AngTables t, double rho0, int feat, double beta,
+                                                       size_t N, AngTables t, double k0, int feat, double beta,
                                                        const double *__restrict__ aux) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
@@ -333,9 +338,11 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     const double *src = xi + r0 * NO;
     for (int i = tid; i < npts * NO; i += kThreads) {
         const double x = src[i];
-        if (feat == 0) o[i] = (T)(rho0 * (x * x - 1.0));
-        else if (feat == 1) o[i] = (T)(exp(-0.5 * fmax(beta * x, -8.0)) - 1.0);
-        else o[i] = (T)(x * x - ((beta * aux[r0 * NO + i] >= -log(DBL_EPSILON) - 0.05) ? 0.0 : 1.0));
+        if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
+        else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
+        else if (feat == 2) o[i] = (T)(x * x - 1.0);
+        else o[i] = (x * x > MDFT_WS_DENSITY_FLOOR * (1.0 + 1e-12)) ? (T)(k0 * aux[r0 * NO + i] / (x * t.w[i % NO]))
+                                                                   : (T)0;
     }
     __syncthreads();
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
@@ -487,9 +494,9 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
             if (df) df[g] = -(k.prefactor / kT) * v;
         } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
             df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
-        } else {  // mode 4: proj holds the projections of rho/rho0 - c (c: the cold start);
-                  // df = sqrt(max(c + v, 0)), 0 where the cold start excludes the solvent
-            df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, 0.0));
+        } else {  // mode 4: proj holds the projections of rho/rho0 - 1;
+                  // df = sqrt(max(1 + v, floor)), 0 where the cold start excludes the solvent
+            df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -521,9 +528,10 @@ template <int PT, typename T>
 bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int feat = 0, const double *aux = nullptr) {
     const size_t nb = (c->N + PT - 1) / PT;
     const double beta = 1.0 / c->d.kT;
+    const double k0 = (feat == 3) ? 0.5 * c->dv : c->rho0;
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0, feat, beta, aux); return true;
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, k0, feat, beta, aux); return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;
@@ -638,11 +646,13 @@ hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, d
 }
 
 // angl2proj of a warm-start field into the context's projection buffer (mrso = 2 kernels):
-// feat 1: the feature field of src = vext; feat 2: the density residual of src = xi against the cold start
-hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat) {
+// feat 1: the feature field of src = vext; feat 2: the density target rho/rho0 - 1 of src = xi;
+// feat 3: the start gradient field of src = xi with aux = dF
+hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat, const double *aux) {
     if (c->d.mrso != 2) return hipErrorNotSupported;
+    if (!aux) aux = c->d_vext;
     bool ok = (c->d.precision == MDFT_PREC_FP64)
-                  ? launch_a2p_t<kPTt, double>(c, src, static_cast<double2 *>(c->d_proj), feat, c->d_vext)
-                  : launch_a2p_t<kPT32, float>(c, src, static_cast<float2 *>(c->d_proj), feat, c->d_vext);
+                  ? launch_a2p_t<kPTt, double>(c, src, static_cast<double2 *>(c->d_proj), feat, aux)
+                  : launch_a2p_t<kPT32, float>(c, src, static_cast<float2 *>(c->d_proj), feat, aux);
     return ok ? hipGetLastError() : hipErrorNotSupported;
 }

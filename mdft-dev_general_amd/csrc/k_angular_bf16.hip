@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void final_bf16(const float *__restrict__
                 } else if (mode == 3) {
                     df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
                 } else {
-                    df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, 0.0));
+                    df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
                 }
             }
         }

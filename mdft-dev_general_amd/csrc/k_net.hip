@@ -311,6 +311,7 @@ struct mdft_wsnet {
     size_t part_cap = 0;
     double *dpart = nullptr, *hpart = nullptr;
     long long step = 0;
+    const float *fwd_in = nullptr;  // input of the last forward pass (its activations are kept)
     hipStream_t stream = nullptr;
     std::string err;
 };
@@ -358,6 +359,7 @@ int ensure_grid(mdft_wsnet *n, int nx, int ny, int nz) {
 }
 
 int forward(mdft_wsnet *n, const float *in) {
+    n->fwd_in = in;
     const Geo g{n->nx, n->ny, n->nz};
     const size_t N = (size_t)g.nx * g.ny * g.nz;
     const float *src = in;
@@ -381,6 +383,65 @@ template <int CO>
 void launch_wgrad(mdft_wsnet *n, const float *dz, const float *a, int cin, int cout, Geo g, int d, unsigned nblk) {
     const int thr = ((cin * 27 + 1 + 63) / 64) * 64;
     hipLaunchKernelGGL((conv_bwd_weight<CO>), dim3(nblk), dim3(thr), 0, n->stream, dz, a, n->part, cin, cout, g, d);
+}
+
+// Backward pass from the output gradient in n->gA, for the activations of the
+// last forward pass on `d_in`: parameter gradients into n->grads, dL/d(input)
+// into d_gin (when not null).
+int backward(mdft_wsnet *n, const float *d_in, float *d_gin) {
+    const Geo g{n->nx, n->ny, n->nz};
+    const int N = g.nx * g.ny * g.nz;
+    const int L = n->nl;
+    // weight-gradient partials
+    const unsigned nblk = (unsigned)((N + kWChunk - 1) / kWChunk);
+    size_t need = 0;
+    for (int l = 0; l < L; l++)
+        need = std::max(need, (size_t)nblk * ((size_t)n->ch[l + 1] * n->ch[l] * n->taps[l] + n->ch[l + 1]));
+    if (need > n->part_cap) {
+        (void)hipFree(n->part);
+        n->part = nullptr;
+        WSCHK(n, hipMalloc(&n->part, sizeof(float) * need));
+        n->part_cap = need;
+    }
+    float *gcur = n->gA, *gnext = n->gB;
+    for (int l = L; l >= 1; l--) {
+        const int cin = n->ch[l - 1], cout = n->ch[l], d = n->dil[l - 1];
+        if (l < L) {
+            hipLaunchKernelGGL(relu_bwd, dim3(grid1((size_t)N * cout)), dim3(kThreads), 0, n->stream, gcur, n->act[l],
+                               (size_t)N * cout);
+            WSCHK(n, hipGetLastError());
+        }
+        const float *a = (l == 1) ? d_in : n->act[l - 1];
+        const int T = n->taps[l - 1];
+        if (T == 27) {
+            if (cout <= 8) launch_wgrad<8>(n, gcur, a, cin, cout, g, d, nblk);
+            else if (cout <= 16) launch_wgrad<16>(n, gcur, a, cin, cout, g, d, nblk);
+            else launch_wgrad<32>(n, gcur, a, cin, cout, g, d, nblk);
+        } else {
+            const int thr = ((cin + 1 + 63) / 64) * 64;
+            hipLaunchKernelGGL(pw_bwd_weight, dim3(nblk, (unsigned)((cout + 31) / 32)), dim3(thr), 0, n->stream, gcur,
+                               a, n->part, cin, cout, N);
+        }
+        WSCHK(n, hipGetLastError());
+        const int nw = cout * cin * T + cout;  // W_l then b_l, contiguous in the parameter vector
+        hipLaunchKernelGGL(sum_parts, dim3((nw + kThreads - 1) / kThreads), dim3(kThreads), 0, n->stream, n->part,
+                           (int)nblk, nw, n->grads + n->woff[l - 1]);
+        WSCHK(n, hipGetLastError());
+        if (l > 1 || d_gin) {
+            float *dst = (l == 1) ? d_gin : gnext;
+            dim3 grid((unsigned)((N + kThreads - 1) / kThreads), (unsigned)((cin + KG - 1) / KG));
+            const size_t lds = sizeof(float) * cout * KG * T;
+            if (T == 27)
+                hipLaunchKernelGGL(conv_bwd_data<27>, grid, dim3(kThreads), lds, n->stream, gcur,
+                                   n->params + n->woff[l - 1], dst, cin, cout, g, d);
+            else
+                hipLaunchKernelGGL(conv_bwd_data<1>, grid, dim3(kThreads), lds, n->stream, gcur,
+                                   n->params + n->woff[l - 1], dst, cin, cout, g, 1);
+            WSCHK(n, hipGetLastError());
+            std::swap(gcur, gnext);
+        }
+    }
+    return MDFT_OK;
 }
 
 }  // namespace
@@ -533,60 +594,25 @@ int mdft_wsnet_loss_grad(mdft_wsnet *n, int nx, int ny, int nz, const float *d_i
     hipLaunchKernelGGL(mse, dim3(nb), dim3(kThreads), 0, n->stream, n->act[L], d_target, d_mask, n->gA, n->ch[L], N,
                        1.0 / cnt, n->dpart);
     WSCHK(n, hipGetLastError());
-    // weight-gradient partials
-    const unsigned nblk = (unsigned)((N + kWChunk - 1) / kWChunk);
-    size_t need = 0;
-    for (int l = 0; l < L; l++)
-        need = std::max(need, (size_t)nblk * ((size_t)n->ch[l + 1] * n->ch[l] * n->taps[l] + n->ch[l + 1]));
-    if (need > n->part_cap) {
-        (void)hipFree(n->part);
-        n->part = nullptr;
-        WSCHK(n, hipMalloc(&n->part, sizeof(float) * need));
-        n->part_cap = need;
-    }
-    float *gcur = n->gA, *gnext = n->gB;
-    for (int l = L; l >= 1; l--) {
-        const int cin = n->ch[l - 1], cout = n->ch[l], d = n->dil[l - 1];
-        if (l < L) {
-            hipLaunchKernelGGL(relu_bwd, dim3(grid1((size_t)N * cout)), dim3(kThreads), 0, n->stream, gcur, n->act[l],
-                               (size_t)N * cout);
-            WSCHK(n, hipGetLastError());
-        }
-        const float *a = (l == 1) ? d_in : n->act[l - 1];
-        const int T = n->taps[l - 1];
-        if (T == 27) {
-            if (cout <= 8) launch_wgrad<8>(n, gcur, a, cin, cout, g, d, nblk);
-            else if (cout <= 16) launch_wgrad<16>(n, gcur, a, cin, cout, g, d, nblk);
-            else launch_wgrad<32>(n, gcur, a, cin, cout, g, d, nblk);
-        } else {
-            const int thr = ((cin + 1 + 63) / 64) * 64;
-            hipLaunchKernelGGL(pw_bwd_weight, dim3(nblk, (unsigned)((cout + 31) / 32)), dim3(thr), 0, n->stream, gcur,
-                               a, n->part, cin, cout, N);
-        }
-        WSCHK(n, hipGetLastError());
-        const int nw = cout * cin * T + cout;  // W_l then b_l, contiguous in the parameter vector
-        hipLaunchKernelGGL(sum_parts, dim3((nw + kThreads - 1) / kThreads), dim3(kThreads), 0, n->stream, n->part,
-                           (int)nblk, nw, n->grads + n->woff[l - 1]);
-        WSCHK(n, hipGetLastError());
-        if (l > 1 || d_gin) {
-            float *dst = (l == 1) ? d_gin : gnext;
-            dim3 grid((unsigned)((N + kThreads - 1) / kThreads), (unsigned)((cin + KG - 1) / KG));
-            const size_t lds = sizeof(float) * cout * KG * T;
-            if (T == 27)
-                hipLaunchKernelGGL(conv_bwd_data<27>, grid, dim3(kThreads), lds, n->stream, gcur,
-                                   n->params + n->woff[l - 1], dst, cin, cout, g, d);
-            else
-                hipLaunchKernelGGL(conv_bwd_data<1>, grid, dim3(kThreads), lds, n->stream, gcur,
-                                   n->params + n->woff[l - 1], dst, cin, cout, g, 1);
-            WSCHK(n, hipGetLastError());
-            std::swap(gcur, gnext);
-        }
-    }
+    if ((rc = backward(n, d_in, d_gin))) return rc;
     WSCHK(n, hipMemcpyAsync(n->hpart, n->dpart, sizeof(double) * nb, hipMemcpyDeviceToHost, n->stream));
     WSCHK(n, hipStreamSynchronize(n->stream));
     double s = 0.0;
     for (unsigned i = 0; i < nb; i++) s += n->hpart[i];
     *loss = s / cnt;
+    return MDFT_OK;
+}
+
+int mdft_wsnet_backward(mdft_wsnet *n, int nx, int ny, int nz, const float *d_in, const float *d_gout,
+                        float *d_gin) {
+    if (!n || !d_in || !d_gout) return MDFT_EINVAL;
+    if (n->nx != nx || n->ny != ny || n->nz != nz || n->fwd_in != d_in)
+        return ws_fail(n, MDFT_ESTATE, "backward must follow forward on the same input and grid");
+    const size_t N = (size_t)nx * ny * nz;
+    WSCHK(n, hipMemcpyAsync(n->gA, d_gout, sizeof(float) * N * n->ch[n->nl], hipMemcpyDeviceToDevice, n->stream));
+    int rc = backward(n, d_in, d_gin);
+    if (rc) return rc;
+    WSCHK(n, hipStreamSynchronize(n->stream));
     return MDFT_OK;
 }
 

@@ -17,10 +17,12 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kTile = 32;  // grid points per workgroup in the transposes
 
-// out[2 ip + c][p] = scale * proj[p][ip].c  (via an LDS tile: both sides coalesced)
+// out[2 ip + c][p] = scale * chscale[2 ip + c] * proj[p][ip].c, chscale optional
+// (via an LDS tile: both sides coalesced)
 template <typename V>
 __global__ __launch_bounds__(kThreads) void proj_to_channels(const V *__restrict__ proj, float *__restrict__ out,
-                                                            size_t N, int np, float scale) {
+                                                            size_t N, int np, float scale,
+                                                            const float *__restrict__ chscale) {
     extern __shared__ float tile[];  // [kTile][2 np]
     const int w2 = 2 * np;
     const size_t p0 = (size_t)blockIdx.x * kTile;
@@ -30,7 +32,7 @@ __global__ __launch_bounds__(kThreads) void proj_to_channels(const V *__restrict
     __syncthreads();
     for (int i = threadIdx.x; i < kTile * w2; i += kThreads) {
         const int ch = i / kTile, pt = i % kTile;
-        if (pt < npts) out[(size_t)ch * N + p0 + pt] = scale * tile[pt * w2 + ch];
+        if (pt < npts) out[(size_t)ch * N + p0 + pt] = (chscale ? scale * chscale[ch] : scale) * tile[pt * w2 + ch];
     }
 }
 
@@ -80,15 +82,15 @@ hipError_t launch_density_mix(mdft_ctx *c, size_t total, double lam, const doubl
     return hipGetLastError();
 }
 
-hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale) {
+hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale, const float *chscale) {
     const size_t nb = (c->N + kTile - 1) / kTile;
     const size_t lds = sizeof(float) * kTile * 2 * c->np;
     if (c->d.precision == MDFT_PREC_FP64)
         hipLaunchKernelGGL(proj_to_channels<double2>, dim3((unsigned)nb), dim3(kThreads), lds, c->stream,
-                           static_cast<const double2 *>(c->d_proj), out, c->N, c->np, scale);
+                           static_cast<const double2 *>(c->d_proj), out, c->N, c->np, scale, chscale);
     else
         hipLaunchKernelGGL(proj_to_channels<float2>, dim3((unsigned)nb), dim3(kThreads), lds, c->stream,
-                           static_cast<const float2 *>(c->d_proj), out, c->N, c->np, scale);
+                           static_cast<const float2 *>(c->d_proj), out, c->N, c->np, scale, chscale);
     return hipGetLastError();
 }
 

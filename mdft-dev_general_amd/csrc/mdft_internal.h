@@ -11,6 +11,10 @@
 
 #define MDFT_MAX_MMAX 6
 #define MDFT_VEC_MAX 16  // vectors per multi-dot / combine pass
+// Floor of rho/rho0 in the density guess of the warm start where the cold start is 1:
+// at xi = 0 both gradient terms vanish (they carry the factor 2 rho0 xi), so a point
+// predicted empty could never be refilled by the minimiser.
+#define MDFT_WS_DENSITY_FLOOR 1e-2
 #define MDFT_OZ32_ROWS_PER_LANE 4  // OZ rows per lane in the FP32 q-space kernel (np <= 256)
 #define MDFT_FFT_MAX_STAGES 12
 
@@ -84,6 +88,8 @@ struct mdft_ctx {
     double *d_sums = nullptr;     // [8] final reductions
     double *d_vpart = nullptr;    // vector-op block partials
     double *d_mpart = nullptr;    // multi-dot block partials [blocks][MDFT_VEC_MAX] + result
+    float *d_chscale = nullptr;   // per-channel input scale of the warm-start features [2 np]
+    float *d_adjscale = nullptr;  // per-channel factor of the adjoint of proj2angl [2 np]
     double *h_sums = nullptr;     // pinned mirror of d_sums
     size_t partials_cap = 0;
     bool have_vext = false, have_ck = false;
@@ -113,8 +119,8 @@ hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
 hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
-hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat);
-hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale);
+hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat, const double *aux = nullptr);
+hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale, const float *chscale = nullptr);
 hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in);
 hipError_t launch_cold_start(mdft_ctx *c, double *xi);
 hipError_t launch_density_mix(mdft_ctx *c, size_t total, double lam, const double *xa, const double *xb,

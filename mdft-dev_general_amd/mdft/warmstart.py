@@ -5,12 +5,18 @@ The reference has no learned component; the minimiser it drives
 src/module_density.f90:267-284.  Here a periodic 3D convolutional network
 (hand-written HIP forward/backward/Adam kernels, FP32) maps the angular
 projections of a feature field of the external potential to the angular
-projections of the part of the converged density the reference's cold start
-misses, rho/rho0 - c (c = 0 where the cold start excludes the solvent, 1
-elsewhere), and the minimiser starts from xi = sqrt(max(c + residual, 0))
-(0 where c = 0) -- or from the best of a few density mixtures
-(1 - lam) rho_cold + lam rho_pred, the cold start included, judged by F.  The minimisation afterwards is the plain one: the network changes the
-number of iterations, not the answer, and the cold start stays the fallback.
+projections of rho/rho0 - 1 of the converged density, and the minimiser
+starts from xi = sqrt(max(1 + prediction, 0.01)), 0 where the reference's
+cold start excludes the solvent.  A zero prediction is the cold start.  The
+floor matters: at xi = 0 both gradient terms vanish, so a point predicted
+empty would stay empty.  (The target is taken against 1 rather than against
+the cold start: the cold start's exclusion boundary, beta vext = 36, is
+invisible in the features, which saturate beyond beta vext = 8, and a target
+with a jump there cannot be learned.)  With a scratch field the start is
+the better (by F) of the prediction and the cold start, or of density
+mixtures (1 - lam) rho_cold + lam rho_pred.  The minimisation afterwards is
+the plain one: the network changes the number of iterations, not the answer,
+and the cold start stays the fallback.
 
 (A prediction of the OZ field gamma with the Euler-Lagrange start
 xi = exp((-beta vext + gamma)/2) is also available, kind MDFT_WS_GAMMA; its
@@ -47,7 +53,7 @@ WSNET_EXPORTS = (
     "mdft_wsnet_create", "mdft_wsnet_destroy", "mdft_wsnet_last_error", "mdft_wsnet_num_params",
     "mdft_wsnet_get_params", "mdft_wsnet_set_params", "mdft_wsnet_get_grads", "mdft_wsnet_forward",
     "mdft_wsnet_loss_grad", "mdft_wsnet_adam", "mdft_ws_features", "mdft_ws_gamma", "mdft_ws_density",
-    "mdft_ws_guess", "mdft_ws_mix",
+    "mdft_ws_guess", "mdft_ws_mix", "mdft_wsnet_backward", "mdft_ws_start_gradient",
 )
 WS_GAMMA, WS_DENSITY = 0, 1
 
@@ -72,11 +78,13 @@ def _lib():
         lib.mdft_wsnet_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp]
         lib.mdft_wsnet_loss_grad.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(C.c_double)]
         lib.mdft_wsnet_adam.argtypes = [vp, C.c_float]
-        lib.mdft_ws_features.argtypes = [vp, vp]
+        lib.mdft_ws_features.argtypes = [vp, vp, vp]
         lib.mdft_ws_gamma.argtypes = [vp, vp, vp, vp]
         lib.mdft_ws_density.argtypes = [vp, vp, vp]
         lib.mdft_ws_guess.argtypes = [vp, vp, C.c_int, vp]
         lib.mdft_ws_mix.argtypes = [vp, C.c_double, vp, vp, vp]
+        lib.mdft_wsnet_backward.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+        lib.mdft_ws_start_gradient.argtypes = [vp, vp, vp, vp]
         for n in WSNET_EXPORTS:
             if n not in ("mdft_wsnet_destroy", "mdft_wsnet_last_error", "mdft_wsnet_num_params"):
                 getattr(lib, n).restype = C.c_int
@@ -89,6 +97,7 @@ class WarmStartNet:
 
     def __init__(self, channels, ksize, dilations, seed: int = 0, device: int = 0):
         self.lib = _lib()
+        self.input_scale = None   # per-channel feature scale [2 np] (float32) or None
         self.channels = tuple(int(c) for c in channels)
         self.ksize = tuple(int(k) for k in ksize)
         self.dilations = tuple(int(d) for d in dilations)
@@ -155,19 +164,27 @@ class WarmStartNet:
                                                   _ptr(d_mask), _ptr(d_gin), C.byref(loss)))
         return loss.value
 
+    def backward(self, n, d_in, d_gout, d_gin=None) -> None:
+        """Parameter gradients (and dL/d(input) into d_gin) for the output gradient
+        d_gout; follows forward() on the same d_in."""
+        self._check(self.lib.mdft_wsnet_backward(self._h, n[0], n[1], n[2], _ptr(d_in), _ptr(d_gout), _ptr(d_gin)))
+
     def adam(self, lr: float) -> None:
         self._check(self.lib.mdft_wsnet_adam(self._h, float(lr)))
 
     def save(self, path: str, **meta) -> None:
         os.makedirs(os.path.dirname(path), exist_ok=True)
+        extra = {} if self.input_scale is None else {"input_scale": np.asarray(self.input_scale, np.float32)}
         np.savez(path, params=self.get_params(), channels=np.array(self.channels), ksize=np.array(self.ksize),
-                 dilations=np.array(self.dilations), **{k: np.asarray(v) for k, v in meta.items()})
+                 dilations=np.array(self.dilations), **extra, **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
     def load(cls, path: str = DEFAULT_WEIGHTS, device: int = 0) -> "WarmStartNet":
         with np.load(path, allow_pickle=False) as z:
             net = cls(tuple(z["channels"]), tuple(z["ksize"]), tuple(z["dilations"]), device=device)
             net.set_params(z["params"])
+            if "input_scale" in z.files:
+                net.input_scale = np.ascontiguousarray(z["input_scale"], dtype=np.float32)
         return net
 
     @classmethod
@@ -181,11 +198,31 @@ def _ptr(b):
     return b.ptr if isinstance(b, DeviceBuffer) else b
 
 
-def features(eng: Engine) -> DeviceBuffer:
-    """[2 np][N] FP32: projections of the feature field of the engine's vext."""
+def features(eng: Engine, scale=None) -> DeviceBuffer:
+    """[2 np][N] FP32: projections of the feature field clip(-beta vext / 2, -4, 4) / 4
+    of the engine's vext, channel k times scale[k] (the network's input scale)."""
     out = DeviceBuffer(eng, 4 * 2 * eng.np_ * eng.N)
-    eng._check(_lib().mdft_ws_features(eng._h, out.ptr))
+    sc = None
+    if scale is not None:
+        sc = np.ascontiguousarray(scale, dtype=np.float32)
+        assert sc.size == 2 * eng.np_
+    eng._check(_lib().mdft_ws_features(eng._h, out.ptr, None if sc is None else sc.ctypes.data))
     return out
+
+
+def input_scale(eng: Engine, solutes, solvent) -> np.ndarray:
+    """Per-channel input scale 1 / rms of the unscaled features over `solutes`
+    (channels that are zero throughout get 1): every input channel of the
+    network then has unit rms on the training distribution."""
+    acc = np.zeros(2 * eng.np_)
+    for sol in solutes:
+        eng.compute_vext(sol, solvent)
+        f = features(eng)
+        a = f.download(np.float32).reshape(2 * eng.np_, eng.N).astype(np.float64)
+        f.free()
+        acc += (a * a).mean(axis=1)
+    rms = np.sqrt(acc / max(len(solutes), 1))
+    return np.where(rms > 1e-12, 1.0 / np.maximum(rms, 1e-12), 1.0).astype(np.float32)
 
 
 def gamma(eng: Engine, xi: DeviceBuffer, want_field: bool = False):
@@ -198,7 +235,7 @@ def gamma(eng: Engine, xi: DeviceBuffer, want_field: bool = False):
 
 
 def density(eng: Engine, xi: DeviceBuffer) -> DeviceBuffer:
-    """Projections [2 np][N] (FP32) of rho/rho0 - c for xi (c: the cold start)."""
+    """Projections [2 np][N] (FP32) of rho/rho0 - 1 for xi."""
     tgt = DeviceBuffer(eng, 4 * 2 * eng.np_ * eng.N)
     eng._check(_lib().mdft_ws_density(eng._h, xi.ptr, tgt.ptr))
     return tgt
@@ -217,19 +254,27 @@ def mix(eng: Engine, lam: float, xa: DeviceBuffer, xb: DeviceBuffer, out: Device
     eng._check(_lib().mdft_ws_mix(eng._h, float(lam), xa.ptr, xb.ptr, out.ptr))
 
 
+def start_gradient(eng: Engine, xi: DeviceBuffer, df: DeviceBuffer, out: DeviceBuffer | None = None) -> DeviceBuffer:
+    """dF/d(pred) [2 np][N] (FP32) at the density guess xi = guess(pred), given
+    dF at xi from energy_and_gradient."""
+    out = out or DeviceBuffer(eng, 4 * 2 * eng.np_ * eng.N)
+    eng._check(_lib().mdft_ws_start_gradient(eng._h, xi.ptr, df.ptr, out.ptr))
+    return out
+
+
 def _F(eng, xi):
     return float(np.sum(eng.energy_and_gradient(xi, None)[0]))
 
 
 def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: DeviceBuffer | None = None,
-               kind: int = WS_DENSITY, lams=(0.5,)) -> dict:
+               kind: int = WS_DENSITY, lams=()) -> dict:
     """xi <- the network's starting density for the engine's vext.  With a
     scratch field `fallback`, F is evaluated at the prediction (lam = 1), at
     the cold start (lam = 0) and at the density mixtures `lams`, and the lowest
-    is kept: a poor prediction costs len(lams) + 2 F evaluations and never a
+    is kept: the selection costs len(lams) + 2 F evaluations and never gives a
     worse start than the cold one."""
     assert net.channels[-1] == 2 * eng.np_ and net.channels[0] == 2 * eng.np_
-    f = features(eng)
+    f = features(eng, net.input_scale)
     pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
     net.forward(eng.n, f, pred)
     f.free()
@@ -260,22 +305,24 @@ def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: Devic
 
 
 def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4, precision: str = "fp32",
-                      maxeval: int = 600, log=None):
+                      maxeval: int = 600, log=None, nscale: int = 16):
     """Solve `nsolutes` seeded solutes of workload `w` (grid, box, mmax, c(k))
-    with the plain method from the reference cold start; returns the engine and
-    per-solute (features, density targets, evaluations, F) with device-resident fields."""
+    with the plain method from the reference cold start; returns the engine, the
+    input scale (from the first `nscale` solutes) and per-solute (features,
+    density targets, evaluations, F) with device-resident fields."""
     from . import workloads
     from .minimize import minimize
     eng = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, precision)
     eng.set_ck(ck_table)
     xi = eng.alloc_field()
     g = eng.alloc_field()
+    scale = input_scale(eng, [workloads.make_solute(w, seed + i) for i in range(min(nscale, nsolutes))], w.solvent)
     out = []
     for i in range(nsolutes):
         t0 = time.perf_counter()
         sol = workloads.make_solute(w, seed + i)
         eng.compute_vext(sol, w.solvent)
-        feat = features(eng)
+        feat = features(eng, scale)
         cold_start(eng, xi)
         res = minimize(eng, xi, g=g, m=5, gtol=gtol, maxeval=maxeval, fnoise=1e-6)
         tgt = density(eng, xi)
@@ -283,7 +330,7 @@ def make_training_set(w, nsolutes: int, seed: int, ck_table, gtol: float = 1e-4,
         if log:
             log(f"  solute {i}: {len(sol.q)} atoms, F={res.F:.3f}, evaluations={res.evaluations}, "
                 f"status={res.status}, {time.perf_counter() - t0:.2f}s")
-    return eng, out
+    return eng, scale, out
 
 
 def train(net: WarmStartNet, n, data, epochs: int, lr: float = 1e-3, seed: int = 0, log=None):
@@ -302,3 +349,57 @@ def train(net: WarmStartNet, n, data, epochs: int, lr: float = 1e-3, seed: int =
         if log:
             log(f"  epoch {ep}: loss {hist[-1]:.5f}")
     return hist
+
+
+def train_energy(net: WarmStartNet, eng: Engine, solutes, solvent, feats, epochs: int, lr: float = 1e-4,
+                 seed: int = 0, log=None):
+    """Train on the free energy of the start itself: for each solute, the loss
+    is F at xi = guess(net(features)) and its gradient flows back through the
+    guess (mdft_ws_start_gradient) into the network (mdft_wsnet_backward).
+    feats[i] are the (scaled) features of solutes[i].  Returns the per-epoch
+    mean of F(start) - F(cold start)."""
+    rng = np.random.default_rng(seed)
+    C_out = net.channels[-1]
+    pred = DeviceBuffer(eng, 4 * C_out * eng.N)
+    gpred = DeviceBuffer(eng, 4 * C_out * eng.N)
+    xi, df = eng.alloc_field(), eng.alloc_field()
+    fcold = []
+    for sol in solutes:
+        eng.compute_vext(sol, solvent)
+        cold_start(eng, xi)
+        fcold.append(_F(eng, xi))
+    hist = []
+    for ep in range(epochs):
+        tot = 0.0
+        for i in rng.permutation(len(solutes)):
+            eng.compute_vext(solutes[i], solvent)
+            net.forward(eng.n, feats[i], pred)
+            guess(eng, pred, xi, WS_DENSITY)
+            parts, _ = eng.energy_and_gradient(xi, df)
+            tot += float(np.sum(parts)) - fcold[i]
+            start_gradient(eng, xi, df, gpred)
+            net.backward(eng.n, feats[i], gpred)
+            net.adam(lr)
+        hist.append(tot / len(solutes))
+        if log:
+            log(f"  energy epoch {ep}: mean F(start) - F(cold) {hist[-1]:.3f} kJ/mol, lr {lr:.1e}")
+    for b in (pred, gpred, xi, df):
+        b.free()
+    return hist
+
+
+def start_gap(net: WarmStartNet, eng: Engine, solutes, solvent, feats) -> list:
+    """F(start) - F(cold start) (kJ/mol) of the network's start for each solute."""
+    pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
+    xi = eng.alloc_field()
+    out = []
+    for sol, f in zip(solutes, feats):
+        eng.compute_vext(sol, solvent)
+        cold_start(eng, xi)
+        fc = _F(eng, xi)
+        net.forward(eng.n, f, pred)
+        guess(eng, pred, xi, WS_DENSITY)
+        out.append(_F(eng, xi) - fc)
+    pred.free()
+    xi.free()
+    return out

@@ -177,6 +177,37 @@ def test_angular_projector(oracle_mod, mmax):
     assert abs(const[0] - 2.5) < 1e-13 and np.all(np.abs(const[1:]) < 1e-13)
 
 
+@pytest.mark.parametrize("mmax", [1, 3, 5])
+def test_proj2angl_adjoint(oracle_mod, mmax):
+    """The transpose of proj2angl (a real-linear map from the 2 np real and
+    imaginary parts to the no orientations) is D angl2proj W^-1, W the
+    quadrature weights and D = 8 pi^2 / mrso on mu = 0 channels, twice that on
+    mu > 0 channels, 0 on the imaginary channel of (m, 0, 0) -- the factors the
+    warm start's training gradient (mdft_ws_start_gradient) applies.  Checked
+    against the transpose of the matrix built column by column."""
+    from mdft.grid import angular_grid
+    o = _ctx(oracle_mod, mmax)
+    w = angular_grid(mmax, 2)["w"]
+    A = np.zeros((o.no, 2 * o.np_))
+    for k in range(2 * o.np_):
+        pv = np.zeros(o.np_, complex)
+        pv[k // 2] = 1.0 if k % 2 == 0 else 1j
+        A[:, k] = o.proj2angl(pv)
+    B = np.zeros((2 * o.np_, o.no))
+    for i in range(o.no):
+        e = np.zeros(o.no)
+        e[i] = 1.0
+        P = o.angl2proj(e)
+        B[0::2, i], B[1::2, i] = P.real, P.imag
+    s = 8.0 * np.pi ** 2 / 2
+    D = np.zeros(2 * o.np_)
+    for ip in range(o.np_):
+        f = 2 * s if o.pmu[ip] > 0 else s
+        D[2 * ip] = f
+        D[2 * ip + 1] = 0.0 if (o.pmup[ip] == 0 and o.pmu[ip] == 0) else f
+    assert np.abs(A.T - D[:, None] * B / w[None, :]).max() < 1e-12 * np.abs(A).max()
+
+
 # ---------------------------------------------------------------- excess functional
 def _setup(oracle_mod, mmax, n=8, length=10.0, seed=4, self_term=1.0):
     o = oracle_mod.Oracle((n, n, n), (length,) * 3, mmax, 2, 2.4777, 0.0333)
