@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="solute20_128")
     ap.add_argument("--gtol", type=float, default=1e-6)
-    ap.add_argument("--memory", type=int, default=5, help="L-BFGS memory m (both paths)")
+    ap.add_argument("--memory", type=int, default=1, help="L-BFGS memory m (both paths; 1 is the reference's)")
     ap.add_argument("--maxeval", type=int, default=1500)
     ap.add_argument("--mode", choices=["converge", "eval", "copy"], default="converge",
                     help="copy: a plain device copy of 1 GiB (z = x through mdft_vec_lincomb), the "

@@ -329,15 +329,27 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     __shared__ T cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], a2p[NP * NTH], fmv[NP];
     __shared__ int pmup[NP], pmu2[NP];
     const int tid = threadIdx.x;
+    const size_t r0 = (size_t)blockIdx.x * PT;
+    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+    // the tile's FP64 input is loaded into registers before the tables are staged,
+    // so its HBM latency overlaps the staging
+    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
+    const double *src = xi + r0 * NO;
+    double xr[NIT];
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+        const int i = tid + q * kThreads;
+        xr[q] = (i < npts * NO) ? src[i] : 0.0;
+    }
     for (int i = tid; i < NPHI; i += kThreads) { cph[i] = (T)t.cph[i]; sph[i] = (T)t.sph[i]; }
     for (int i = tid; i < NPSI; i += kThreads) { cps[i] = (T)t.cps[i]; sps[i] = (T)t.sps[i]; }
     for (int i = tid; i < NP * NTH; i += kThreads) a2p[i] = (T)t.a2p[i];
     for (int i = tid; i < NP; i += kThreads) { fmv[i] = (T)t.fm[i]; pmup[i] = t.pmup[i]; pmu2[i] = t.pmu[i] / 2; }
-    const size_t r0 = (size_t)blockIdx.x * PT;
-    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
-    const double *src = xi + r0 * NO;
-    for (int i = tid; i < npts * NO; i += kThreads) {
-        const double x = src[i];
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+        const int i = tid + q * kThreads;
+        if (i >= npts * NO) break;
+        const double x = xr[q];
         if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
         else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
         else if (feat == 2) o[i] = (T)(x * x - 1.0);
@@ -394,7 +406,8 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
 }
 
 template <int L, int PT, typename T>
-__global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
+// 7 waves per SIMD (the LDS limit at mmax = 5, FP32): caps the registers the prefetch below adds
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode) {
@@ -408,13 +421,28 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
     __shared__ double wq[NO];
     __shared__ int pidx[(L + 1) * W * H1];
     const int tid = threadIdx.x;
+    const size_t r0 = (size_t)blockIdx.x * PT;
+    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+    // the tile's FP64 xi and vext go into registers first: their HBM latency then
+    // overlaps the angular synthesis below instead of following it
+    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
+    double xr[NIT], vr[NIT];
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+        const int i = tid + q * kThreads;
+        xr[q] = 0.0;
+        vr[q] = 0.0;
+        if (i < npts * NO) {
+            const size_t g = r0 * NO + i;
+            if (mode < 3) xr[q] = xi[g];
+            if (mode == 0 || mode >= 3) vr[q] = vext[g];
+        }
+    }
     for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
     for (int i = tid; i < NPHI; i += kThreads) { cph[i] = (T)t.cph[i]; sph[i] = (T)t.sph[i]; }
     for (int i = tid; i < NPSI; i += kThreads) { cps[i] = (T)t.cps[i]; sps[i] = (T)t.sps[i]; }
     for (int i = tid; i < NP * NTH; i += kThreads) p2a[i] = (T)t.p2a[i];
     for (int i = tid; i < NO; i += kThreads) wq[i] = t.w[i];
-    const size_t r0 = (size_t)blockIdx.x * PT;
-    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
     const V *src = proj + r0 * NP;
     for (int i = tid; i < npts * NP; i += kThreads) Gm[i] = src[i];
     __syncthreads();
@@ -458,7 +486,10 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
     __syncthreads();
     double fid = 0.0, fext = 0.0, fexc = 0.0;
     const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
-    for (int i = tid; i < npts * NO; i += kThreads) {
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+        const int i = tid + q * kThreads;
+        if (i >= npts * NO) break;
         const int pt = i / NO, io = i - pt * NO;
         const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
         const V *hr = h + (pt * NTH * NPHI + line) * H1;
@@ -472,12 +503,12 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
         const double v = (double)vt;
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
-        const size_t g = (r0 + pt) * (size_t)NO + io;
-        const double x = (mode >= 3) ? 0.0 : xi[g];
+        const size_t g = r0 * NO + i;
+        const double x = xr[q];
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
         if (mode == 0) {
-            const double ve = vext[g];
+            const double ve = vr[q];
             double gid = 0.0;
             if (rho > DBL_EPSILON) {
                 const double lg = log(rho / rho0);
@@ -493,10 +524,10 @@ __global__ __launch_bounds__(kThreads) void final_t(const typename C2<T>::t *__r
         } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
             if (df) df[g] = -(k.prefactor / kT) * v;
         } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
-            df[g] = exp(0.5 * fmin(fmax(-vext[g] / kT + v, -200.0), 7.0));
+            df[g] = exp(0.5 * fmin(fmax(-vr[q] / kT + v, -200.0), 7.0));
         } else {  // mode 4: proj holds the projections of rho/rho0 - 1;
                   // df = sqrt(max(1 + v, floor)), 0 where the cold start excludes the solvent
-            df[g] = (vext[g] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
+            df[g] = (vr[q] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
         }
     }
     block_reduce3(fid, fext, fexc, partials);
