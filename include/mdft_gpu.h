@@ -156,6 +156,24 @@ int mdft_vec_multi_dot(mdft_ctx *ctx, size_t n, int k, const double *const *vecs
 /* One pass: z = sum_j coef[j] vecs[j], k <= 16 (z may be one of vecs). */
 int mdft_vec_combine(mdft_ctx *ctx, size_t n, int k, const double *coef, const double *const *vecs, double *z);
 
+/* ---- restart (density.bin) support, all device pointers ----
+ * angl2proj / proj2angl of src/module_orientation_projection_transform.f90:159-199 and
+ * :202-243 for npoints points (any count: a restart file's grid may differ from the
+ * context's): d_f [npoints][no] real, d_proj [npoints][np] complex (interleaved FP64),
+ * the context's orientation tables.  The reader of src/module_density.f90:61-261 rebuilds
+ * xi with proj2angl from the projections the writer of src/module_postprocessing.f90:46-94
+ * stored with angl2proj.  Molecular symmetry order 2. */
+int mdft_angl2proj(mdft_ctx *ctx, const double *d_f, size_t npoints, double *d_proj);
+int mdft_proj2angl(mdft_ctx *ctx, const double *d_proj, size_t npoints, double *d_f);
+/* Trilinear regridding of src/module_density.f90:131-133,171-226: d_prev [n_prev][no]
+ * (x fastest) on a grid of n_prev[3] nodes with spacing d_spacing[3], centred in the
+ * context's box, onto d_xi [N][no]; points of the context's grid outside the previous
+ * grid's nodes get `outside` (the reference leaves them at their initial xi = 1).  The
+ * cell that writes a point is the reference's: the last of its loop order whose index
+ * range ceiling(x0/d + 1) .. floor(x1/d + 1) contains it. */
+int mdft_regrid_trilinear(mdft_ctx *ctx, const double *d_prev, const int *n_prev, const double *d_spacing,
+                          double outside, double *d_xi);
+
 /* Device memory owned by the caller, for device-resident fields
  * (MDFT_DEVICE_PTRS) and the device L-BFGS vectors. */
 int mdft_device_alloc(mdft_ctx *ctx, size_t bytes, void **dptr);

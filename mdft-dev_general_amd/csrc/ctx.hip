@@ -994,6 +994,71 @@ int mdft_ws_start_gradient(mdft_ctx *c, const double *d_xi, const double *d_df, 
     return MDFT_OK;
 }
 
+// ---- restart (src/module_density.f90:61-261, src/module_postprocessing.f90:46-94)
+
+int mdft_angl2proj(mdft_ctx *c, const double *d_f, size_t npoints, double *d_proj) {
+    if (!c || !d_f || !d_proj || npoints == 0) return MDFT_EINVAL;
+    if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the restart transforms need molecular symmetry order 2");
+    HIPCHK(c, launch_transform_f64(c, d_f, reinterpret_cast<double2 *>(d_proj), npoints, true));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+int mdft_proj2angl(mdft_ctx *c, const double *d_proj, size_t npoints, double *d_f) {
+    if (!c || !d_f || !d_proj || npoints == 0) return MDFT_EINVAL;
+    if (c->d.mrso != 2) return fail(c, MDFT_EINVAL, "the restart transforms need molecular symmetry order 2");
+    HIPCHK(c, launch_transform_f64(c, d_f, reinterpret_cast<double2 *>(const_cast<double *>(d_proj)), npoints, false));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDFT_OK;
+}
+
+int mdft_regrid_trilinear(mdft_ctx *c, const double *d_prev, const int *n_prev, const double *d_spacing,
+                          double outside, double *d_xi) {
+    if (!c || !d_prev || !n_prev || !d_spacing || !d_xi) return MDFT_EINVAL;
+    for (int a = 0; a < 3; a++)
+        if (n_prev[a] < 2 || !(d_spacing[a] > 0.0)) return fail(c, MDFT_EINVAL, "previous grid needs >= 2 nodes and a positive spacing per axis");
+    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+    const double len[3] = {c->d.lx, c->d.ly, c->d.lz};
+    std::vector<int> cell;
+    std::vector<double> wgt;
+    for (int a = 0; a < 3; a++) {
+        // the previous grid is centred in the box (:131-133); a new index i (1-based) is written
+        // by every previous cell cp with ceiling(x0/d + 1) <= i <= floor(x1/d + 1) (:191-197),
+        // the last one in the loop order winning -- the largest such cell
+        const double d = len[a] / n[a], dp = d_spacing[a];
+        const double off = (len[a] - dp * n_prev[a]) / 2;
+        for (int i = 1; i <= n[a]; i++) {
+            const double x = (i - 1) * d;
+            int best = -1;
+            double w = 0.0;
+            const int est = (int)floor((x - off) / dp) + 1;
+            for (int cp = std::min(n_prev[a] - 1, est + 1); cp >= std::max(1, est - 1); cp--) {
+                const double x0 = (cp - 1) * dp + off, x1 = cp * dp + off;
+                const int lo = std::max(1, (int)ceil(x0 / d + 1)), hi = std::min(n[a], (int)floor(x1 / d + 1));
+                if (i >= lo && i <= hi) {
+                    best = cp - 1;
+                    w = (x - x0) / (x1 - x0);
+                    break;
+                }
+            }
+            cell.push_back(best);
+            wgt.push_back(w);
+        }
+    }
+    int *d_cell = nullptr;
+    double *d_wgt = nullptr;
+    HIPCHK(c, hipMalloc(&d_cell, sizeof(int) * cell.size()));
+    hipError_t e = hipMalloc(&d_wgt, sizeof(double) * wgt.size());
+    if (e == hipSuccess) e = hipMemcpy(d_cell, cell.data(), sizeof(int) * cell.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_wgt, wgt.data(), sizeof(double) * wgt.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_regrid(c, d_prev, n_prev[0], n_prev[1], d_cell, d_wgt, outside, d_xi);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_cell);
+    (void)hipFree(d_wgt);
+    if (e != hipSuccess) return fail(c, MDFT_EHIP, std::string("regrid: ") + hipGetErrorString(e));
+    return MDFT_OK;
+}
+
 int mdft_ws_mix(mdft_ctx *c, double lam, const double *d_xa, const double *d_xb, double *d_out) {
     if (!c || !d_xa || !d_xb || !d_out) return MDFT_EINVAL;
     HIPCHK(c, launch_density_mix(c, c->N * (size_t)c->no, lam, d_xa, d_xb, d_out));

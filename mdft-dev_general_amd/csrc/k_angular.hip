@@ -138,6 +138,7 @@ struct FinalConsts {
 // mode 4: proj holds the projections of rho/rho0 - 1: df = 0 where the reference cold start
 // excludes the solvent (beta vext >= -ln(DBL_EPSILON) - 0.05), else
 // sqrt(max(1 + proj2angl(proj), MDFT_WS_DENSITY_FLOOR))
+// mode 5 (specialised kernels only): df = proj2angl(proj)
 template <int PT>
 __global__ __launch_bounds__(kThreads) void final_f64(const double2 *__restrict__ proj, const double *__restrict__ xi,
                                                      const double *__restrict__ vext, double *__restrict__ df,
@@ -314,8 +315,8 @@ struct Dims {
 // warm start's density target; 0 in the bulk, -1 where the solvent is excluded); feat = 3:
 // o = k0 aux / (xi w(io)) where xi^2 > MDFT_WS_DENSITY_FLOOR, else 0, with aux = dF at the
 // density guess xi and k0 = dv / 2: dF/dr / w for r = rho/rho0, whose angl2proj is the
-// adjoint of proj2angl up to a per-projection factor (the warm start's training gradient).
-// k0 is rho0 for feat 0.
+// adjoint of proj2angl up to a per-projection factor (the warm start's training gradient);
+// feat = 4: o = xi (the plain transform). k0 is rho0 for feat 0.
 template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
                                                        size_t N, AngTables t, double k0, int feat, double beta,
@@ -353,6 +354,7 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
         if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
         else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
         else if (feat == 2) o[i] = (T)(x * x - 1.0);
+        else if (feat == 4) o[i] = (T)x;
         else o[i] = (x * x > MDFT_WS_DENSITY_FLOOR * (1.0 + 1e-12)) ? (T)(k0 * aux[r0 * NO + i] / (x * t.w[i % NO]))
                                                                    : (T)0;
     }
@@ -435,7 +437,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         if (i < npts * NO) {
             const size_t g = r0 * NO + i;
             if (mode < 3) xr[q] = xi[g];
-            if (mode == 0 || mode >= 3) vr[q] = vext[g];
+            if (mode == 0 || mode == 3 || mode == 4) vr[q] = vext[g];
         }
     }
     for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
@@ -525,9 +527,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
             if (df) df[g] = -(k.prefactor / kT) * v;
         } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
             df[g] = exp(0.5 * fmin(fmax(-vr[q] / kT + v, -200.0), 7.0));
-        } else {  // mode 4: proj holds the projections of rho/rho0 - 1;
-                  // df = sqrt(max(1 + v, floor)), 0 where the cold start excludes the solvent
+        } else if (mode == 4) {  // proj holds the projections of rho/rho0 - 1;
+                                 // df = sqrt(max(1 + v, floor)), 0 where the cold start excludes the solvent
             df[g] = (vr[q] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
+        } else {  // mode 5: df = proj2angl(proj), the plain transform
+            df[g] = v;
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -556,13 +560,15 @@ hipError_t ensure_partials(mdft_ctx *c, size_t nblocks) {
 }  // namespace
 
 template <int PT, typename T>
-bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int feat = 0, const double *aux = nullptr) {
-    const size_t nb = (c->N + PT - 1) / PT;
+bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int feat = 0, const double *aux = nullptr,
+                  size_t npts = 0) {
+    if (!npts) npts = c->N;
+    const size_t nb = (npts + PT - 1) / PT;
     const double beta = 1.0 / c->d.kT;
     const double k0 = (feat == 3) ? 0.5 * c->dv : c->rho0;
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, k0, feat, beta, aux); return true;
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, npts, c->ang, k0, feat, beta, aux); return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;
@@ -570,11 +576,13 @@ bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int fe
 }
 
 template <int PT, typename T>
-bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi, double *df, int mode, FinalConsts k) {
-    const size_t nb = (c->N + PT - 1) / PT;
+bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi, double *df, int mode, FinalConsts k,
+                    size_t npts = 0) {
+    if (!npts) npts = c->N;
+    const size_t nb = (npts + PT - 1) / PT;
     switch (c->L) {
 #define FIN_CASE(LL) \
-    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, c->N, c->ang, k, mode); return true;
+    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode); return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
         default: return false;
@@ -686,4 +694,16 @@ hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat, const
                   ? launch_a2p_t<kPTt, double>(c, src, static_cast<double2 *>(c->d_proj), feat, aux)
                   : launch_a2p_t<kPT32, float>(c, src, static_cast<float2 *>(c->d_proj), feat, aux);
     return ok ? hipGetLastError() : hipErrorNotSupported;
+}
+
+// The plain FP64 transforms of npts points (any count, not only the context's grid):
+// proj [npts][np] = angl2proj(f [npts][no]) and f = proj2angl(proj); mrso = 2 kernels.
+hipError_t launch_transform_f64(mdft_ctx *c, const double *f, double2 *proj, size_t npts, bool to_proj) {
+    if (c->d.mrso != 2) return hipErrorNotSupported;
+    if (to_proj) return launch_a2p_t<kPTt, double>(c, f, proj, 4, nullptr, npts) ? hipGetLastError() : hipErrorNotSupported;
+    hipError_t e = ensure_partials(c, (npts + kPTt - 1) / kPTt);
+    if (e != hipSuccess) return e;
+    return launch_final_t<kPTt, double>(c, proj, nullptr, const_cast<double *>(f), 5, final_consts(c), npts)
+               ? hipGetLastError()
+               : hipErrorNotSupported;
 }

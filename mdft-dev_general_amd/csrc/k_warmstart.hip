@@ -63,6 +63,42 @@ __global__ __launch_bounds__(kThreads) void cold_start(const double *__restrict_
         xi[g] = (beta * vext[g] >= vmax) ? 0.0 : 1.0;
 }
 
+// Trilinear regridding of a restart field (src/module_density.f90:171-226): new point
+// (ix, iy, iz) takes the value interpolated in the previous grid's cell (cx, cy, cz) with
+// weights (wx, wy, wz), per axis precomputed on the host (cell -1: the point lies outside
+// the previous grid and gets `outside`).  Orientation fastest: one thread per (point, io).
+struct RegridArgs {
+    int nx, ny, nz, no, pnx, pny;
+    double outside;
+    const int *cell;      // [nx + ny + nz] 0-based cell of each new index along x, y, z, or -1
+    const double *wgt;    // [nx + ny + nz] the fractional position inside that cell
+};
+
+__global__ __launch_bounds__(kThreads) void regrid_trilinear(const double *__restrict__ prev, double *__restrict__ xi,
+                                                            RegridArgs a) {
+    const size_t total = (size_t)a.nx * a.ny * a.nz * a.no;
+    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads) {
+        const int io = (int)(g % a.no);
+        const size_t r = g / a.no;
+        const int ix = (int)(r % a.nx), iy = (int)((r / a.nx) % a.ny), iz = (int)(r / ((size_t)a.nx * a.ny));
+        const int cx = a.cell[ix], cy = a.cell[a.nx + iy], cz = a.cell[a.nx + a.ny + iz];
+        if (cx < 0 || cy < 0 || cz < 0) {
+            xi[g] = a.outside;
+            continue;
+        }
+        const double xd = a.wgt[ix], yd = a.wgt[a.nx + iy], zd = a.wgt[a.nx + a.ny + iz];
+        auto p = [&](int dx, int dy, int dz) {
+            return prev[(((size_t)(cz + dz) * a.pny + (cy + dy)) * a.pnx + (cx + dx)) * a.no + io];
+        };
+        const double x00 = p(0, 0, 0) * (1 - xd) + p(1, 0, 0) * xd;
+        const double x01 = p(0, 0, 1) * (1 - xd) + p(1, 0, 1) * xd;
+        const double x10 = p(0, 1, 0) * (1 - xd) + p(1, 1, 0) * xd;
+        const double x11 = p(0, 1, 1) * (1 - xd) + p(1, 1, 1) * xd;
+        const double x0 = x00 * (1 - yd) + x10 * yd, x1 = x01 * (1 - yd) + x11 * yd;
+        xi[g] = x0 * (1 - zd) + x1 * zd;
+    }
+}
+
 // out = sqrt((1 - lam) xa^2 + lam xb^2): the density mixture (1 - lam) rho_a + lam rho_b in xi
 __global__ __launch_bounds__(kThreads) void density_mix(size_t total, double lam, const double *__restrict__ xa,
                                                        const double *__restrict__ xb, double *__restrict__ out) {
@@ -112,5 +148,15 @@ hipError_t launch_cold_start(mdft_ctx *c, double *xi) {
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(cold_start, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, c->d_vext, xi, total,
                        1.0 / c->d.kT);
+    return hipGetLastError();
+}
+
+hipError_t launch_regrid(mdft_ctx *c, const double *prev, int pnx, int pny, const int *d_cell, const double *d_wgt,
+                         double outside, double *xi) {
+    RegridArgs a{c->d.nx, c->d.ny, c->d.nz, c->no, pnx, pny, outside, d_cell, d_wgt};
+    const size_t total = c->N * (size_t)c->no;
+    size_t nb = (total + kThreads - 1) / kThreads;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(regrid_trilinear, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, prev, xi, a);
     return hipGetLastError();
 }

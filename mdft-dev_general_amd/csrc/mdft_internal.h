@@ -120,6 +120,9 @@ hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
 hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat, const double *aux = nullptr);
+hipError_t launch_transform_f64(mdft_ctx *c, const double *f, double2 *proj, size_t npts, bool to_proj);
+hipError_t launch_regrid(mdft_ctx *c, const double *prev, int pnx, int pny, const int *d_cell, const double *d_wgt,
+                         double outside, double *xi);
 hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale, const float *chscale = nullptr);
 hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in);
 hipError_t launch_cold_start(mdft_ctx *c, double *xi);
