@@ -125,6 +125,19 @@ int mdft_phase_times(const mdft_ctx *ctx, double *ms, int n);
  * sites[nsite][6] = x, y, z (A, molecular frame), q, sigma, epsilon. */
 int mdft_compute_vext(mdft_ctx *ctx, int natom, const double *atoms, int nsite, const double *sites,
                       double rcut);
+/* Electrostatics of mdft_compute_vext_ex. */
+enum { MDFT_ELEC_POISSON = 0, MDFT_ELEC_DIRECT = 1, MDFT_ELEC_NONE = 2 };
+/* The same with a choice of electrostatics:
+ * MDFT_ELEC_POISSON -- the reference's default (direct_sum = F): the fast Poisson
+ *   solver of src/module_fastpoissonsolver.f90:121-288 on the device -- cloud-in-cell
+ *   solute charges (src/module_solute.f90:193-232), V(k) = 4 pi rho(k) / k^2, and per
+ *   orientation the c2r of V(k) conj(sigma_io(k)) with the smeared solvent charge
+ *   density sigma_io (src/module_solvent.f90:367-426), times qfact / N;
+ * MDFT_ELEC_DIRECT -- minimum-image Coulomb sum (mdft_compute_vext);
+ * MDFT_ELEC_NONE -- Lennard-Jones only.
+ * The clip at qfact/2 follows the sum of the terms (src/module_vext.f90:600-669). */
+int mdft_compute_vext_ex(mdft_ctx *ctx, int natom, const double *atoms, int nsite, const double *sites,
+                         double rcut, int electrostatics);
 /* Copies the context's vext(no,nx,ny,nz) out (host or, with MDFT_DEVICE_PTRS, device). */
 int mdft_get_vext(mdft_ctx *ctx, double *out, int flags);
 

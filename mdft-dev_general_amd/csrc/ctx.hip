@@ -708,8 +708,15 @@ int mdft_ideal_external(mdft_ctx *c, const double *xi, double *fid, double *fext
 }
 
 int mdft_compute_vext(mdft_ctx *c, int natom, const double *atoms, int nsite, const double *sites, double rcut) {
+    return mdft_compute_vext_ex(c, natom, atoms, nsite, sites, rcut, MDFT_ELEC_DIRECT);
+}
+
+int mdft_compute_vext_ex(mdft_ctx *c, int natom, const double *atoms, int nsite, const double *sites, double rcut,
+                         int electrostatics) {
     if (!c || natom < 0 || nsite < 1 || (natom > 0 && !atoms) || !sites || !(rcut > 0))
         return c ? fail(c, MDFT_EINVAL, "bad vext arguments") : MDFT_EINVAL;
+    if (electrostatics != MDFT_ELEC_POISSON && electrostatics != MDFT_ELEC_DIRECT && electrostatics != MDFT_ELEC_NONE)
+        return fail(c, MDFT_EINVAL, "unknown electrostatics");
     // physical constants of src/module_constants.f90:54-57
     const double pi = acos(-1.0);
     const double eps0 = 8.854187817e-12, qunit = 1.602176565e-19, navo = 6.02214129e23;
@@ -740,7 +747,15 @@ int mdft_compute_vext(mdft_ctx *c, int natom, const double *atoms, int nsite, co
     HIPCHK(c, upload(c, &d_off, off.data(), off.size()));
     HIPCHK(c, upload(c, &d_sp, sp.data(), sp.size()));
     HIPCHK(c, upload(c, &d_at, at.data(), at.size()));
-    hipError_t e = launch_vext(c, natom, d_at, nsite, d_sp, d_off, rcut, qfact, qfact / 2.0);
+    // the clip at qfact/2 (src/module_vext.f90:638-640) follows the sum of all terms
+    const bool direct = electrostatics == MDFT_ELEC_DIRECT, poisson = electrostatics == MDFT_ELEC_POISSON;
+    hipError_t e = launch_vext(c, natom, d_at, nsite, d_sp, d_off, rcut, qfact, poisson ? HUGE_VAL : qfact / 2.0, direct);
+    if (e == hipSuccess && poisson) {
+        std::vector<double> sq(nsite);
+        for (int s = 0; s < nsite; s++) sq[s] = sites[6 * s + 3];
+        e = launch_poisson_vext(c, natom, at.data(), nsite, sq.data(), d_off, qfact);
+        if (e == hipSuccess) e = launch_vext_clip(c, qfact / 2.0);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d_off);
     (void)hipFree(d_sp);

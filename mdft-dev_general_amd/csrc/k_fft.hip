@@ -278,8 +278,8 @@ bool launch_axis_c(int n, dim3 grid, hipStream_t st, V *proj, int np, const Line
 }
 
 template <typename V>
-hipError_t launch_fft3d(mdft_ctx *c, V *proj, int sign) {
-    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, np = c->np;
+hipError_t launch_fft3d(mdft_ctx *c, V *proj, int sign, int batch) {
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, np = batch > 0 ? batch : c->np;
     const double scale_last = (sign < 0) ? 1.0 / (double)((long long)nx * ny * nz) : 1.0;
     LineGeom geo[3] = {
         {nx, 1, ny, (long long)nx, (long long)nx * ny},
@@ -325,6 +325,10 @@ int fft_make_plan(int n, int *radix) {
     return (n == 1) ? k : -1;
 }
 
-hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign) { return launch_fft3d<double2>(c, proj, sign); }
+hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign, int batch) {
+    return launch_fft3d<double2>(c, proj, sign, batch);
+}
 
-hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign) { return launch_fft3d<float2>(c, proj, sign); }
+hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign, int batch) {
+    return launch_fft3d<float2>(c, proj, sign, batch);
+}

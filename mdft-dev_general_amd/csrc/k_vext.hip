@@ -6,9 +6,9 @@
 // solute's field taken at the minimum image, and the clip at qfact/2 of
 // src/module_vext.f90:638-640.  The reference computes the electrostatic part
 // with a periodic Poisson solver (src/module_fastpoissonsolver.f90:121-288);
-// the minimum-image sum used here agrees with it for neutral solutes far
-// from the box faces and is the input both the oracle and the device path
-// receive.
+// the minimum-image sum here is the reference's direct_sum alternative in
+// spirit; the Poisson path itself is in k_poisson.hip (this kernel then runs
+// with the Coulomb term and the clip off, and the clip follows the sum).
 //
 // One thread per (grid point, orientation); solute atoms are staged through
 // LDS in tiles shared by the workgroup.
@@ -22,6 +22,7 @@ constexpr int kTile = 256;
 struct VextArgs {
     int nx, ny, nz, no, nsite, natom;
     double dx, dy, dz, lx, ly, lz, rc2, qfact, clip;
+    int coulomb;  // 1: minimum-image Coulomb of the site charges in the solute's field
     const double *site_off;  // [no][nsite][3]
     const double *site_p;    // [nsite][3]: q, sigma, epsilon
     const double *atoms;     // [natom][6]: x, y, z, q, sigma, epsilon
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(kThreads) void vext_kernel(double *__restrict__ vex
                     acc += 4.0 * sqrt(e) * (sr6 * sr6 - sr6);
                 }
                 const double qq = qs * at[k][3];
-                if (qq != 0.0) acc += a.qfact * qq / sqrt(r2);
+                if (a.coulomb && qq != 0.0) acc += a.qfact * qq / sqrt(r2);
             }
         }
     }
@@ -71,8 +72,9 @@ __global__ __launch_bounds__(kThreads) void vext_kernel(double *__restrict__ vex
 }  // namespace
 
 hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,
-                       const double *d_site_off, double rcut, double qfact, double clip) {
+                       const double *d_site_off, double rcut, double qfact, double clip, bool coulomb) {
     VextArgs a;
+    a.coulomb = coulomb ? 1 : 0;
     a.nx = c->d.nx;
     a.ny = c->d.ny;
     a.nz = c->d.nz;

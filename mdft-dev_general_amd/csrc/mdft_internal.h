@@ -115,8 +115,8 @@ hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, d
 hipError_t launch_oz32(mdft_ctx *c, float2 *proj);
 int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &lanerow);
 hipError_t launch_reduce3(mdft_ctx *c, int nblocks);
-hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign);
-hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign);
+hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign, int batch = 0);  // batch 0: np
+hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign, int batch = 0);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
 hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat, const double *aux = nullptr);
@@ -140,4 +140,7 @@ hipError_t launch_density_integrals(mdft_ctx *c, const double *xi, double cav, i
 hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out);
 hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z);
 hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,
-                       const double *d_site_off, double rcut, double qfact, double clip);
+                       const double *d_site_off, double rcut, double qfact, double clip, bool coulomb = true);
+hipError_t launch_poisson_vext(mdft_ctx *c, int natom, const double *h_atoms, int nsite, const double *h_site_q,
+                               const double *d_site_off, double qfact);
+hipError_t launch_vext_clip(mdft_ctx *c, double clip);

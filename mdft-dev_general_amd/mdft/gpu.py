@@ -29,7 +29,7 @@ MDFT_EXPORTS = (
     "mdft_device_free", "mdft_memcpy_h2d", "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext",
     "mdft_get_vext", "mdft_vec_dot", "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot",
     "mdft_vec_combine", "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj",
-    "mdft_proj2angl", "mdft_regrid_trilinear",
+    "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex",
 )
 
 
@@ -81,6 +81,7 @@ def load_library(path: str = LIBPATH):
     lib.mdft_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
     lib.mdft_synchronize.argtypes = [vp]
     lib.mdft_compute_vext.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double]
+    lib.mdft_compute_vext_ex.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double, C.c_int]
     lib.mdft_get_vext.argtypes = [vp, vp, C.c_int]
     lib.mdft_vec_dot.argtypes = [vp, C.c_size_t, vp, vp, _dp]
     lib.mdft_vec_axpy_dot.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp, _dp]
@@ -97,7 +98,8 @@ def load_library(path: str = LIBPATH):
                  "mdft_phase_times", "mdft_device_alloc", "mdft_device_free", "mdft_memcpy_h2d",
                  "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext", "mdft_get_vext", "mdft_vec_dot",
                  "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine",
-                 "mdft_pressure_correction", "mdft_lj_tail_correction"):
+                 "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj", "mdft_proj2angl",
+                 "mdft_regrid_trilinear", "mdft_compute_vext_ex"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib
@@ -199,12 +201,17 @@ class Engine:
                                          c.shape[0], float(table.dq), c.view(np.float64).ctypes.data_as(_dp)))
         self._refresh()
 
-    def compute_vext(self, solute, solvent, rcut: float = 10.0):
-        """vext of `solute` (mdft.workloads.Solute) for `solvent` (mdft.constants.Solvent), on the device."""
+    ELECTROSTATICS = {"poisson": 0, "direct": 1, "none": 2}
+
+    def compute_vext(self, solute, solvent, rcut: float = 10.0, electrostatics: str = "direct"):
+        """vext of `solute` (mdft.workloads.Solute) for `solvent` (mdft.constants.Solvent), on the device.
+        electrostatics: "poisson" (the reference's fast Poisson solver, its default),
+        "direct" (minimum-image Coulomb) or "none"."""
         atoms = np.ascontiguousarray(np.column_stack([solute.xyz, solute.q, solute.sig, solute.eps]), dtype=np.float64)
         sites = np.ascontiguousarray([[*s.r, s.q, s.sig, s.eps] for s in solvent.sites], dtype=np.float64)
-        self._check(self.lib.mdft_compute_vext(self._h, len(atoms), atoms.ctypes.data_as(_dp), len(sites),
-                                               sites.ctypes.data_as(_dp), float(rcut)))
+        self._check(self.lib.mdft_compute_vext_ex(self._h, len(atoms), atoms.ctypes.data_as(_dp), len(sites),
+                                                  sites.ctypes.data_as(_dp), float(rcut),
+                                                  self.ELECTROSTATICS[electrostatics]))
 
     def pressure_correction(self, xi) -> dict:
         """src/module_pressure_correction.f90:7-156 for xi (host array or DeviceBuffer)."""

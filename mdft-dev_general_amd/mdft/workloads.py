@@ -135,11 +135,13 @@ def orientation_site_offsets(mmax: int, mrso: int, solvent) -> np.ndarray:
     return out
 
 
-def vext_numpy(w: Workload, solute: Solute, rcut: float = 10.0, clip: float | None = None) -> np.ndarray:
+def vext_numpy(w: Workload, solute: Solute, rcut: float = 10.0, clip: float | None = None,
+               coulomb: bool = True) -> np.ndarray:
     """External potential vext[N, no] (kJ/mol) on the host: Lennard-Jones between
     solute atoms and solvent sites (Lorentz-Berthelot, cut at rcut, minimum
-    image) plus minimum-image Coulomb between charges, clipped at qfact/2
-    (src/module_solvent.f90:49, src/module_vext.f90:638-640).  For small grids."""
+    image) plus (coulomb) minimum-image Coulomb between charges, clipped at
+    qfact/2 (src/module_solvent.f90:49, src/module_vext.f90:638-640; clip=inf:
+    no clip).  For small grids."""
     solv = w.solvent
     offs = orientation_site_offsets(w.mmax, w.mrso, solv)
     no = offs.shape[0]
@@ -164,7 +166,7 @@ def vext_numpy(w: Workload, solute: Solute, rcut: float = 10.0, clip: float | No
                     sg = 0.5 * (site.sig + solute.sig[a])
                     sr6 = (sg * sg / r2) ** 3
                     acc += np.where(r2 < rcut * rcut, 4.0 * e * (sr6 * sr6 - sr6), 0.0)
-                if site.q != 0.0 and solute.q[a] != 0.0:
+                if coulomb and site.q != 0.0 and solute.q[a] != 0.0:
                     acc += QFACT * site.q * solute.q[a] / np.sqrt(r2)
             v[:, io] += acc
     cap = QFACT / 2.0 if clip is None else clip
