@@ -179,7 +179,7 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 (FP64 reductions, ideal term, L-BFGS)",
                   "bf16": "bf16 MFMA angular transforms, f32 FFT/OZ (FP64 reductions, ideal term, L-BFGS)"}[args.precision],
-        "data": "synthetic (seeded 20-atom solute, seeded c(k) with the convex self term, reference cold start)",
+        "data": "synthetic (seeded 20-atom solute, vext with Poisson electrostatics as the reference by default, seeded c(k) with the convex self term, reference cold start)",
         "config": {"workload": w.name, "grid": list(w.n), "box_A": list(w.length), "mmax": w.mmax,
                    "orientations": eng.no, "projections": eng.np_, "natoms": int(len(sol.q))},
         "evaluations_per_s": 1.0 / dt,

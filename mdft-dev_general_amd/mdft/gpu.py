@@ -203,7 +203,7 @@ class Engine:
 
     ELECTROSTATICS = {"poisson": 0, "direct": 1, "none": 2}
 
-    def compute_vext(self, solute, solvent, rcut: float = 10.0, electrostatics: str = "direct"):
+    def compute_vext(self, solute, solvent, rcut: float = 10.0, electrostatics: str = "poisson"):
         """vext of `solute` (mdft.workloads.Solute) for `solvent` (mdft.constants.Solvent), on the device.
         electrostatics: "poisson" (the reference's fast Poisson solver, its default),
         "direct" (minimum-image Coulomb) or "none"."""

@@ -22,7 +22,7 @@ def test_device_vext_matches_host():
     say(f"[vext] {w.name} natom={len(sol.q)}")
     from mdft.gpu import Engine
     eng = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0)
-    eng.compute_vext(sol, w.solvent)
+    eng.compute_vext(sol, w.solvent, electrostatics="direct")
     v = eng.get_vext()
     ref = workloads.vext_numpy(w, sol).ravel()
     assert np.abs(v - ref).max() < 1e-9 * np.abs(ref).max()
