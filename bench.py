@@ -55,6 +55,50 @@ def phase_bytes(N, no, np_, b=16):
     }
 
 
+# per-launch view of the phases: (kernel, launches per evaluation, kernel name in the traffic profile)
+KERNELS = {
+    "angl2proj": ("angl2proj", 1, "angl2proj_t"),
+    "fft": ("fft_axis_c (one axis pass)", 6, "fft_axis_c"),
+    "oz": ("oz", 1, "oz"),
+    "proj2angl_ideal_grad": ("final_t (proj2angl + ideal + gradient)", 1, "final_t"),
+}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_eval_fp32_solute20_128_mmax5.md")
+
+
+def measured_traffic(prefix):
+    """HBM bytes per launch (fetched + written) of the kernel whose name starts with
+    `prefix` in the committed rocprofv3 PMC summary of the FP32 evaluation, or None."""
+    import re
+    try:
+        txt = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    for line in txt.splitlines():
+        m = re.match(r"\|\s*`([^`]+)`\s*\|\s*([0-9.]+)\s*\|\s*([0-9.]+)\s*\|", line)
+        if m and m.group(1).startswith(prefix):
+            return (float(m.group(2)) + float(m.group(3))) * 1e9
+    return None
+
+
+def roofline_of(ph, N, no, np_, cbytes, traffic=False):
+    """Roofline of the dominant kernel from median phase times: per launch, the
+    algorithmic bytes over the average launch duration (HIP events on the
+    library's stream)."""
+    pbytes = phase_bytes(N, no, np_, cbytes)
+    per = {"angl2proj": (ph[0], pbytes["angl2proj"]),
+           "fft": (ph[1] + ph[3], pbytes["fft_forward"] + pbytes["fft_inverse"]),
+           "oz": (ph[2], pbytes["oz"]),
+           "proj2angl_ideal_grad": (ph[4], pbytes["proj2angl_ideal_grad"])}
+    dom = max(per, key=lambda k: per[k][0])
+    name, launches, prof = KERNELS[dom]
+    ms, nbytes = per[dom][0] / launches, per[dom][1] / launches
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    t = measured_traffic(prof) if traffic else None
+    return {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": t, "algorithmic_bytes_per_launch": nbytes,
+            "launch_ms": ms, "launches_per_evaluation": launches}
+
+
 def algorithmic_bytes(N, no, np_, b=16):
     """Fused minimum of one evaluation: xi twice, vext once, dF once (FP64) and
     the projection field read and written in four phases."""
@@ -164,11 +208,9 @@ def main():
         ph.append(eng.phase_times())
     ph = np.median(np.array(ph), axis=0)
     cbytes = 16 if args.precision == "fp64" else 8  # projection field: complex FP64 or FP32
-    pbytes = phase_bytes(eng.N, eng.no, eng.np_, cbytes)
-    dom = max(pbytes, key=lambda k: ph[PHASES.index(k)])
-    dom_ms = float(ph[PHASES.index(dom)])
-    achieved = pbytes[dom] / (dom_ms * 1e-3) / 1e9
     algo = algorithmic_bytes(eng.N, eng.no, eng.np_, cbytes)
+    roof = roofline_of(ph, eng.N, eng.no, eng.np_, cbytes, traffic=args.precision == "fp32")
+    roof.update({"whole_evaluation_algorithmic_GB": algo / 1e9, "whole_evaluation_frac": algo / dt / 1e9 / HBM_PEAK_GBS})
 
     line = {
         "metric": METRIC,
@@ -185,10 +227,7 @@ def main():
         "evaluations_per_s": 1.0 / dt,
         "ms_per_evaluation": dt * 1e3,
         "phases_ms": {k: float(v) for k, v in zip(PHASES, ph)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "whole_evaluation_algorithmic_GB": algo / 1e9,
-                     "whole_evaluation_frac": algo / dt / 1e9 / HBM_PEAK_GBS},
+        "roofline": roof,
         "setup_s": t_setup,
     }
 
@@ -244,6 +283,16 @@ def main():
             t_fast, res_f, dg_fast = fast(True)
         else:
             t_fast, res_f, dg_fast = t_cold32, res_c32, dg_c32
+        # the fast path's evaluations: phase times of the FP32 engine (HIP events on its
+        # stream) give the roofline of the dominant kernel of the timed region
+        ph32 = []
+        for _ in range(3):
+            e32.energy_and_gradient(xi, df, timers=True)
+            ph32.append(e32.phase_times())
+        ph32 = np.median(np.array(ph32), axis=0)
+        line["roofline_fp64"] = line["roofline"]
+        line["roofline"] = roofline_of(ph32, e32.N, e32.no, e32.np_, 8, traffic=True)
+        line["phases_ms_fp32"] = {k: float(v) for k, v in zip(PHASES, ph32)}
         # reference-style solvation free energy of the fast path's final density
         # (src/module_pressure_correction.f90: F + PMV correction + LJ tail), outside the timed region
         pc = eng.pressure_correction(xi)
