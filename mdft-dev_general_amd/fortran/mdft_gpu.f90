@@ -60,6 +60,16 @@ module mdft_gpu
             real(c_double), intent(in) :: atoms(*), sites(*)
             real(c_double), value :: rcut
         end function
+        ! electrostatics: MDFT_ELEC_POISSON (0, the reference's fast Poisson solver),
+        ! MDFT_ELEC_DIRECT (1, minimum-image Coulomb) or MDFT_ELEC_NONE (2)
+        integer(c_int) function mdft_compute_vext_ex(ctx, natom, atoms, nsite, sites, rcut, elec) &
+                bind(C, name="mdft_compute_vext_ex")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: natom, nsite, elec
+            real(c_double), intent(in) :: atoms(*), sites(*)
+            real(c_double), value :: rcut
+        end function
         integer(c_int) function mdft_set_vext(ctx, vext, flags) bind(C, name="mdft_set_vext")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, vext
@@ -134,7 +144,7 @@ module mdft_gpu
         end function
     end interface
 
-    public :: mdft_ctx_create, mdft_ctx_destroy, mdft_get_info, mdft_set_ck, mdft_compute_vext, mdft_set_vext
+    public :: mdft_ctx_create, mdft_ctx_destroy, mdft_get_info, mdft_set_ck, mdft_compute_vext, mdft_compute_vext_ex, mdft_set_vext
     public :: mdft_energy_and_gradient, mdft_excess_energy, mdft_device_alloc, mdft_device_free
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
     public :: mdft_vec_multi_dot, mdft_vec_combine
