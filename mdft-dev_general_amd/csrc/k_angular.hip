@@ -327,7 +327,9 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     __shared__ T o[PT * NO];
     __shared__ V G[PT * NTH * NPHI * H1];
     __shared__ V F[PT * NTH * W * H1];
-    __shared__ T cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], a2p[NP * NTH], fmv[NP];
+    __shared__ V twpsi[H1 * NPSI];  // (cos, sin)(2 pi mu2 ipsi / npsi) at [mu2][ipsi]
+    __shared__ V twphi[W * NPHI];   // (cos, sin)(2 pi mup iphi / nphi) / (nphi npsi) at [mup + L][iphi]
+    __shared__ T a2p[NP * NTH], fmv[NP];
     __shared__ int pmup[NP], pmu2[NP];
     const int tid = threadIdx.x;
     const size_t r0 = (size_t)blockIdx.x * PT;
@@ -342,8 +344,17 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
         const int i = tid + q * kThreads;
         xr[q] = (i < npts * NO) ? src[i] : 0.0;
     }
-    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = (T)t.cph[i]; sph[i] = (T)t.sph[i]; }
-    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = (T)t.cps[i]; sps[i] = (T)t.sps[i]; }
+    for (int i = tid; i < H1 * NPSI; i += kThreads) {
+        const int kk = ((i / NPSI) * (i % NPSI)) % NPSI;
+        twpsi[i] = mk<V>((T)t.cps[kk], (T)t.sps[kk]);
+    }
+    for (int i = tid; i < W * NPHI; i += kThreads) {
+        const int mup = i / NPHI - L, iphi = i % NPHI;
+        int kk = (mup * iphi) % NPHI;
+        kk += (kk < 0) ? NPHI : 0;
+        const double nrm = 1.0 / (double)(NPHI * NPSI);
+        twphi[i] = mk<V>((T)(t.cph[kk] * nrm), (T)(t.sph[kk] * nrm));
+    }
     for (int i = tid; i < NP * NTH; i += kThreads) a2p[i] = (T)t.a2p[i];
     for (int i = tid; i < NP; i += kThreads) { fmv[i] = (T)t.fm[i]; pmup[i] = t.pmup[i]; pmu2[i] = t.pmu[i] / 2; }
 #pragma unroll
@@ -362,33 +373,31 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
         const int mu2 = i % H1;
         const T *line = o + (i / H1) * NPSI;
+        const V *tw = twpsi + mu2 * NPSI;
         T re = 0, im = 0;
 #pragma unroll
         for (int ipsi = 0; ipsi < NPSI; ipsi++) {
-            const int k = (mu2 * ipsi) % NPSI;
             const T v = line[ipsi];
-            re += v * cps[k];
-            im += v * sps[k];
+            re += v * tw[ipsi].x;
+            im += v * tw[ipsi].y;
         }
         G[i] = mk<V>(re, im);
     }
     __syncthreads();
-    const T norm = (T)(1.0 / (double)(NPHI * NPSI));
     for (int i = tid; i < npts * NTH * W * H1; i += kThreads) {
         const int mu2 = i % H1;
         const int rest = i / H1;
-        const int mup = rest % W - L;
+        const int jm = rest % W;  // mup + L
         const V *g = G + (rest / W) * NPHI * H1 + mu2;
+        const V *tw = twphi + jm * NPHI;
         T re = 0, im = 0;
 #pragma unroll
         for (int iphi = 0; iphi < NPHI; iphi++) {
-            int k = (mup * iphi) % NPHI;
-            k += (k < 0) ? NPHI : 0;
-            const V v = g[iphi * H1];
-            re += v.x * cph[k] - v.y * sph[k];
-            im += v.x * sph[k] + v.y * cph[k];
+            const V v = g[iphi * H1], e = tw[iphi];
+            re += v.x * e.x - v.y * e.y;
+            im += v.x * e.y + v.y * e.x;
         }
-        F[i] = mk<V>(re * norm, im * norm);
+        F[i] = mk<V>(re, im);
     }
     __syncthreads();
     V *dst = proj + r0 * NP;
@@ -419,7 +428,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     __shared__ V Gm[PT * NP];
     __shared__ V f[PT * NTH * W * H1];
     __shared__ V h[PT * NTH * NPHI * H1];
-    __shared__ T cph[NPHI], sph[NPHI], cps[NPSI], sps[NPSI], p2a[NP * NTH];
+    __shared__ V twphi[NPHI * W];   // (cos, sin)(2 pi mup iphi / nphi) at [iphi][mup + L]
+    __shared__ V twpsi[NPSI * H1];  // 2 (cos, sin)(2 pi mu2 ipsi / npsi) at [ipsi][mu2]
+    __shared__ T p2a[NP * NTH];
     __shared__ double wq[NO];
     __shared__ int pidx[(L + 1) * W * H1];
     const int tid = threadIdx.x;
@@ -441,31 +452,43 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         }
     }
     for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
-    for (int i = tid; i < NPHI; i += kThreads) { cph[i] = (T)t.cph[i]; sph[i] = (T)t.sph[i]; }
-    for (int i = tid; i < NPSI; i += kThreads) { cps[i] = (T)t.cps[i]; sps[i] = (T)t.sps[i]; }
+    for (int i = tid; i < NPHI * W; i += kThreads) {
+        const int iphi = i / W, mup = i % W - L;
+        int kk = (mup * iphi) % NPHI;
+        kk += (kk < 0) ? NPHI : 0;
+        twphi[i] = mk<V>((T)t.cph[kk], (T)t.sph[kk]);
+    }
+    for (int i = tid; i < NPSI * H1; i += kThreads) {
+        const int kk = ((i % H1) * (i / H1)) % NPSI;
+        twpsi[i] = mk<V>((T)(2.0 * t.cps[kk]), (T)(2.0 * t.sps[kk]));
+    }
     for (int i = tid; i < NP * NTH; i += kThreads) p2a[i] = (T)t.p2a[i];
     for (int i = tid; i < NO; i += kThreads) wq[i] = t.w[i];
     const V *src = proj + r0 * NP;
     for (int i = tid; i < npts * NP; i += kThreads) Gm[i] = src[i];
     __syncthreads();
-    // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m)
-    for (int i = tid; i < npts * NTH * W * H1; i += kThreads) {
+    // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m): one thread per
+    // (point, mup, mu2), all theta at once (each projection read once)
+    for (int i = tid; i < npts * W * H1; i += kThreads) {
         const int mu2 = i % H1;
-        int rest = i / H1;
-        const int mup = rest % W - L;
-        rest /= W;
-        const int it = rest % NTH, pt = rest / NTH;
+        const int mup = (i / H1) % W - L, pt = i / (H1 * W);
         const int a = mup < 0 ? -mup : mup, b = 2 * mu2;
         const int mlo = a > b ? a : b;
-        T re = 0, im = 0;
+        T re[NTH], im[NTH];
+#pragma unroll
+        for (int it = 0; it < NTH; it++) re[it] = im[it] = (T)0;
         for (int m = mlo; m <= L; m++) {
             const int p = pidx[(m * W + mup + L) * H1 + mu2];
             const V v = Gm[pt * NP + p];
-            const T c = p2a[p * NTH + it];
-            re += v.x * c;
-            im += v.y * c;
+            const T *c = p2a + p * NTH;
+#pragma unroll
+            for (int it = 0; it < NTH; it++) {
+                re[it] += v.x * c[it];
+                im[it] += v.y * c[it];
+            }
         }
-        f[i] = mk<V>(re, im);
+#pragma unroll
+        for (int it = 0; it < NTH; it++) f[((pt * NTH + it) * W + (mup + L)) * H1 + mu2] = mk<V>(re[it], im[it]);
     }
     __syncthreads();
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
@@ -474,14 +497,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         const int iphi = rest % NPHI;
         rest /= NPHI;
         const V *fr = f + rest * W * H1 + mu2;
+        const V *tw = twphi + iphi * W;
         T re = 0, im = 0;
 #pragma unroll
-        for (int mup = -L; mup <= L; mup++) {
-            int kk = (mup * iphi) % NPHI;
-            kk += (kk < 0) ? NPHI : 0;
-            const V v = fr[(mup + L) * H1];
-            re += v.x * cph[kk] + v.y * sph[kk];
-            im += v.x * sph[kk] - v.y * cph[kk];
+        for (int j = 0; j < W; j++) {
+            const V v = fr[j * H1], e = tw[j];
+            re += v.x * e.x + v.y * e.y;
+            im += v.x * e.y - v.y * e.x;
         }
         h[i] = mk<V>(re, im);
     }
@@ -495,12 +517,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         const int pt = i / NO, io = i - pt * NO;
         const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
         const V *hr = h + (pt * NTH * NPHI + line) * H1;
+        const V *tp = twpsi + ipsi * H1;
         T vt = hr[0].x;
 #pragma unroll
-        for (int mu2 = 1; mu2 < H1; mu2++) {
-            const int kk = (mu2 * ipsi) % NPSI;
-            vt += (T)2 * (hr[mu2].x * cps[kk] - hr[mu2].y * sps[kk]);
-        }
+        for (int mu2 = 1; mu2 < H1; mu2++) vt += hr[mu2].x * tp[mu2].x - hr[mu2].y * tp[mu2].y;
         // from here on FP64: free-energy reductions, rho ln rho and the gradient
         const double v = (double)vt;
         const double wio = wq[io];
@@ -513,7 +533,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
             const double ve = vr[q];
             double gid = 0.0;
             if (rho > DBL_EPSILON) {
-                const double lg = log(rho / rho0);
+                const double lg = log(x * x);  // ln(rho / rho0)
                 fid += kT * dv * wio * (rho * lg - rho + rho0);
                 fext += dv * wio * rho * ve;
                 gid = wio * 2.0 * rho0 * x * (kT * lg + ve);
