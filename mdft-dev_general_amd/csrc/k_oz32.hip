@@ -53,29 +53,46 @@ struct Oz32Args {
     const int *pq, *pmq, *pflag, *cstart, *ccount, *ciq;
     const double2 *ck;
     const float *ftab;   // Fourier coefficients of d^m(theta)
-    const int *lanerow;  // [kRowsPerLane][64] OZ rows of each lane, -1 = none
+    const int *itab;     // descriptor table (see oz32_itab_size)
 };
+
+// Integer descriptor table (host-built by oz32_tables, staged in LDS), in order:
+//   rin  [2 np][4]  rotation to the q frame, output o (o >= np: the -q side):
+//                   {Y offset, step, d offset, W | side << 8 | (m odd) << 9}
+//   rout [2 np][4]  rotation back, output o: {G offset, step, d row, W | side << 8 |
+//                   (m odd) << 9 | |mup| << 10 | (mup < 0) << 14}
+//   lrow [kRowsPerLane][64]  OZ rows of each lane (-1: none)
+//   dd   [nR]       entry e of the d^m(theta) block: Fourier-table offset << 4 | m << 1 | (a - b odd)
+//   ncol [kNKhi]    OZ columns of each khi;  ctab [kNKhi][kCMax]: A/B offset << 1 | (nu < 0)
+constexpr int kNKhi = 2 * kMaxL + 1;
+constexpr int kCMax = 32;
+
+__host__ __device__ inline int oz32_itab_size(int np, int nR) {
+    return 16 * np + kRowsPerLane * 64 + nR + kNKhi + kNKhi * kCMax;
+}
 
 template <int kWaves>
 __global__ __launch_bounds__(64 * kWaves) void oz32(float2 *__restrict__ proj, Oz32Args a) {
     constexpr int kThreads = 64 * kWaves;
     extern __shared__ __align__(16) unsigned char sm[];
-    const int np = a.np, npc = a.npc, L = a.L, mrso = a.mrso;
+    const int np = a.np, npc = a.npc, L = a.L;
     const int nR = roff(L + 1);
     int nTab = 0;
     for (int m = 0; m <= L; m++) nTab += (2 * m + 1) * (2 * m + 1) * (m + 1);
-    // ---- LDS carve-up (float2 first, then floats, then ints)
+    const int nF = (nTab + kWaves * nR + 3) & ~3;  // floats, padded to 16 bytes
+    const int nI = oz32_itab_size(np, nR);
+    // ---- LDS carve-up: float2, then floats, then the integer table
     float2 *c0 = reinterpret_cast<float2 *>(sm), *c1 = c0 + npc;
-    float2 *wv2 = c1 + npc;  // per wave: Y0 Y1 A B G0 G1 [np] + phase [kMaxL+1]
-    const int per2 = 6 * np + kMaxL + 1;
+    float2 *wv2 = c1 + npc;  // per wave: Y [2 np] (q, -q), A [2 np], G [2 np], phase [kMaxL + 1]
+    const int per2 = (6 * np + kMaxL + 2) & ~1;  // even: every region below stays 16-byte aligned
     float *ftab = reinterpret_cast<float *>(wv2 + kWaves * per2);
     float *wvd = ftab + nTab;  // per wave: d [nR]
-    int *tm = reinterpret_cast<int *>(wvd + kWaves * nR);
-    int *tmup = tm + np, *tmu2 = tmup + np, *tia0 = tmu2 + np, *poff = tia0 + np, *foff = poff + (kMaxL + 2);
-    int *lrow = foff + (kMaxL + 2);
+    int *itab = reinterpret_cast<int *>(ftab + nF);
+    int *tmup = itab + nI, *tia0 = tmup + np;
+    const int4 *rin = reinterpret_cast<const int4 *>(itab), *rout = rin + 2 * np;
+    const int *lrow = itab + 16 * np, *dd = lrow + kRowsPerLane * 64, *ncol = dd + nR, *ctab = ncol + kNKhi;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    float2 *Y0 = wv2 + w * per2, *Y1 = Y0 + np, *A = Y1 + np, *B = A + np, *G0 = B + np, *G1 = G0 + np;
-    float2 *ph = G1 + np;
+    float2 *Y = wv2 + w * per2, *A = Y + 2 * np, *G = A + 2 * np, *ph = G + 2 * np;
     float *d = wvd + w * nR;
 
     const int chunk = blockIdx.x;
@@ -86,24 +103,12 @@ __global__ __launch_bounds__(64 * kWaves) void oz32(float2 *__restrict__ proj, O
         c1[i] = make_float2((float)v.x, (float)v.y);
     }
     for (int i = threadIdx.x; i < nTab; i += kThreads) ftab[i] = a.ftab[i];
+    for (int i = threadIdx.x; i < nI; i += kThreads) itab[i] = a.itab[i];
     for (int i = threadIdx.x; i < np; i += kThreads) {
-        tm[i] = a.pm[i];
         tmup[i] = a.pmup[i];
-        tmu2[i] = a.pmu[i] / mrso;
         tia0[i] = a.ia0[i];
     }
-    for (int i = threadIdx.x; i < kRowsPerLane * 64; i += kThreads) lrow[i] = a.lanerow[i];
-    if (threadIdx.x == 0) {
-        int o = 0, f = 0;
-        for (int m = 0; m <= L; m++) {
-            poff[m] = o;
-            foff[m] = f;
-            o += (2 * m + 1) * (m / mrso + 1);
-            f += (2 * m + 1) * (2 * m + 1) * (m + 1);
-        }
-    }
     __syncthreads();
-#define PIDX(m, mup, mu2) (poff[(m)] + ((mup) + (m)) * ((m) / mrso + 1) + (mu2))
 
     const int start = a.cstart[chunk], count = a.ccount[chunk];
     for (int k = w; k < count; k += kWaves) {
@@ -137,119 +142,110 @@ __global__ __launch_bounds__(64 * kWaves) void oz32(float2 *__restrict__ proj, O
         }
         // d^m_{a,b}(theta), every entry from its Fourier coefficients
         for (int e = lane; e < nR; e += 64) {
-            int m = 0;
-            while (m < L && roff(m + 1) <= e) m++;
-            const int rem = e - roff(m), W = 2 * m + 1;
-            const int ab = (rem / W) - (rem % W);  // a - b
-            const float *T = ftab + foff[m] + rem * (m + 1);
+            const int ds = dd[e];
+            const float *T = ftab + (ds >> 4);
+            const int m = (ds >> 1) & 7;
+            const bool odd = ds & 1;
             float s = 0.0f;
 #pragma unroll
             for (int j = 0; j <= kMaxL; j++)
-                if (j <= m) s += T[j] * ((ab & 1) ? skt[j] : ckt[j]);
+                if (j <= m) s += T[j] * (odd ? skt[j] : ckt[j]);
             d[e] = s;
         }
         wave_sync();
-        // phase multiply: Y = X exp(-i mup phi)
+        // phase multiply: Y = X exp(-i mup phi), q side then -q side
         const float2 *src0 = proj + (size_t)gq * np, *src1 = proj + (size_t)gmq * np;
         for (int i = lane; i < np; i += 64) {
             const int mup = tmup[i];
             const float2 e = ph[mup < 0 ? -mup : mup];
             const float2 em = make_float2(e.x, mup < 0 ? e.y : -e.y);
-            Y0[i] = cmulf(src0[i], em);
-            Y1[i] = cmulf(src1[i], em);
+            Y[i] = cmulf(src0[i], em);
+            Y[np + i] = cmulf(src1[i], em);
         }
         wave_sync();
-        // rotation to the q frame: A(m,khi,mu2) = sum_mup Y0(m,mup,mu2) d(mup,khi),
-        // B(m,khi,mu2) = (-1)^m sum_mup Y1(m,mup,mu2) d(mup,-khi)
+        // rotation to the q frame: A(m,khi,mu2) = sum_mup Y_q(m,mup,mu2) d(mup,khi),
+        // B(m,khi,mu2) = (-1)^m sum_mup Y_-q(m,mup,mu2) d(mup,-khi)   (B = A + np)
         for (int o = lane; o < 2 * np; o += 64) {
-            const bool second = o >= np;
-            const int ip = second ? o - np : o;
-            const int m = tm[ip], khi = tmup[ip], mu2 = tmu2[ip];
-            const int W = 2 * m + 1, step = m / mrso + 1;
-            const float2 *Y = (second ? Y1 : Y0) + PIDX(m, -m, mu2);
-            const float *dc = d + roff(m) + (second ? -khi : khi) + m;
+            const int4 r = rin[o];
+            const int W = r.w & 255;
+            const float2 *y = Y + r.x;
+            const float *dc = d + r.z;
             float sx = 0.0f, sy = 0.0f;
             for (int t = 0; t < W; t++) {
                 const float dv = dc[t * W];
-                const float2 y = Y[t * step];
-                sx += dv * y.x;
-                sy += dv * y.y;
+                const float2 v = y[t * r.y];
+                sx += dv * v.x;
+                sy += dv * v.y;
             }
-            if (second) {
-                const float sg = (m & 1) ? -1.0f : 1.0f;
-                B[ip] = make_float2(sg * sx, sg * sy);
-            } else {
-                A[ip] = make_float2(sx, sy);
-            }
+            const float sg = ((r.w >> 8) & (r.w >> 9) & 1) ? -1.0f : 1.0f;
+            A[o] = make_float2(sg * sx, sg * sy);
         }
         wave_sync();
         // molecular OZ (:561-606): nu < 0 from the same side, nu >= 0 conjugated
         // from the other side; both sides share each interpolated coefficient
 #pragma unroll
-        for (int r = 0; r < kRowsPerLane; r++) {
-            const int ip = lrow[r * 64 + lane];
+        for (int rr = 0; rr < kRowsPerLane; rr++) {
+            const int ip = lrow[rr * 64 + lane];
             if (ip < 0) continue;
-            const int khi = tmup[ip];
-            int ia = tia0[ip];
+            const int kk = tmup[ip] + kMaxL;
+            const int nc = ncol[kk];
+            const int *ctk = ctab + kk * kCMax;
+            const float2 *lo = c0 + tia0[ip], *hi = c1 + tia0[ip];
             float qx = 0.0f, qy = 0.0f, mx = 0.0f, my = 0.0f;
-            for (int n = khi < 0 ? -khi : khi; n <= L; n++) {
-                const int h = n / mrso;
-                const int base = PIDX(n, khi, 0);
-                for (int nu2 = -h; nu2 <= h; nu2++, ia++) {
-                    const float2 lo = c0[ia], hi = c1[ia];
-                    const float cx = lo.x + alpha * (hi.x - lo.x), cy = lo.y + alpha * (hi.y - lo.y);
-                    float2 u, v;  // sources of the q and -q sums
-                    if (nu2 < 0) {
-                        u = A[base - nu2];
-                        v = B[base - nu2];
-                    } else {
-                        const float2 bb = B[base + nu2], aa = A[base + nu2];
-                        u = make_float2(bb.x, -bb.y);
-                        v = make_float2(aa.x, -aa.y);
-                    }
-                    qx += cx * u.x - cy * u.y;
-                    qy += cx * u.y + cy * u.x;
-                    mx += cx * v.x - cy * v.y;
-                    my += cx * v.y + cy * v.x;
+            for (int c = 0; c < nc; c++) {
+                const int ent = ctk[c];
+                const int off = ent >> 1;
+                const float2 l = lo[c], h = hi[c];
+                const float cx = l.x + alpha * (h.x - l.x), cy = l.y + alpha * (h.y - l.y);
+                const float2 aa = A[off], bb = A[np + off];
+                float2 u, v;  // sources of the q and -q sums
+                if (ent & 1) {
+                    u = aa;
+                    v = bb;
+                } else {
+                    u = make_float2(bb.x, -bb.y);
+                    v = make_float2(aa.x, -aa.y);
                 }
+                qx += cx * u.x - cy * u.y;
+                qy += cx * u.y + cy * u.x;
+                mx += cx * v.x - cy * v.y;
+                my += cx * v.y + cy * v.x;
             }
-            G0[ip] = make_float2(qx, qy);
-            G1[ip] = make_float2(mx, my);
+            G[ip] = make_float2(qx, qy);
+            G[np + ip] = make_float2(mx, my);
         }
         wave_sync();
         // rotation back (:611-641) and store (:648-656)
         for (int o = lane; o < 2 * np; o += 64) {
-            const bool second = o >= np;
+            const int4 r = rout[o];
+            const bool second = (r.w >> 8) & 1;
             if (!second && eq) continue;  // q == -q: the second store wins
-            const int ip = second ? o - np : o;
-            const int m = tm[ip], mup = tmup[ip], mu2 = tmu2[ip];
-            const int W = 2 * m + 1, step = m / mrso + 1;
-            const float2 *G = (second ? G1 : G0) + PIDX(m, -m, mu2);
-            const float *dr = d + roff(m) + (mup + m) * W;
+            const int W = r.w & 255;
+            const float2 *g = G + r.x;
+            const float *dr = d + r.z;
             float sx = 0.0f, sy = 0.0f;
             for (int t = 0; t < W; t++) {
-                const int kk = second ? (W - 1 - t) : t;  // d(mup, -khi) for the -q side
-                const float dv = dr[kk];
-                const float2 g = G[t * step];
-                sx += dv * g.x;
-                sy += dv * g.y;
+                const float dv = dr[second ? (W - 1 - t) : t];  // d(mup, -khi) for the -q side
+                const float2 v = g[t * r.y];
+                sx += dv * v.x;
+                sy += dv * v.y;
             }
-            const float2 e = ph[mup < 0 ? -mup : mup];
-            const float2 ep = make_float2(e.x, mup < 0 ? -e.y : e.y);  // exp(+i mup phi)
+            const float2 e = ph[(r.w >> 10) & 15];
+            const float2 ep = make_float2(e.x, ((r.w >> 14) & 1) ? -e.y : e.y);  // exp(+i mup phi)
             float2 out = cmulf(make_float2(sx, sy), ep);
             if (second) {
-                const float sg = (m & 1) ? -1.0f : 1.0f;
-                out.x *= sg;
-                out.y *= sg;
+                if ((r.w >> 9) & 1) {
+                    out.x = -out.x;
+                    out.y = -out.y;
+                }
                 if (eq && nyq) out.y = -out.y;
-                proj[(size_t)gmq * np + ip] = out;
+                proj[(size_t)gmq * np + (o - np)] = out;
             } else {
-                proj[(size_t)gq * np + ip] = out;
+                proj[(size_t)gq * np + o] = out;
             }
         }
         wave_sync();
     }
-#undef PIDX
 }
 
 double textbook_d(int j, int mp, int m, double beta) {
@@ -271,8 +267,10 @@ double textbook_d(int j, int mp, int m, double beta) {
 }  // namespace
 
 // Host tables of the FP32 q-space kernel: Fourier coefficients of d^m(theta)
-// and the balanced assignment of OZ rows to lanes.
-int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &lanerow) {
+// and the integer descriptor table (layout at oz32_itab_size): per-output
+// offsets of the two rotations, the balanced assignment of OZ rows to lanes,
+// the d-block entries and the OZ columns of each khi.
+int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &itab) {
     const int L = c->L;
     const double hp = 0.5 * acos(-1.0);
     ftab.clear();
@@ -308,7 +306,7 @@ int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &lanerow
         order[ip] = ip;
     }
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return len[x] > len[y]; });
-    lanerow.assign(MDFT_OZ32_ROWS_PER_LANE * 64, -1);
+    std::vector<int> lanerow(MDFT_OZ32_ROWS_PER_LANE * 64, -1);
     std::vector<int> load(64, 0), cnt(64, 0);
     for (int ip : order) {
         int best = -1;
@@ -318,6 +316,49 @@ int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &lanerow
         lanerow[cnt[best] * 64 + best] = ip;
         cnt[best]++;
         load[best] += len[ip];
+    }
+    const int np = c->np, mrso = c->d.mrso;
+    if (L > kMaxL) return MDFT_EINVAL;
+    std::vector<int> poff(L + 2, 0), foff(L + 2, 0);
+    for (int m = 0; m <= L; m++) {
+        poff[m + 1] = poff[m] + (2 * m + 1) * (m / mrso + 1);
+        foff[m + 1] = foff[m] + (2 * m + 1) * (2 * m + 1) * (m + 1);
+    }
+    auto pidx = [&](int m, int mup, int mu2) { return poff[m] + (mup + m) * (m / mrso + 1) + mu2; };
+    auto roffh = [](int m) { return m * (4 * m * m - 1) / 3; };
+    const int nR = roffh(L + 1);
+    itab.assign(oz32_itab_size(np, nR), 0);
+    int *rin = itab.data(), *rout = rin + 8 * np, *lrow = itab.data() + 16 * np, *dd = lrow + MDFT_OZ32_ROWS_PER_LANE * 64;
+    int *ncol = dd + nR, *ctab = ncol + kNKhi;
+    for (int o = 0; o < 2 * np; o++) {
+        const int second = o >= np ? 1 : 0, ip = o - second * np;
+        const int m = c->pm[ip], mup = c->pmup[ip], mu2 = c->pmu[ip] / mrso, W = 2 * m + 1;
+        const int base = second * np + pidx(m, -m, mu2), step = m / mrso + 1;
+        rin[4 * o] = base;
+        rin[4 * o + 1] = step;
+        rin[4 * o + 2] = roffh(m) + (second ? -mup : mup) + m;  // mup plays khi here
+        rin[4 * o + 3] = W | second << 8 | (m & 1) << 9;
+        rout[4 * o] = base;
+        rout[4 * o + 1] = step;
+        rout[4 * o + 2] = roffh(m) + (mup + m) * W;
+        rout[4 * o + 3] = W | second << 8 | (m & 1) << 9 | std::abs(mup) << 10 | (mup < 0 ? 1 : 0) << 14;
+    }
+    std::copy(lanerow.begin(), lanerow.end(), lrow);
+    for (int e = 0, m = 0; e < nR; e++) {
+        while (m < L && roffh(m + 1) <= e) m++;
+        const int rem = e - roffh(m), W = 2 * m + 1, ab = rem / W - rem % W;
+        dd[e] = (foff[m] + rem * (m + 1)) << 4 | m << 1 | (ab & 1);
+    }
+    for (int khi = -L; khi <= L; khi++) {
+        int n_ = 0;
+        for (int n = std::abs(khi); n <= L; n++) {
+            const int h = n / mrso;
+            for (int nu2 = -h; nu2 <= h; nu2++) {
+                if (n_ >= kCMax) return MDFT_EINVAL;
+                ctab[(khi + kMaxL) * kCMax + n_++] = (pidx(n, khi, 0) + std::abs(nu2)) << 1 | (nu2 < 0 ? 1 : 0);
+            }
+        }
+        ncol[khi + kMaxL] = n_;
     }
     return MDFT_OK;
 }
@@ -346,14 +387,14 @@ hipError_t launch_oz32(mdft_ctx *c, float2 *proj) {
     a.ciq = c->pairs.ciq;
     a.ck = c->d_ck;
     a.ftab = c->d_oz32_ftab;
-    a.lanerow = c->d_oz32_lanerow;
+    a.itab = c->d_oz32_lanerow;
     int nTab = 0;
     for (int m = 0; m <= c->L; m++) nTab += (2 * m + 1) * (2 * m + 1) * (m + 1);
     const int nR = (c->L + 1) * (4 * (c->L + 1) * (c->L + 1) - 1) / 3;
     auto lds_for = [&](int waves) {
-        return sizeof(float2) * (2 * (size_t)c->npc + (size_t)waves * (6 * c->np + MDFT_MAX_MMAX + 1)) +
-               sizeof(float) * (nTab + (size_t)waves * nR) +
-               sizeof(int) * (4 * (size_t)c->np + 2 * (MDFT_MAX_MMAX + 2) + MDFT_OZ32_ROWS_PER_LANE * 64);
+        const size_t nF = ((size_t)nTab + (size_t)waves * nR + 3) & ~(size_t)3;
+        return sizeof(float2) * (2 * (size_t)c->npc + (size_t)waves * ((6 * c->np + MDFT_MAX_MMAX + 2) & ~1)) +
+               sizeof(float) * nF + sizeof(int) * ((size_t)oz32_itab_size(c->np, nR) + 2 * (size_t)c->np);
     };
     const dim3 grid((unsigned)c->pairs.nchunks);
     if (lds_for(8) <= c->lds_limit)

@@ -74,7 +74,7 @@ struct mdft_ctx {
     double dq = 0;
     double2 *d_ck = nullptr;      // [nq][npc]
     float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
-    int *d_oz32_lanerow = nullptr; // [MDFT_OZ32_ROWS_PER_LANE][64] OZ rows per lane
+    int *d_oz32_lanerow = nullptr; // descriptor table of the FP32 q-space kernel (rotation offsets, lane rows, OZ columns)
     void *d_bf16_M = nullptr;     // angl2proj matrix, BF16 MFMA fragment order
     void *d_bf16_Q = nullptr;     // proj2angl matrix, BF16 MFMA fragment order
     PairSchedule pairs;
@@ -113,7 +113,7 @@ hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblo
 hipError_t launch_angl2proj_f32(mdft_ctx *c, const double *xi, float2 *proj);
 hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, double *df, int mode, int *nblocks_out);
 hipError_t launch_oz32(mdft_ctx *c, float2 *proj);
-int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &lanerow);
+int oz32_tables(mdft_ctx *c, std::vector<float> &ftab, std::vector<int> &itab);
 hipError_t launch_reduce3(mdft_ctx *c, int nblocks);
 hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign, int batch = 0);  // batch 0: np
 hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign, int batch = 0);
