@@ -141,6 +141,7 @@ struct LineGeom {
     int na;         // size of the first other axis
     long long sa;   // stride of the first other axis (points)
     long long sb;   // stride of the second other axis (points)
+    int align;      // > 0: batches start where runs of `align` elements meet the cache-line grid
 };
 
 template <int PB, typename V>
@@ -245,8 +246,18 @@ __global__ __launch_bounds__(kThreads) void fft_axis_c(V *__restrict__ data, int
     const int line = blockIdx.x;
     const int a = line % g.na, b = line / g.na;
     const long long base = (long long)a * g.sa + (long long)b * g.sb;
-    const int p0 = blockIdx.y * PB;
-    const int pb = (np - p0) < PB ? (np - p0) : PB;
+    // projection batch of this workgroup; with g.align (every point of the line then
+    // has the same record offset modulo a cache line) the batches start on line boundaries
+    int p0, pb;
+    const int s = g.align > 0 ? (int)((g.align - (base * np) % g.align) % g.align) : 0;
+    if (s == 0) {
+        p0 = blockIdx.y * PB;
+        pb = (np - p0) < PB ? (np - p0) : PB;
+    } else {
+        p0 = blockIdx.y == 0 ? 0 : s + (blockIdx.y - 1) * PB;
+        pb = blockIdx.y == 0 ? s : ((np - p0) < PB ? (np - p0) : PB);
+    }
+    if (pb <= 0) return;
     for (int idx = threadIdx.x; idx < N * PB; idx += kThreads) {
         const int p = idx % PB, e = idx / PB;
         if (p < pb) A[p * (N + 1) + e] = data[(base + (long long)e * g.es) * np + p0 + p];
@@ -282,17 +293,23 @@ hipError_t launch_fft3d(mdft_ctx *c, V *proj, int sign, int batch) {
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, np = batch > 0 ? batch : c->np;
     const double scale_last = (sign < 0) ? 1.0 / (double)((long long)nx * ny * nz) : 1.0;
     LineGeom geo[3] = {
-        {nx, 1, ny, (long long)nx, (long long)nx * ny},
-        {ny, (long long)nx, nx, 1, (long long)nx * ny},
-        {nz, (long long)nx * ny, nx, 1, (long long)nx},
+        {nx, 1, ny, (long long)nx, (long long)nx * ny, 0},
+        {ny, (long long)nx, nx, 1, (long long)nx * ny, 0},
+        {nz, (long long)nx * ny, nx, 1, (long long)nx, 0},
     };
     // PB projections per workgroup: 8 (FP64) or 16 (FP32) keeps 128-byte runs per grid point
     constexpr int PBmax = sizeof(V) == 16 ? 8 : 16;
     const int PB = (np % PBmax == 0 || np > PBmax) ? PBmax : 4;
+    // a point record of np elements that is not a whole number of cache lines puts the
+    // runs of odd points (say) across two lines; along an axis whose point stride times
+    // np is a whole number of lines, every point of a line shares the offset, and the
+    // batches can start on line boundaries (one more, shorter, batch per line)
+    for (int ax = 0; ax < 3; ax++)
+        if (PB == PBmax && (np % PBmax) != 0 && (geo[ax].es * np) % PBmax == 0) geo[ax].align = PBmax;
     for (int ax = 0; ax < 3; ax++) {
         const LineGeom &g = geo[ax];
         const size_t lines = c->N / (size_t)g.n;
-        dim3 grid((unsigned)lines, (unsigned)((np + PB - 1) / PB));
+        dim3 grid((unsigned)lines, (unsigned)((np + PB - 1) / PB + (g.align > 0 ? 1 : 0)));
         const double scale = (ax == 2) ? scale_last : 1.0;
         const size_t lds = sizeof(V) * 2 * (size_t)PB * (g.n + 1);
         if (PB == PBmax && launch_axis_c<PBmax, V>(g.n, grid, c->stream, proj, np, g, c->fft[ax].tw, (double)sign, scale)) {
