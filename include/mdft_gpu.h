@@ -168,6 +168,12 @@ int mdft_vec_lincomb(mdft_ctx *ctx, size_t n, double a, const double *x, double 
 int mdft_vec_multi_dot(mdft_ctx *ctx, size_t n, int k, const double *const *vecs, const double *x, double *out);
 /* One pass: z = sum_j coef[j] vecs[j], k <= 16 (z may be one of vecs). */
 int mdft_vec_combine(mdft_ctx *ctx, size_t n, int k, const double *coef, const double *const *vecs, double *z);
+/* One pass: z = sum_j coef[j] vecs[j] and y = x + a z, k <= 16 -- the new search
+ * direction and the first trial point of the minimiser's line search (the
+ * dcopy/daxpy pair of src/module_lbfgs_nocedal_mdft.f90:1207-1336 fused); y must
+ * not alias x, z or any vecs[j]. */
+int mdft_vec_combine_step(mdft_ctx *ctx, size_t n, int k, const double *coef, const double *const *vecs, double *z,
+                          double a, const double *x, double *y);
 
 /* ---- restart (density.bin) support, all device pointers ----
  * angl2proj / proj2angl of src/module_orientation_projection_transform.f90:159-199 and

@@ -816,6 +816,19 @@ int mdft_vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const dou
     return MDFT_OK;
 }
 
+int mdft_vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z,
+                          double a, const double *x, double *y) {
+    if (!c || !coef || !vecs || !z || !x || !y || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    if (!aligned16(z) || !aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    for (int j = 0; j < k; j++) {
+        if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+        if (vecs[j] == y) return fail(c, MDFT_EINVAL, "y must not alias an input vector");
+    }
+    if (y == x || y == z) return fail(c, MDFT_EINVAL, "y must not alias x or z");
+    HIPCHK(c, vec_combine_step(c, n, k, coef, vecs, z, a, x, y));
+    return MDFT_OK;
+}
+
 // src/module_pressure_correction.f90:7-156 (the bridge-free branch)
 int mdft_pressure_correction(mdft_ctx *c, const double *xi, int flags, double *out) {
     int rc = ready(c);

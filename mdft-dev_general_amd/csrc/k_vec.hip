@@ -210,6 +210,34 @@ __global__ __launch_bounds__(kThreads) void combine(size_t n, VecSet v, double *
     }
 }
 
+// z = sum_j c_j v_j and y = x + a z in one pass (the new search direction and the
+// first trial point of the line search); y must not alias x, z or a v_j
+__global__ __launch_bounds__(kThreads) void combine_step(size_t n, VecSet v, double *z, double a,
+                                                        const double *__restrict__ x, double *__restrict__ y) {
+    const size_t n2 = n / 2;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += (size_t)gridDim.x * kThreads) {
+        const double2 xv = reinterpret_cast<const double2 *>(x)[i];
+        double sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) {
+                const double2 w = reinterpret_cast<const double2 *>(v.p[j])[i];
+                sx += v.c[j] * w.x;
+                sy += v.c[j] * w.y;
+            }
+        reinterpret_cast<double2 *>(z)[i] = make_double2(sx, sy);
+        reinterpret_cast<double2 *>(y)[i] = make_double2(xv.x + a * sx, xv.y + a * sy);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) s += v.c[j] * v.p[j][n - 1];
+        z[n - 1] = s;
+        y[n - 1] = x[n - 1] + a * s;
+    }
+}
+
 // Per grid point (one wave each): n(r) = rho0 sum_io w_io xi^2; block partials of
 // sum_r n(r) and of the number of points with n(r) / n0 < cav (cavity points).
 __global__ __launch_bounds__(kThreads) void density_integrals(const double *__restrict__ xi,
@@ -330,6 +358,20 @@ hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const d
     }
     v.k = k;
     hipLaunchKernelGGL(combine, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z,
+                            double a, const double *x, double *y) {
+    VecSet v{};
+    for (int j = 0; j < k; j++) {
+        v.p[j] = vecs[j];
+        v.c[j] = coef[j];
+    }
+    v.k = k;
+    hipLaunchKernelGGL(combine_step, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);

@@ -139,6 +139,8 @@ hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double 
 hipError_t launch_density_integrals(mdft_ctx *c, const double *xi, double cav, int *nblocks_out);
 hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x, double *out);
 hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z);
+hipError_t vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z,
+                            double a, const double *x, double *y);
 hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,
                        const double *d_site_off, double rcut, double qfact, double clip, bool coulomb = true);
 hipError_t launch_poisson_vext(mdft_ctx *c, int natom, const double *h_atoms, int nsite, const double *h_site_q,

@@ -142,12 +142,23 @@ module mdft_gpu
             real(c_double), intent(in) :: coef(*)
             type(c_ptr), intent(in) :: vecs(*)
         end function
+        ! z = sum coef(j) vecs(j) and y = x + a z in one pass
+        integer(c_int) function mdft_vec_combine_step(ctx, n, k, coef, vecs, z, a, x, y) &
+                bind(C, name="mdft_vec_combine_step")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, z, x, y
+            integer(c_size_t), value :: n
+            integer(c_int), value :: k
+            real(c_double), intent(in) :: coef(*)
+            type(c_ptr), intent(in) :: vecs(*)
+            real(c_double), value :: a
+        end function
     end interface
 
     public :: mdft_ctx_create, mdft_ctx_destroy, mdft_get_info, mdft_set_ck, mdft_compute_vext, mdft_compute_vext_ex, mdft_set_vext
     public :: mdft_energy_and_gradient, mdft_excess_energy, mdft_device_alloc, mdft_device_free
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
-    public :: mdft_vec_multi_dot, mdft_vec_combine
+    public :: mdft_vec_multi_dot, mdft_vec_combine, mdft_vec_combine_step
     public :: check, last_error_text
 
     ! ---- module state, the analogue of the reference's solvent(1)%xi and grid globals

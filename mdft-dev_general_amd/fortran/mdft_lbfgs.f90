@@ -8,16 +8,17 @@
 ! search directions d_i (s_i = -t_i d_i) and the gradients g_i (y_i = g_{i+1} - g_i),
 ! their FP64 Gram matrix is kept on the host, and the two-loop recursion runs on
 ! coefficient vectors of length <= 2m+3.  Per iteration the device does one
-! linear combination of <= 2m+1 vectors (the new direction), one step update
-! and one multi-dot of the new gradient with the <= 2m+3 live vectors -- about
-! 28 passes over n instead of the ~66 of the two-loop recursion on vectors.  The line search enforces the strong Wolfe conditions with cubic
+! pass forming the new direction from <= 2m+1 vectors together with the first
+! trial point, and one multi-dot of the new gradient with the <= 2m+3 live
+! vectors (plus one step update per further trial) -- 4m + 7 passes over n
+! (11 at m = 1, 27 at m = 5) instead of the ~66 of the two-loop recursion on vectors.  The line search enforces the strong Wolfe conditions with cubic
 ! interpolation inside a bracket (the role of dcsrch,
 ! src/module_lbfgs_nocedal_mdft.f90:1653).  Stopping rule: |dF|/|dF_0| < gtol,
 ! or a relative decrease of F below ftol, or maxeval evaluations.
 module mdft_lbfgs
     use iso_c_binding
     use mdft_gpu, only: mdft_energy_and_gradient, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
-                        mdft_vec_multi_dot, mdft_vec_combine, MDFT_DEVICE_PTRS
+                        mdft_vec_multi_dot, mdft_vec_combine_step, MDFT_DEVICE_PTRS
     implicit none
     private
     integer, parameter :: dp = c_double
@@ -57,6 +58,7 @@ contains
         real(dp) :: coef(16), dots(16)
         integer :: lst(16)
         real(dp) :: parts(3), pn(3), f, fn, gnorm, g0, gd, gdn, t, beta, sy, yy, gam, f_prev
+        real(dp) :: t_ready   ! step for which xn already holds x - t d (negative: none)
         integer :: i, j, k, mm, nb, nl, fd, fg, gcur, neval, iter, status
         integer(c_size_t) :: bytes
         logical :: ok
@@ -67,6 +69,7 @@ contains
         bytes = 8_c_size_t * n
         owned = .false.
         xn_own = c_null_ptr
+        t_ready = -1.0_dp
         do i = 1, 2 * mm + 3
             vec(i) = c_null_ptr
         end do
@@ -150,7 +153,9 @@ contains
                     coef(nb) = r(j)
                 end if
             end do
-            rc = mdft_vec_combine(ctx, n, nb, coef, pv, vec(fd)); if (rc /= 0) goto 900
+            ! the direction and the line search's first trial point in one pass
+            rc = mdft_vec_combine_step(ctx, n, nb, coef, pv, vec(fd), -t, x, xn); if (rc /= 0) goto 900
+            t_ready = t
             do j = 1, 2 * mm + 3
                 if (j == fd) cycle
                 gram(fd, j) = dot_product(r, gram(:, j))
@@ -274,7 +279,10 @@ contains
         ! evaluate at xn = x - t d into slot fg and its Gram row (one multi-dot with the live vectors)
         subroutine trial()
             integer :: a
-            rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, -t, vec(fd), xn); if (rc /= 0) return
+            if (t /= t_ready) then
+                rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, -t, vec(fd), xn); if (rc /= 0) return
+            end if
+            t_ready = -1.0_dp
             rc = mdft_energy_and_gradient(ctx, xn, pn, vec(fg), MDFT_DEVICE_PTRS); if (rc /= 0) return
             neval = neval + 1
             fn = sum(pn)

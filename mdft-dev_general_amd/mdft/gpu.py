@@ -29,7 +29,7 @@ MDFT_EXPORTS = (
     "mdft_device_free", "mdft_memcpy_h2d", "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext",
     "mdft_get_vext", "mdft_vec_dot", "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot",
     "mdft_vec_combine", "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj",
-    "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex",
+    "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_vec_combine_step",
 )
 
 
@@ -93,11 +93,12 @@ def load_library(path: str = LIBPATH):
     lib.mdft_proj2angl.argtypes = [vp, vp, C.c_size_t, vp]
     lib.mdft_regrid_trilinear.argtypes = [vp, vp, _ip, _dp, C.c_double, vp]
     lib.mdft_vec_combine.argtypes = [vp, C.c_size_t, C.c_int, _dp, C.POINTER(vp), vp]
+    lib.mdft_vec_combine_step.argtypes = [vp, C.c_size_t, C.c_int, _dp, C.POINTER(vp), vp, C.c_double, vp, vp]
     for name in ("mdft_get_info", "mdft_get_quadrature", "mdft_set_vext", "mdft_set_ck",
                  "mdft_energy_and_gradient", "mdft_excess_energy", "mdft_ideal_external", "mdft_excess_stage",
                  "mdft_phase_times", "mdft_device_alloc", "mdft_device_free", "mdft_memcpy_h2d",
                  "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext", "mdft_get_vext", "mdft_vec_dot",
-                 "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine",
+                 "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine", "mdft_vec_combine_step",
                  "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj", "mdft_proj2angl",
                  "mdft_regrid_trilinear", "mdft_compute_vext_ex"):
         getattr(lib, name).restype = C.c_int

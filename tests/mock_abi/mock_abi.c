@@ -145,6 +145,20 @@ int mdft_vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, 
     return 0;
 }
 
+int mdft_vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z,
+                          double a, const double *x, double *y)
+{
+    (void)c;
+    if (k < 1 || k > 16) return -1;
+    for (size_t i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < k; j++) s += coef[j] * vecs[j][i];
+        z[i] = s;
+        y[i] = x[i] + a * s;
+    }
+    return 0;
+}
+
 int mdft_vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z)
 {
     (void)c;

@@ -59,7 +59,16 @@ def test_vector_ops():
     assert np.allclose(dots, ref, rtol=1e-10, atol=1e-9)
     coef = rng.standard_normal(k)
     eng._check(eng.lib.mdft_vec_combine(eng._h, n, k, coef.ctypes.data_as(C.POINTER(C.c_double)), ptrs, db.ptr))
-    assert np.allclose(db.download(), sum(c * v for c, v in zip(coef, vs)), rtol=1e-12, atol=1e-12)
+    z = sum(c * v for c, v in zip(coef, vs))
+    assert np.allclose(db.download(), z, rtol=1e-12, atol=1e-12)
+    # direction and first trial point in one pass: z = sum c v, y = a_vec + s z
+    dy = DeviceBuffer(eng, 8 * n)
+    eng._check(eng.lib.mdft_vec_combine_step(eng._h, n, k, coef.ctypes.data_as(C.POINTER(C.c_double)), ptrs,
+                                             db.ptr, -0.37, da.ptr, dy.ptr))
+    assert np.allclose(db.download(), z, rtol=1e-12, atol=1e-12)
+    assert np.allclose(dy.download(), a - 0.37 * z, rtol=1e-12, atol=1e-12)
+    assert eng.lib.mdft_vec_combine_step(eng._h, n, k, coef.ctypes.data_as(C.POINTER(C.c_double)), ptrs,
+                                         db.ptr, 1.0, da.ptr, da.ptr) != 0   # y aliasing x is refused
 
 
 def test_lbfgs_reaches_the_cpu_minimum(oracle_mod):
