@@ -198,18 +198,33 @@ constexpr int pick_radix(int rem) {
     return (rem % 8 == 0 && rem > 8) ? 8 : (rem % 4 == 0 ? 4 : (rem % 2 == 0 ? 2 : (rem % 3 == 0 ? 3 : 5)));
 }
 
-template <int R, int N, int Ns, int PB, typename V>
-__device__ inline void stage_c(const V *src, V *dst, const V *twl, decltype(V::x) s) {
-    constexpr int nb = N / R, ld = N + 1, tstep = N / (Ns * R);
-    for (int task = threadIdx.x; task < PB * nb; task += kThreads) {
-        const int row = task / nb, j = task - row * nb;
-        const V *in = src + row * ld;
-        V *out = dst + row * ld;
-        const int k = j % Ns;
+// In-place variant: one LDS line buffer per projection, the stages staged through
+// registers.  Task t of a stage is (row = t % PB, butterfly j = t / PB), so a wave's
+// consecutive lanes are consecutive projections of one grid point: the first stage
+// reads its inputs straight from HBM and the last writes its outputs straight back,
+// both in the 16 x 8 B (FP32) / 8 x 16 B (FP64) runs of a point record.
+template <int N, int Ns, int REM, int PB, bool FIRST, typename V>
+__device__ inline void run_stages(V *A, const V *twl, decltype(V::x) s, V *__restrict__ data, int np,
+                                  const LineGeom &g, long long base, int p0, int pb, decltype(V::x) scale) {
+    constexpr int R = pick_radix(REM), nb = N / R, ld = N + 1, tstep = N / (Ns * R);
+    constexpr bool LAST = (REM == R);
+    constexpr int NTASK = PB * nb, NT = (NTASK + kThreads - 1) / kThreads;
+    V o[NT][R];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int task = threadIdx.x + q * kThreads;
+        if (NTASK % kThreads != 0 && task >= NTASK) continue;
+        const int row = task % PB, j = task / PB, k = j % Ns;
         V v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            V x = in[j + r * nb];
+            const int e = j + r * nb;
+            V x;
+            if constexpr (FIRST) {
+                x = (row < pb) ? data[(base + (long long)e * g.es) * np + p0 + row] : mk<V>(0, 0);
+            } else {
+                x = A[row * ld + e];
+            }
             if (r > 0 && Ns > 1) {
                 const V w = twl[(k * r * tstep) % N];
                 x = cmul(x, mk<V>(w.x, w.y * s));
@@ -217,37 +232,46 @@ __device__ inline void stage_c(const V *src, V *dst, const V *twl, decltype(V::x
             v[r] = x;
         }
         Dft<R, V>::run(v, s);
-        const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; r++) out[base + r * Ns] = v[r];
+        for (int r = 0; r < R; r++) o[q][r] = v[r];
     }
-}
-
-template <int N, int Ns, int REM, int PB, typename V>
-__device__ inline const V *stages_c(V *src, V *dst, const V *twl, decltype(V::x) s) {
-    if constexpr (REM == 1) {
-        return src;
-    } else {
-        constexpr int R = pick_radix(REM);
-        stage_c<R, N, Ns, PB, V>(src, dst, twl, s);
+    if constexpr (!FIRST) __syncthreads();  // every read of this stage precedes its in-place writes
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int task = threadIdx.x + q * kThreads;
+        if (NTASK % kThreads != 0 && task >= NTASK) continue;
+        const int row = task % PB, j = task / PB, k = j % Ns;
+        const int ob = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if constexpr (LAST) {
+                if (row < pb) {
+                    V v = o[q][r];
+                    v.x *= scale;
+                    v.y *= scale;
+                    data[(base + (long long)(ob + r * Ns) * g.es) * np + p0 + row] = v;
+                }
+            } else {
+                A[row * ld + ob + r * Ns] = o[q][r];
+            }
+        }
+    }
+    if constexpr (!LAST) {
         __syncthreads();
-        return stages_c<N, Ns * R, REM / R, PB, V>(dst, src, twl, s);
+        run_stages<N, Ns * R, REM / R, PB, false, V>(A, twl, s, data, np, g, base, p0, pb, scale);
     }
 }
 
 template <int N, int PB, typename V>
-__global__ __launch_bounds__(kThreads) void fft_axis_c(V *__restrict__ data, int np, LineGeom g, const double2 *tw,
+__global__ __launch_bounds__(kThreads) void fft_axis_r(V *__restrict__ data, int np, LineGeom g, const double2 *tw,
                                                       double sgn_d, double scale_d) {
     using T = decltype(V::x);
     extern __shared__ __align__(16) unsigned char fft_smem[];
-    V *A = reinterpret_cast<V *>(fft_smem), *B = A + PB * (N + 1), *twl = B + PB * (N + 1);
+    V *A = reinterpret_cast<V *>(fft_smem), *twl = A + PB * (N + 1);
     const T sgn = (T)sgn_d, scale = (T)scale_d;
-    for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = mk<V>((T)tw[i].x, (T)tw[i].y);
     const int line = blockIdx.x;
     const int a = line % g.na, b = line / g.na;
     const long long base = (long long)a * g.sa + (long long)b * g.sb;
-    // projection batch of this workgroup; with g.align (every point of the line then
-    // has the same record offset modulo a cache line) the batches start on line boundaries
     int p0, pb;
     const int s = g.align > 0 ? (int)((g.align - (base * np) % g.align) % g.align) : 0;
     if (s == 0) {
@@ -258,30 +282,18 @@ __global__ __launch_bounds__(kThreads) void fft_axis_c(V *__restrict__ data, int
         pb = blockIdx.y == 0 ? s : ((np - p0) < PB ? (np - p0) : PB);
     }
     if (pb <= 0) return;
-    for (int idx = threadIdx.x; idx < N * PB; idx += kThreads) {
-        const int p = idx % PB, e = idx / PB;
-        if (p < pb) A[p * (N + 1) + e] = data[(base + (long long)e * g.es) * np + p0 + p];
-    }
+    for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = mk<V>((T)tw[i].x, (T)tw[i].y);
     __syncthreads();
-    const V *res = stages_c<N, 1, N, PB, V>(A, B, twl, sgn);
-    for (int idx = threadIdx.x; idx < N * PB; idx += kThreads) {
-        const int p = idx % PB, e = idx / PB;
-        if (p < pb) {
-            V v = res[p * (N + 1) + e];
-            v.x *= scale;
-            v.y *= scale;
-            data[(base + (long long)e * g.es) * np + p0 + p] = v;
-        }
-    }
+    run_stages<N, 1, N, PB, true, V>(A, twl, sgn, data, np, g, base, p0, pb, scale);
 }
 
 template <int PB, typename V>
 bool launch_axis_c(int n, dim3 grid, hipStream_t st, V *proj, int np, const LineGeom &g, const double2 *tw,
                    double sign, double scale) {
-    const size_t lds = sizeof(V) * (2 * (size_t)PB * (n + 1) + n);
+    const size_t lds = sizeof(V) * ((size_t)PB * (n + 1) + n);
     switch (n) {
 #define FFT_C(NN) \
-    case NN: hipLaunchKernelGGL((fft_axis_c<NN, PB, V>), grid, dim3(kThreads), lds, st, proj, np, g, tw, sign, scale); return true;
+    case NN: hipLaunchKernelGGL((fft_axis_r<NN, PB, V>), grid, dim3(kThreads), lds, st, proj, np, g, tw, sign, scale); return true;
         FFT_C(16) FFT_C(32) FFT_C(48) FFT_C(64) FFT_C(96) FFT_C(128) FFT_C(192) FFT_C(256)
 #undef FFT_C
         default: return false;
