@@ -130,6 +130,7 @@ struct VecSet {
     const double *p[MDFT_VEC_MAX];
     double c[MDFT_VEC_MAX];
     int k;
+    int self;  // multi-dot: bit j set when p[j] is x itself (its values are not loaded twice)
 };
 
 // part[blk][j] = sum_i x[i] v_j[i]   (16-byte accesses)
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(kThreads) void multi_dot_partial(size_t n, const do
 #pragma unroll
         for (int j = 0; j < MDFT_VEC_MAX; j++)
             if (j < v.k) {
-                const double2 w = reinterpret_cast<const double2 *>(v.p[j])[i];
+                const double2 w = ((v.self >> j) & 1) ? xv : reinterpret_cast<const double2 *>(v.p[j])[i];
                 acc[j] += xv.x * w.x + xv.y * w.y;
             }
     }
@@ -337,7 +338,10 @@ hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs
         if (e != hipSuccess) return e;
     }
     VecSet v{};
-    for (int j = 0; j < k; j++) v.p[j] = vecs[j];
+    for (int j = 0; j < k; j++) {
+        v.p[j] = vecs[j];
+        if (vecs[j] == x) v.self |= 1 << j;
+    }
     v.k = k;
     const unsigned nb = blocks_for(n);
     hipLaunchKernelGGL(multi_dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart);

@@ -10,8 +10,9 @@
 ! coefficient vectors of length <= 2m+3.  Per iteration the device does one
 ! pass forming the new direction from <= 2m+1 vectors together with the first
 ! trial point, and one multi-dot of the new gradient with the <= 2m+3 live
-! vectors (plus one step update per further trial) -- 4m + 7 passes over n
-! (11 at m = 1, 27 at m = 5) instead of the ~66 of the two-loop recursion on vectors.  The line search enforces the strong Wolfe conditions with cubic
+! vectors that stay live (plus one step update per further trial) -- 4m + 5
+! passes over n (9 at m = 1, 25 at m = 5) instead of the ~66 of the two-loop
+! recursion on vectors.  The line search enforces the strong Wolfe conditions with cubic
 ! interpolation inside a bracket (the role of dcsrch,
 ! src/module_lbfgs_nocedal_mdft.f90:1653).  Stopping rule: |dF|/|dF_0| < gtol,
 ! or a relative decrease of F below ftol, or maxeval evaluations.
@@ -278,7 +279,7 @@ contains
 
         ! evaluate at xn = x - t d into slot fg and its Gram row (one multi-dot with the live vectors)
         subroutine trial()
-            integer :: a
+            integer :: a, a0
             if (t /= t_ready) then
                 rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, -t, vec(fd), xn); if (rc /= 0) return
             end if
@@ -286,14 +287,19 @@ contains
             rc = mdft_energy_and_gradient(ctx, xn, pn, vec(fg), MDFT_DEVICE_PTRS); if (rc /= 0) return
             neval = neval + 1
             fn = sum(pn)
+            ! with the memory full, the oldest pair (ds(1), gs(1)) is dropped after this
+            ! iteration whatever the line search gives: its products with the new
+            ! gradient are never used, so they are not computed
+            a0 = 1
+            if (k == mm) a0 = 2
             nl = 0
-            do a = 1, k
+            do a = a0, k
                 nl = nl + 1
                 lst(nl) = ds(a)
             end do
             nl = nl + 1
             lst(nl) = fd
-            do a = 1, k + 1
+            do a = a0, k + 1
                 nl = nl + 1
                 lst(nl) = gs(a)
             end do
