@@ -264,12 +264,14 @@ def main():
         net = wsm.WarmStartNet.load(args.weights) if os.path.exists(args.weights) else None
         info = {}
 
-        def fast(use_net):
+        def fast(start):
             e32.synchronize()
             eng.synchronize()
             t0 = time.perf_counter()
-            if use_net:
+            if start == "net":
                 info.update(wsm.warm_start(e32, net, xi))
+            elif start == "boltzmann":
+                wsm.boltzmann_start(e32, xi)
             else:
                 wsm.cold_start(e32, xi)
             r = minimize(e32, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref,
@@ -278,9 +280,10 @@ def main():
             eng.synchronize()
             return time.perf_counter() - t0, r, float(np.sum(c))
 
-        t_cold32, res_c32, dg_c32 = fast(False)
+        t_cold32, res_c32, dg_c32 = fast("cold")
+        t_boltz32, res_b32, dg_b32 = fast("boltzmann")
         if net is not None:
-            t_fast, res_f, dg_fast = fast(True)
+            t_fast, res_f, dg_fast = fast("net")
         else:
             t_fast, res_f, dg_fast = t_cold32, res_c32, dg_c32
         # the fast path's evaluations: phase times of the FP32 engine (HIP events on its
@@ -306,7 +309,9 @@ def main():
             "ms_per_step": t_fast / max(res_f.evaluations, 1) * 1e3,
             "t_fast_s": t_fast, "t_plain_fp64_s": t_conv, "speedup_vs_plain": t_conv / t_fast,
             "t_plain_fp64_to_band_s": t_plain, "speedup_band_to_band": t_plain / t_fast,
-            "t_fp32_cold_start_s": t_cold32,
+            "t_fp32_cold_start_s": t_cold32, "t_fp32_boltzmann_start_s": t_boltz32,
+            "fp32_cold_start_evaluations": res_c32.evaluations, "fp32_boltzmann_start_evaluations": res_b32.evaluations,
+            "dG_fp32_boltzmann_error_kJmol": dg_b32 - dg_ref,
             "fast_path": ("learned warm start (network inference inside the timed region) + FP32 evaluations + "
                           "FP64 L-BFGS, FP64 confirmation") if net is not None else
                          "FP32 evaluations + FP64 L-BFGS from the reference cold start, FP64 confirmation "

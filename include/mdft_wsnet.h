@@ -79,7 +79,7 @@ int mdft_ws_gamma(struct mdft_ctx *ctx, const double *d_xi, double *d_gamma, flo
  * (device): 0 in the bulk, -1 where the solvent is excluded. */
 int mdft_ws_density(struct mdft_ctx *ctx, const double *d_xi, float *d_target);
 
-enum { MDFT_WS_GAMMA = 0, MDFT_WS_DENSITY = 1 };
+enum { MDFT_WS_GAMMA = 0, MDFT_WS_DENSITY = 1, MDFT_WS_BOLTZMANN = 2 };
 
 /* Starting density from predicted projections d_pred [2 np][N]:
  * kind MDFT_WS_DENSITY: d_pred are projections of rho/rho0 - 1 (see mdft_ws_density),
@@ -89,7 +89,9 @@ enum { MDFT_WS_GAMMA = 0, MDFT_WS_DENSITY = 1 };
  * kind MDFT_WS_GAMMA: d_pred are projections of gamma, d_xi = exp((-beta vext + gamma) / 2),
  * exponent clipped to [-200, 7].
  * With d_pred NULL: the reference's cold start (src/module_density.f90:267-284),
- * xi = 0 where vext >= (-ln eps - 0.05) kT, else 1.  All on the device. */
+ * xi = 0 where vext >= (-ln eps - 0.05) kT, else 1; or, kind MDFT_WS_BOLTZMANN, the
+ * Boltzmann factor capped at rho/rho0 = e^3, xi = exp(-max(beta vext, -3) / 2) (kind
+ * MDFT_WS_BOLTZMANN with a prediction is MDFT_EINVAL).  All on the device. */
 int mdft_ws_guess(struct mdft_ctx *ctx, const float *d_pred, int kind, double *d_xi);
 
 /* Gradient of F at a density guess with respect to the prediction it came from:

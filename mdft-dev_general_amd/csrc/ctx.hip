@@ -982,10 +982,12 @@ int mdft_ws_density(mdft_ctx *c, const double *d_xi, float *d_target) {
 int mdft_ws_guess(mdft_ctx *c, const float *d_pred, int kind, double *d_xi) {
     if (!c || !d_xi) return MDFT_EINVAL;
     if (!c->have_vext) return fail(c, MDFT_ESTATE, "vext not set");
+    if (kind != MDFT_WS_GAMMA && kind != MDFT_WS_DENSITY && kind != MDFT_WS_BOLTZMANN)
+        return fail(c, MDFT_EINVAL, "unknown guess kind");
     if (!d_pred) {
-        HIPCHK(c, launch_cold_start(c, d_xi));
+        HIPCHK(c, launch_cold_start(c, d_xi, kind == MDFT_WS_BOLTZMANN));
     } else {
-        if (kind != MDFT_WS_GAMMA && kind != MDFT_WS_DENSITY) return fail(c, MDFT_EINVAL, "unknown guess kind");
+        if (kind == MDFT_WS_BOLTZMANN) return fail(c, MDFT_EINVAL, "the Boltzmann start takes no prediction");
         HIPCHK(c, launch_channels_to_proj(c, d_pred));
         int nb = 0;
         HIPCHK(c, launch_final(c, nullptr, d_xi, kind == MDFT_WS_GAMMA ? 3 : 4, &nb));

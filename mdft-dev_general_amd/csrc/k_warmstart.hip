@@ -55,12 +55,16 @@ __global__ __launch_bounds__(kThreads) void channels_to_proj(const float *__rest
 }
 
 // reference cold start (src/module_density.f90:267-284):
-// xi = 0 where vext >= (-ln(DBL_EPSILON) - 0.05) kT, 1 elsewhere
+// xi = 0 where vext >= (-ln(DBL_EPSILON) - 0.05) kT, 1 elsewhere;
+// with `boltzmann` set: xi = exp(-max(beta vext, MDFT_WS_BOLTZMANN_MIN_BV) / 2), the density
+// of the solvent without its correlations, rho = rho0 exp(-beta vext), capped at e^3 rho0
 __global__ __launch_bounds__(kThreads) void cold_start(const double *__restrict__ vext, double *__restrict__ xi,
-                                                      size_t total, double beta) {
+                                                      size_t total, double beta, int boltzmann) {
     const double vmax = -log(DBL_EPSILON) - 0.05;
-    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads)
-        xi[g] = (beta * vext[g] >= vmax) ? 0.0 : 1.0;
+    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads) {
+        const double bv = beta * vext[g];
+        xi[g] = boltzmann ? exp(-0.5 * fmax(bv, MDFT_WS_BOLTZMANN_MIN_BV)) : (bv >= vmax ? 0.0 : 1.0);
+    }
 }
 
 // Trilinear regridding of a restart field (src/module_density.f90:171-226): new point
@@ -142,12 +146,12 @@ hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in) {
     return hipGetLastError();
 }
 
-hipError_t launch_cold_start(mdft_ctx *c, double *xi) {
+hipError_t launch_cold_start(mdft_ctx *c, double *xi, bool boltzmann) {
     const size_t total = c->N * (size_t)c->no;
     size_t nb = (total + kThreads - 1) / kThreads;
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(cold_start, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, c->d_vext, xi, total,
-                       1.0 / c->d.kT);
+                       1.0 / c->d.kT, boltzmann ? 1 : 0);
     return hipGetLastError();
 }
 

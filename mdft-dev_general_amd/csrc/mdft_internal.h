@@ -15,6 +15,10 @@
 // at xi = 0 both gradient terms vanish (they carry the factor 2 rho0 xi), so a point
 // predicted empty could never be refilled by the minimiser.
 #define MDFT_WS_DENSITY_FLOOR 1e-2
+// Lowest beta vext of the Boltzmann start, xi = exp(-max(beta vext, this) / 2): rho/rho0
+// stays <= e^3 (next to charged sites beta vext reaches -90, whose Boltzmann factor
+// overflows the FP32 excess term; the converged rho/rho0 rarely exceeds 10 there)
+#define MDFT_WS_BOLTZMANN_MIN_BV (-3.0)
 #define MDFT_OZ32_ROWS_PER_LANE 4  // OZ rows per lane in the FP32 q-space kernel (np <= 256)
 #define MDFT_FFT_MAX_STAGES 12
 
@@ -125,7 +129,7 @@ hipError_t launch_regrid(mdft_ctx *c, const double *prev, int pnx, int pny, cons
                          double outside, double *xi);
 hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale, const float *chscale = nullptr);
 hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in);
-hipError_t launch_cold_start(mdft_ctx *c, double *xi);
+hipError_t launch_cold_start(mdft_ctx *c, double *xi, bool boltzmann = false);
 hipError_t launch_density_mix(mdft_ctx *c, size_t total, double lam, const double *xa, const double *xb,
                               double *out);
 int bf16_tables(mdft_ctx *c, const std::vector<double> &a2p, const std::vector<double> &p2a,

@@ -55,7 +55,7 @@ WSNET_EXPORTS = (
     "mdft_wsnet_loss_grad", "mdft_wsnet_adam", "mdft_ws_features", "mdft_ws_gamma", "mdft_ws_density",
     "mdft_ws_guess", "mdft_ws_mix", "mdft_wsnet_backward", "mdft_ws_start_gradient",
 )
-WS_GAMMA, WS_DENSITY = 0, 1
+WS_GAMMA, WS_DENSITY, WS_BOLTZMANN = 0, 1, 2
 
 _bound = False
 
@@ -248,6 +248,12 @@ def guess(eng: Engine, pred: DeviceBuffer | None, xi: DeviceBuffer, kind: int = 
 def cold_start(eng: Engine, xi: DeviceBuffer) -> None:
     """xi <- the reference cold start (src/module_density.f90:267-284), on the device."""
     guess(eng, None, xi)
+
+
+def boltzmann_start(eng: Engine, xi: DeviceBuffer) -> None:
+    """xi <- exp(-max(beta vext, -3) / 2): the solvent density without correlations,
+    capped at e^3 rho0, on the device."""
+    guess(eng, None, xi, WS_BOLTZMANN)
 
 
 def mix(eng: Engine, lam: float, xa: DeviceBuffer, xb: DeviceBuffer, out: DeviceBuffer) -> None:

@@ -159,3 +159,28 @@ def test_pressure_and_lj_tail_corrections(oracle_mod):
                 sg = 0.5 * (sol.sig[a] + site.sig)
                 ref -= (16.0 * 3.14159 / 3.0) * e * sg ** 6 * 0.0333 / 10.0 ** 3
     assert abs(eng.lj_tail_correction(sol, solv, 10.0) - ref) < 1e-12 * abs(ref)
+
+
+def test_boltzmann_and_reference_starts_reach_one_minimum():
+    """The two cold starts -- the reference's xi in {0, 1} and the Boltzmann factor
+    exp(-beta vext / 2) -- lead the device L-BFGS to the same minimum; the
+    evaluations each needs are reported."""
+    from mdft import warmstart as ws
+    from mdft.gpu import Engine
+    from mdft.minimize import minimize
+    w = workloads.Workload("starts", 2, (24, 24, 24), (14.0, 14.0, 14.0), 3, 512)
+    say("[starts] reference / Boltzmann start, 24^3 mmax=3, charged solute")
+    eng = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0)
+    eng.set_ck(ck.synthetic_ck(3, 512, 0.05, seed=41, n0=w.n0))
+    sol = workloads.Solute(np.array([[7.0, 7.0, 7.0], [8.3, 7.2, 6.5], [6.1, 8.0, 7.4]]),
+                           np.array([0.6, -0.4, -0.2]), np.array([3.2, 2.9, 2.6]), np.array([0.6, 0.5, 0.4]))
+    eng.compute_vext(sol, w.solvent)
+    x = eng.alloc_field()
+    out = {}
+    for name, start in (("reference", ws.cold_start), ("boltzmann", ws.boltzmann_start)):
+        start(eng, x)
+        r = minimize(eng, x, m=5, gtol=1e-8, maxeval=3000)
+        out[name] = r
+        say(f"  {name}: F={r.F:.6f} after {r.evaluations} evaluations, |g|/|g0|={r.grad_ratio:.1e}")
+    fs = [r.F for r in out.values()]
+    assert max(fs) - min(fs) < 1e-4

@@ -169,6 +169,19 @@ def test_gamma_field_features_and_guess(oracle_mod):
     eng.set_vext(vext)
     ws.cold_start(eng, bx)
     assert np.array_equal(bx.download(), workloads.guess_xi(vext, kT))
+    # Boltzmann start: xi = exp(-max(beta vext, -3) / 2) (the device's exp against numpy's: a few ulp)
+    vext[::13] = -40.0                                 # strongly attractive: capped at rho/rho0 = e^3
+    eng.set_vext(vext)
+    ws.boltzmann_start(eng, bx)
+    ref = np.exp(-0.5 * np.maximum(vext * (1.0 / kT), -3.0))
+    err = np.abs(bx.download() - ref).max()
+    say(f"  Boltzmann start: max err {err:.2e}")
+    assert err < 1e-14 and np.all(bx.download() <= np.exp(1.5) * (1 + 1e-15))
+    from mdft.gpu import MdftError
+    with pytest.raises(MdftError):
+        ws.guess(eng, bp, bx, ws.WS_BOLTZMANN)
+    with pytest.raises(MdftError):
+        ws.guess(eng, None, bx, 7)
 
 
 def test_euler_lagrange_at_the_minimum():
