@@ -58,7 +58,7 @@ def phase_bytes(N, no, np_, b=16):
 # per-launch view of the phases: (kernel, launches per evaluation, kernel name in the traffic profile)
 KERNELS = {
     "angl2proj": ("angl2proj", 1, "angl2proj_t"),
-    "fft": ("fft_axis_c (one axis pass)", 6, "fft_axis_c"),
+    "fft": ("fft_axis_r (one axis pass)", 6, "fft_axis_r"),
     "oz": ("oz", 1, "oz"),
     "proj2angl_ideal_grad": ("final_t (proj2angl + ideal + gradient)", 1, "final_t"),
 }
