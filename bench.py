@@ -300,10 +300,20 @@ def main():
         # (src/module_pressure_correction.f90: F + PMV correction + LJ tail), outside the timed region
         pc = eng.pressure_correction(xi)
         lj = eng.lj_tail_correction(sol, w.solvent)
+        qc = eng.charged_solute_corrections(sol, w.solvent)   # 0 for a neutral solute
+        dq = qc["pbc_correction"] + qc["pscheme_correction"]
         line["dG_post_processing"] = {"F_kJmol": dg_fast, "pmv_correction": pc["pmv_correction"],
                                       "lj_tail_correction": lj, "pmv_A3": pc["pmv_A3"],
-                                      "P_bulk_atm": pc["P_bulk_kJmol_A3"] * 16394.078514951,
-                                      "estimated_solvation_free_energy": dg_fast + pc["pmv_correction"] + lj}
+                                      "P_bulk_atm": pc["P_bulk_kJmol_A3"] * 16394.078514951, **qc,
+                                      "estimated_solvation_free_energy": dg_fast + dq + pc["pmv_correction"] + lj}
+        # seconds-to-band roofline of the fast path: each FP32 evaluation at its fused
+        # algorithmic bytes and each L-BFGS iteration's 4m + 5 vector passes over n FP64
+        # values, all at the HBM peak
+        nvec = eng.N * eng.no
+        t_roof = (res_f.evaluations * algorithmic_bytes(eng.N, eng.no, eng.np_, 8)
+                  + res_f.iterations * (4 * args.memory + 5) * nvec * 8) / (HBM_PEAK_GBS * 1e9)
+        line["seconds_to_band_roofline"] = {"t_roofline_s": t_roof, "t_fast_s": t_fast, "frac": t_roof / t_fast,
+                                            "vector_passes_per_iteration": 4 * args.memory + 5}
         line.update({
             "value": t_fast, "unit": "s", "steps": res_f.evaluations,
             "ms_per_step": t_fast / max(res_f.evaluations, 1) * 1e3,

@@ -154,6 +154,17 @@ int mdft_pressure_correction(mdft_ctx *ctx, const double *xi, int flags, double 
  * sites as for mdft_compute_vext, rcut the LJ cutoff (A). */
 int mdft_lj_tail_correction(mdft_ctx *ctx, int natom, const double *atoms, int nsite, const double *sites,
                             double rcut, double *out);
+/* Hunenberger's corrections for a solute with a net charge Q (|Q| > 1e-7; else both 0),
+ * which the reference adds to F inside energy_and_gradient when the electrostatics are the
+ * lattice (Poisson) ones (src/module_energy_and_gradient.f90:190-203): out[0] the type B
+ * periodicity correction -1971.01 Q^2 / L (1 - 1 / eps_r), L the box side (the mean side
+ * of a box that is not cubic) (:287-315); out[1] the type C P-scheme correction
+ * -tr(Q_solvent) n0 2909.857 Q, tr(Q_solvent) = sum over axes of sum_s q_s r_s,i^2, each
+ * term at or below 1e-10 taken as 0 (:321-344, src/module_solvent.f90:323-326).  kJ/mol.
+ * lengths: the box (A); atoms and sites as for mdft_compute_vext.  Host arithmetic only:
+ * no context, no device. */
+int mdft_charged_solute_corrections(const double *lengths, double n0, int natom, const double *atoms, int nsite,
+                                    const double *sites, double relative_permittivity, double *out);
 
 /* FP64 vector algebra on device vectors of length n (16-byte aligned device
  * pointers), for a device-resident minimiser: the dot/axpy/copy/scal loops of

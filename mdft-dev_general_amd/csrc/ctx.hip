@@ -894,6 +894,33 @@ int mdft_lj_tail_correction(mdft_ctx *c, int natom, const double *atoms, int nsi
     return MDFT_OK;
 }
 
+int mdft_charged_solute_corrections(const double *lengths, double n0, int natom, const double *atoms, int nsite,
+                                    const double *sites, double relative_permittivity, double *out) {
+    if (!lengths || !atoms || !sites || !out || natom < 0 || nsite < 0) return MDFT_EINVAL;
+    out[0] = out[1] = 0.0;
+    double charge = 0.0;
+    for (int u = 0; u < natom; u++) charge += atoms[6 * u + 3];
+    // only a solute with a net charge is corrected (src/module_energy_and_gradient.f90:192-203)
+    if (fabs(charge) <= 1e-7) return MDFT_OK;
+    // type B, periodicity (:287-315): the lattice sum of a point charge in a cubic box of side L,
+    // screened by the solvent's permittivity; the mean side of a box that is not cubic
+    const bool cubic = lengths[1] == lengths[0] && lengths[2] == lengths[0];
+    const double L = cubic ? lengths[0] : (lengths[0] + lengths[1] + lengths[2]) / 3.0;
+    if (!(L > DBL_EPSILON) || !(relative_permittivity != 0.0)) return MDFT_EINVAL;
+    out[0] = -1971.01 * charge * charge / L * (1.0 - 1.0 / relative_permittivity);
+    // type C, P-scheme (:321-344): the trace of the solvent molecule's quadrupole
+    // sum_s q_s r_s r_s^T, each diagonal component set to 0 at or below 1e-10
+    // (src/module_solvent.f90:323-326, src/module_mathematica.f90:48-70)
+    double trace = 0.0;
+    for (int i = 0; i < 3; i++) {
+        double qii = 0.0;
+        for (int s = 0; s < nsite; s++) qii += sites[6 * s + 3] * sites[6 * s + i] * sites[6 * s + i];
+        trace += (fabs(qii) <= 1e-10) ? 0.0 : qii;
+    }
+    out[1] = -trace * n0 * 2909.857 * charge;
+    return MDFT_OK;
+}
+
 int mdft_phase_times(const mdft_ctx *c, double *ms, int n) {
     if (!c || !ms) return MDFT_EINVAL;
     const int k = std::min(n, c->nphase);

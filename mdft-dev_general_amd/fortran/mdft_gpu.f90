@@ -153,12 +153,21 @@ module mdft_gpu
             type(c_ptr), intent(in) :: vecs(*)
             real(c_double), value :: a
         end function
+        integer(c_int) function mdft_charged_solute_corrections(lengths, n0, natom, atoms, nsite, sites, eps_r, out) &
+                bind(C, name="mdft_charged_solute_corrections")
+            import :: c_int, c_double
+            real(c_double), intent(in) :: lengths(3), atoms(6, *), sites(6, *)
+            real(c_double), value :: n0, eps_r
+            integer(c_int), value :: natom, nsite
+            real(c_double), intent(out) :: out(2)
+        end function
     end interface
 
     public :: mdft_ctx_create, mdft_ctx_destroy, mdft_get_info, mdft_set_ck, mdft_compute_vext, mdft_compute_vext_ex, mdft_set_vext
     public :: mdft_energy_and_gradient, mdft_excess_energy, mdft_device_alloc, mdft_device_free
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
     public :: mdft_vec_multi_dot, mdft_vec_combine, mdft_vec_combine_step
+    public :: mdft_charged_solute_corrections
     public :: check, last_error_text
 
     ! ---- module state, the analogue of the reference's solvent(1)%xi and grid globals
@@ -167,8 +176,12 @@ module mdft_gpu
     type(c_ptr), public :: df_dev = c_null_ptr      ! scratch gradient on the device
     integer(c_size_t), public :: nvar = 0           ! no*nx*ny*nz
     real(dp), public :: f_id = 0, f_ext = 0, f_exc = 0, f_tot = 0
+    ! Hunenberger's corrections for a charged solute (module_energy_and_gradient.f90:21-26,190-203):
+    ! set by set_charged_solute_corrections, added to f while the flag is set
+    real(dp), public :: pbc_correction = 0, pscheme_correction = 0
+    logical, public :: apply_energy_corrections_due_to_charged_solute = .true.
 
-    public :: gpu_init, gpu_finalize, energy_and_gradient, excess_energy_gpu
+    public :: gpu_init, gpu_finalize, energy_and_gradient, excess_energy_gpu, set_charged_solute_corrections
 
 contains
 
@@ -242,7 +255,21 @@ contains
         f_ext = parts(2)
         f_exc = parts(3)
         f = parts(1) + parts(2) + parts(3)
+        if (apply_energy_corrections_due_to_charged_solute) f = f + pbc_correction + pscheme_correction
         f_tot = f
+    end subroutine
+
+    ! The type B and type C corrections of a solute (atoms(1:6, natom): x, y, z, q, sigma, eps)
+    ! in a solvent (sites(1:6, nsite)) for lattice electrostatics; a caller using the direct
+    ! Coulomb sum (the reference's direct_sum = T) leaves them at 0.
+    subroutine set_charged_solute_corrections(lengths, n0, atoms, sites, eps_r)
+        real(dp), intent(in) :: lengths(3), n0, atoms(:, :), sites(:, :), eps_r
+        real(dp) :: out(2)
+        call check(mdft_charged_solute_corrections(lengths, n0, int(size(atoms, 2), c_int), atoms, &
+                                                   int(size(sites, 2), c_int), sites, eps_r, out), &
+                   "charged_solute_corrections")
+        pbc_correction = out(1)
+        pscheme_correction = out(2)
     end subroutine
 
     ! excess_energy_t-conformant plug-in (module_proc_ptrs.f90:3-21): df accumulated.

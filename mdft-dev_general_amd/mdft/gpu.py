@@ -30,6 +30,7 @@ MDFT_EXPORTS = (
     "mdft_get_vext", "mdft_vec_dot", "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot",
     "mdft_vec_combine", "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj",
     "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_vec_combine_step",
+    "mdft_charged_solute_corrections",
 )
 
 
@@ -89,6 +90,7 @@ def load_library(path: str = LIBPATH):
     lib.mdft_vec_multi_dot.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(vp), vp, _dp]
     lib.mdft_pressure_correction.argtypes = [vp, vp, C.c_int, _dp]
     lib.mdft_lj_tail_correction.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double, _dp]
+    lib.mdft_charged_solute_corrections.argtypes = [_dp, C.c_double, C.c_int, _dp, C.c_int, _dp, C.c_double, _dp]
     lib.mdft_angl2proj.argtypes = [vp, vp, C.c_size_t, vp]
     lib.mdft_proj2angl.argtypes = [vp, vp, C.c_size_t, vp]
     lib.mdft_regrid_trilinear.argtypes = [vp, vp, _ip, _dp, C.c_double, vp]
@@ -100,7 +102,7 @@ def load_library(path: str = LIBPATH):
                  "mdft_memcpy_d2h", "mdft_synchronize", "mdft_compute_vext", "mdft_get_vext", "mdft_vec_dot",
                  "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine", "mdft_vec_combine_step",
                  "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj", "mdft_proj2angl",
-                 "mdft_regrid_trilinear", "mdft_compute_vext_ex"):
+                 "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_charged_solute_corrections"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib
@@ -235,6 +237,11 @@ class Engine:
                                                      float(rcut), _p(out)))
         return float(out[0])
 
+    def charged_solute_corrections(self, solute, solvent) -> dict:
+        """Type B (periodicity) and type C (P-scheme) corrections for a solute with a net
+        charge (src/module_energy_and_gradient.f90:190-203, 287-344), kJ/mol."""
+        return charged_solute_corrections(self.length, self.n0, solute, solvent)
+
     def get_vext(self):
         out = np.empty(self.N * self.no)
         self._check(self.lib.mdft_get_vext(self._h, out.ctypes.data, 0))
@@ -294,3 +301,20 @@ class Engine:
 
     def synchronize(self):
         self._check(self.lib.mdft_synchronize(self._h))
+
+
+def charged_solute_corrections(length, n0: float, solute, solvent) -> dict:
+    """Hunenberger's corrections that the reference adds to F for a solute with a net
+    charge under lattice electrostatics (src/module_energy_and_gradient.f90:190-203):
+    {"pbc_correction": type B, "pscheme_correction": type C}, kJ/mol, both 0 for a
+    neutral solute.  Host arithmetic in the library; needs no device."""
+    lib = load_library()
+    atoms = np.ascontiguousarray(np.column_stack([solute.xyz, solute.q, solute.sig, solute.eps]), dtype=np.float64)
+    sites = np.ascontiguousarray([[*s.r, s.q, s.sig, s.eps] for s in solvent.sites], dtype=np.float64)
+    box = np.ascontiguousarray(length, dtype=np.float64)
+    out = np.zeros(2)
+    rc = lib.mdft_charged_solute_corrections(_p(box), float(n0), len(atoms), _p(atoms), len(sites), _p(sites),
+                                             float(solvent.relative_permittivity), _p(out))
+    if rc != 0:
+        raise MdftError(rc, "mdft_charged_solute_corrections: invalid box or permittivity")
+    return {"pbc_correction": float(out[0]), "pscheme_correction": float(out[1])}

@@ -170,3 +170,12 @@ int mdft_vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const dou
     }
     return 0;
 }
+
+/* present for the link of the Fortran module only; the host tests do not call it */
+int mdft_charged_solute_corrections(const double *lengths, double n0, int natom, const double *atoms, int nsite,
+                                    const double *sites, double relative_permittivity, double *out)
+{
+    (void)lengths; (void)n0; (void)natom; (void)atoms; (void)nsite; (void)sites; (void)relative_permittivity;
+    out[0] = out[1] = 0.0;
+    return 0;
+}
