@@ -48,7 +48,10 @@ enum { MDFT_PREC_FP64 = 0, MDFT_PREC_FP32 = 1, MDFT_PREC_BF16 = 2 };
 /* flags for evaluation calls */
 enum {
     MDFT_DEVICE_PTRS = 1, /* xi / df / vext arguments are device pointers */
-    MDFT_TIMERS = 2       /* record per-phase HIP event timings (mdft_phase_times) */
+    MDFT_TIMERS = 2,      /* record per-phase HIP event timings (mdft_phase_times) */
+    MDFT_DF_FP32 = 4      /* mdft_energy_and_gradient only, with MDFT_DEVICE_PTRS and molecular
+                             symmetry order 2, FP64 or FP32 contexts: df is a float array that
+                             receives dF rounded to FP32 (the minimiser's FP32 vector storage) */
 };
 
 typedef struct mdft_desc {
@@ -185,6 +188,20 @@ int mdft_vec_combine(mdft_ctx *ctx, size_t n, int k, const double *coef, const d
  * not alias x, z or any vecs[j]. */
 int mdft_vec_combine_step(mdft_ctx *ctx, size_t n, int k, const double *coef, const double *const *vecs, double *z,
                           double a, const double *x, double *y);
+
+/* The same passes with the minimiser's directions and gradients stored in FP32 (half
+ * the bytes; FP64 accumulation; the iterate x, y stays FP64).  No reference
+ * counterpart: the reference's L-BFGS vectors are FP64.  Device pointers, 16-byte
+ * aligned. */
+/* out[j] = x.vecs[j], FP64 sums of FP32 products (out on the host). */
+int mdft_vec_multi_dot_f32(mdft_ctx *ctx, size_t n, int k, const float *const *vecs, const float *x, double *out);
+/* z = (float) sum_j coef[j] vecs[j] and y = x + a z with z as stored. */
+int mdft_vec_combine_step_f32(mdft_ctx *ctx, size_t n, int k, const double *coef, const float *const *vecs, float *z,
+                              double a, const double *x, double *y);
+/* y = x + a z (z FP32; y may alias x). */
+int mdft_vec_axpy_f32(mdft_ctx *ctx, size_t n, double a, const float *z, const double *x, double *y);
+/* out = (double) in. */
+int mdft_vec_widen_f32(mdft_ctx *ctx, size_t n, const float *in, double *out);
 
 /* ---- restart (density.bin) support, all device pointers ----
  * angl2proj / proj2angl of src/module_orientation_projection_transform.f90:159-199 and

@@ -620,10 +620,14 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     if (!xi) return fail(c, MDFT_EINVAL, "xi is null");
     const bool dev = flags & MDFT_DEVICE_PTRS;
     const bool timers = flags & MDFT_TIMERS;
+    const bool df32 = flags & MDFT_DF_FP32;
+    if (df32 && (!dev || mode != 0 || !df || c->d.mrso != 2 || c->d.precision == MDFT_PREC_BF16))
+        return fail(c, MDFT_EINVAL, "MDFT_DF_FP32 needs device pointers, a gradient, molecular symmetry order 2 "
+                                    "and an FP64 or FP32 context");
     if (timers && (rc = ensure_events(c))) return rc;
     const size_t nbytes = sizeof(double) * c->N * c->no;
     const double *xd = xi;
-    double *dd = df;
+    double *dd = df32 ? nullptr : df;
     if (!dev) {
         HIPCHK(c, ensure_stage(c, &c->d_xi_stage));
         HIPCHK(c, hipMemcpyAsync(c->d_xi_stage, xi, nbytes, hipMemcpyHostToDevice, c->stream));
@@ -638,7 +642,10 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     rc = run_excess_pipeline(c, xd, 4, timers);
     if (rc) return rc;
     int nb = 0;
-    HIPCHK(c, launch_final(c, xd, dd, mode, &nb));
+    c->df32 = df32 ? reinterpret_cast<float *>(df) : nullptr;
+    const hipError_t fe = launch_final(c, xd, dd, mode, &nb);
+    c->df32 = nullptr;
+    HIPCHK(c, fe);
     if (timers) HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     rc = read_sums(c, nb, sums);
     if (rc) return rc;
@@ -826,6 +833,39 @@ int mdft_vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, cons
     }
     if (y == x || y == z) return fail(c, MDFT_EINVAL, "y must not alias x or z");
     HIPCHK(c, vec_combine_step(c, n, k, coef, vecs, z, a, x, y));
+    return MDFT_OK;
+}
+
+int mdft_vec_multi_dot_f32(mdft_ctx *c, size_t n, int k, const float *const *vecs, const float *x, double *out) {
+    if (!c || !vecs || !x || !out || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    if (!aligned16(x)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    for (int j = 0; j < k; j++)
+        if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "null or unaligned vector");
+    HIPCHK(c, vec_multi_dot32(c, n, k, vecs, x, out));
+    return MDFT_OK;
+}
+
+int mdft_vec_combine_step_f32(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs, float *z,
+                              double a, const double *x, double *y) {
+    if (!c || !coef || !vecs || !z || !x || !y || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    if (!aligned16(z) || !aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    for (int j = 0; j < k; j++)
+        if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    if (y == x) return fail(c, MDFT_EINVAL, "y must not alias x");
+    HIPCHK(c, vec_combine_step32(c, n, k, coef, vecs, z, a, x, y));
+    return MDFT_OK;
+}
+
+int mdft_vec_axpy_f32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y) {
+    if (!c || !z || !x || !y) return MDFT_EINVAL;
+    if (!aligned16(z) || !aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    HIPCHK(c, vec_axpy32(c, n, a, z, x, y));
+    return MDFT_OK;
+}
+
+int mdft_vec_widen_f32(mdft_ctx *c, size_t n, const float *in, double *out) {
+    if (!c || !in || !out) return MDFT_EINVAL;
+    HIPCHK(c, vec_widen(c, n, in, out));
     return MDFT_OK;
 }
 

@@ -421,7 +421,7 @@ template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
-                                                   FinalConsts k, int mode) {
+                                                   FinalConsts k, int mode, float *__restrict__ df32) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -540,7 +540,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
             } else {
                 fid += kT * dv * wio * rho0;
             }
-            if (df) df[g] = gid + 2.0 * rho0 * x * vexc;
+            const double gv = gid + 2.0 * rho0 * x * vexc;
+            if (df32) df32[g] = (float)gv;
+            else if (df) df[g] = gv;
         } else if (mode == 1) {
             if (df) df[g] += 2.0 * rho0 * x * vexc;
         } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
@@ -602,7 +604,7 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
     const size_t nb = (npts + PT - 1) / PT;
     switch (c->L) {
 #define FIN_CASE(LL) \
-    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode); return true;
+    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr); return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
         default: return false;

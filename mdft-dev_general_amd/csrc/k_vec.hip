@@ -239,6 +239,106 @@ __global__ __launch_bounds__(kThreads) void combine_step(size_t n, VecSet v, dou
     }
 }
 
+// ---- FP32 storage of the minimiser's directions and gradients (the mixed-precision
+// path): the same passes over half the bytes, FP64 accumulation and an FP64 iterate.
+// 16-byte accesses (four floats); n % 4 tail elements handled by one thread.
+struct VecSet32 {
+    const float *p[MDFT_VEC_MAX];
+    double c[MDFT_VEC_MAX];
+    int k;
+    int self;
+};
+
+__global__ __launch_bounds__(kThreads) void multi_dot_partial32(size_t n, const float *__restrict__ x, VecSet32 v,
+                                                               double *__restrict__ part) {
+    double acc[MDFT_VEC_MAX];
+#pragma unroll
+    for (int j = 0; j < MDFT_VEC_MAX; j++) acc[j] = 0.0;
+    const size_t n4 = n / 4;
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
+        const float4 xv = x4[i];
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) {
+                const float4 w = ((v.self >> j) & 1) ? xv : reinterpret_cast<const float4 *>(v.p[j])[i];
+                acc[j] += (double)xv.x * w.x + (double)xv.y * w.y + (double)xv.z * w.z + (double)xv.w * w.w;
+            }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (size_t t = 4 * n4; t < n; t++)
+#pragma unroll
+            for (int j = 0; j < MDFT_VEC_MAX; j++)
+                if (j < v.k) acc[j] += (double)x[t] * v.p[j][t];
+    __shared__ double red[MDFT_VEC_MAX][kThreads / 64];
+#pragma unroll
+    for (int j = 0; j < MDFT_VEC_MAX; j++) {
+        if (j >= v.k) break;
+        const double s = wave_sum(acc[j]);
+        if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < v.k) {
+        double s = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) s += red[threadIdx.x][w];
+        part[(size_t)blockIdx.x * MDFT_VEC_MAX + threadIdx.x] = s;
+    }
+}
+
+// z = (float) sum_j c_j v_j and y = x + a z (with z as stored, so that y - x is exactly
+// a times the stored direction); y must not alias x, z or a v_j
+__global__ __launch_bounds__(kThreads) void combine_step32(size_t n, VecSet32 v, float *z, double a,
+                                                          const double *__restrict__ x, double *__restrict__ y) {
+    const size_t n4 = n / 4;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
+        const double2 x0 = reinterpret_cast<const double2 *>(x)[2 * i];
+        const double2 x1 = reinterpret_cast<const double2 *>(x)[2 * i + 1];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) {
+                const float4 w = reinterpret_cast<const float4 *>(v.p[j])[i];
+                s0 += v.c[j] * w.x;
+                s1 += v.c[j] * w.y;
+                s2 += v.c[j] * w.z;
+                s3 += v.c[j] * w.w;
+            }
+        const float4 zs = make_float4((float)s0, (float)s1, (float)s2, (float)s3);
+        reinterpret_cast<float4 *>(z)[i] = zs;
+        reinterpret_cast<double2 *>(y)[2 * i] = make_double2(x0.x + a * zs.x, x0.y + a * zs.y);
+        reinterpret_cast<double2 *>(y)[2 * i + 1] = make_double2(x1.x + a * zs.z, x1.y + a * zs.w);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (size_t t = 4 * n4; t < n; t++) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < MDFT_VEC_MAX; j++)
+                if (j < v.k) s += v.c[j] * v.p[j][t];
+            z[t] = (float)s;
+            y[t] = x[t] + a * (double)z[t];
+        }
+}
+
+// y = x + a z, z FP32 (a line-search trial point along a stored direction)
+__global__ __launch_bounds__(kThreads) void axpy32(size_t n, double a, const float *__restrict__ z,
+                                                  const double *__restrict__ x, double *__restrict__ y) {
+    const size_t n4 = n / 4;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
+        const float4 w = reinterpret_cast<const float4 *>(z)[i];
+        const double2 x0 = reinterpret_cast<const double2 *>(x)[2 * i];
+        const double2 x1 = reinterpret_cast<const double2 *>(x)[2 * i + 1];
+        reinterpret_cast<double2 *>(y)[2 * i] = make_double2(x0.x + a * w.x, x0.y + a * w.y);
+        reinterpret_cast<double2 *>(y)[2 * i + 1] = make_double2(x1.x + a * w.z, x1.y + a * w.w);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (size_t t = 4 * n4; t < n; t++) y[t] = x[t] + a * (double)z[t];
+}
+
+__global__ __launch_bounds__(kThreads) void widen1(size_t n, const float *__restrict__ in, double *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        out[i] = (double)in[i];
+}
+
 // Per grid point (one wave each): n(r) = rho0 sum_io w_io xi^2; block partials of
 // sum_r n(r) and of the number of points with n(r) / n0 < cav (cavity points).
 __global__ __launch_bounds__(kThreads) void density_integrals(const double *__restrict__ xi,
@@ -376,6 +476,56 @@ hipError_t vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, co
     }
     v.k = k;
     hipLaunchKernelGGL(combine_step, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_multi_dot32(mdft_ctx *c, size_t n, int k, const float *const *vecs, const float *x, double *out) {
+    if (!c->d_mpart) {
+        hipError_t e = hipMalloc(&c->d_mpart, sizeof(double) * ((size_t)kMaxBlocks * MDFT_VEC_MAX + MDFT_VEC_MAX));
+        if (e != hipSuccess) return e;
+    }
+    VecSet32 v{};
+    for (int j = 0; j < k; j++) {
+        v.p[j] = vecs[j];
+        if (vecs[j] == x) v.self |= 1 << j;
+    }
+    v.k = k;
+    const unsigned nb = blocks_for(n / 2);
+    hipLaunchKernelGGL(multi_dot_partial32, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    double *dout = c->d_mpart + (size_t)kMaxBlocks * MDFT_VEC_MAX;
+    hipLaunchKernelGGL(multi_dot_final, dim3(1), dim3(kThreads), 0, c->stream, c->d_mpart, (int)nb, k, dout);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(out, dout, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs, float *z,
+                              double a, const double *x, double *y) {
+    VecSet32 v{};
+    for (int j = 0; j < k; j++) {
+        v.p[j] = vecs[j];
+        v.c[j] = coef[j];
+    }
+    v.k = k;
+    hipLaunchKernelGGL(combine_step32, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_axpy32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y) {
+    hipLaunchKernelGGL(axpy32, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, a, z, x, y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_widen(mdft_ctx *c, size_t n, const float *in, double *out) {
+    hipLaunchKernelGGL(widen1, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);

@@ -15,7 +15,7 @@ module mdft_gpu
 
     integer, parameter :: dp = c_double
     integer(c_int), parameter, public :: MDFT_PREC_FP64 = 0, MDFT_PREC_FP32 = 1, MDFT_PREC_BF16 = 2
-    integer(c_int), parameter, public :: MDFT_DEVICE_PTRS = 1, MDFT_TIMERS = 2
+    integer(c_int), parameter, public :: MDFT_DEVICE_PTRS = 1, MDFT_TIMERS = 2, MDFT_DF_FP32 = 4
 
     type, bind(C), public :: mdft_desc
         integer(c_int) :: nx, ny, nz
@@ -153,6 +153,36 @@ module mdft_gpu
             type(c_ptr), intent(in) :: vecs(*)
             real(c_double), value :: a
         end function
+        ! the same passes over FP32-stored directions and gradients (FP64 iterate x, y)
+        integer(c_int) function mdft_vec_multi_dot_f32(ctx, n, k, vecs, x, out) bind(C, name="mdft_vec_multi_dot_f32")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, x
+            integer(c_size_t), value :: n
+            integer(c_int), value :: k
+            type(c_ptr), intent(in) :: vecs(*)
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function mdft_vec_combine_step_f32(ctx, n, k, coef, vecs, z, a, x, y) &
+                bind(C, name="mdft_vec_combine_step_f32")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, z, x, y
+            integer(c_size_t), value :: n
+            integer(c_int), value :: k
+            real(c_double), intent(in) :: coef(*)
+            type(c_ptr), intent(in) :: vecs(*)
+            real(c_double), value :: a
+        end function
+        integer(c_int) function mdft_vec_axpy_f32(ctx, n, a, z, x, y) bind(C, name="mdft_vec_axpy_f32")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, z, x, y
+            integer(c_size_t), value :: n
+            real(c_double), value :: a
+        end function
+        integer(c_int) function mdft_vec_widen_f32(ctx, n, in, out) bind(C, name="mdft_vec_widen_f32")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: ctx, in, out
+            integer(c_size_t), value :: n
+        end function
         integer(c_int) function mdft_charged_solute_corrections(lengths, n0, natom, atoms, nsite, sites, eps_r, out) &
                 bind(C, name="mdft_charged_solute_corrections")
             import :: c_int, c_double
@@ -167,6 +197,7 @@ module mdft_gpu
     public :: mdft_energy_and_gradient, mdft_excess_energy, mdft_device_alloc, mdft_device_free
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
     public :: mdft_vec_multi_dot, mdft_vec_combine, mdft_vec_combine_step
+    public :: mdft_vec_multi_dot_f32, mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32
     public :: mdft_charged_solute_corrections
     public :: check, last_error_text
 

@@ -19,15 +19,29 @@
 module mdft_lbfgs
     use iso_c_binding
     use mdft_gpu, only: mdft_energy_and_gradient, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
-                        mdft_vec_multi_dot, mdft_vec_combine_step, MDFT_DEVICE_PTRS
+                        mdft_vec_multi_dot, mdft_vec_combine_step, mdft_vec_multi_dot_f32, &
+                        mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32, &
+                        MDFT_DEVICE_PTRS, MDFT_DF_FP32
     implicit none
     private
     integer, parameter :: dp = c_double
     integer, parameter :: mmem_max = 6           ! 2 m + 3 live vectors per multi-dot <= 16
     integer, parameter :: nslot = 2 * mmem_max + 3
-    public :: mdft_minimize
+    public :: mdft_minimize, mdft_minimize_ex
 
 contains
+
+    ! mdft_minimize_ex with FP64 directions and gradients (the reference's precision)
+    integer(c_int) function mdft_minimize(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, &
+                                          stats, trace, ntrace) bind(C, name="mdft_minimize") result(rc)
+        type(c_ptr), value :: ctx, xi, g
+        integer(c_size_t), value :: n
+        integer(c_int), value :: m, maxeval, ntrace
+        real(c_double), value :: gtol, ftol, fnoise, ftarget, fband
+        real(c_double), intent(out) :: stats(16)
+        real(c_double), intent(out) :: trace(3, *)
+        rc = mdft_minimize_ex(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, stats, trace, ntrace, 0)
+    end function
 
     ! Minimise F over xi (device vector of length n), starting from the values in xi.
     ! On return xi holds the last accepted iterate and g its gradient.
@@ -42,11 +56,15 @@ contains
     ! fnoise*|F| so that steps are judged by the curvature condition once F stops
     ! resolving the decrease (the evaluation's single-precision chops put ~1e-8
     ! relative noise on F in FP64, more in the mixed-precision modes)
-    integer(c_int) function mdft_minimize(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, &
-                                          stats, trace, ntrace) bind(C, name="mdft_minimize") result(rc)
+    ! vec32 /= 0: the stored directions and gradients are FP32 (half the bytes of every
+    ! vector pass; FP64 dot products, iterate and Gram matrix) -- for the mixed-precision
+    ! evaluations, whose gradients carry FP32-level errors anyway.  g still receives the
+    ! final gradient in FP64.
+    integer(c_int) function mdft_minimize_ex(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, &
+                                             stats, trace, ntrace, vec32) bind(C, name="mdft_minimize_ex") result(rc)
         type(c_ptr), value :: ctx, xi, g
         integer(c_size_t), value :: n
-        integer(c_int), value :: m, maxeval, ntrace
+        integer(c_int), value :: m, maxeval, ntrace, vec32
         real(c_double), value :: gtol, ftol, fnoise, ftarget, fband
         real(c_double), intent(out) :: stats(16)
         real(c_double), intent(out) :: trace(3, *)
@@ -62,12 +80,16 @@ contains
         real(dp) :: t_ready   ! step for which xn already holds x - t d (negative: none)
         integer :: i, j, k, mm, nb, nl, fd, fg, gcur, neval, iter, status
         integer(c_size_t) :: bytes
-        logical :: ok
+        integer(c_int) :: eflags
+        logical :: ok, f32
 
         rc = 0
         stats = 0
         mm = max(1, min(int(m), mmem_max))
-        bytes = 8_c_size_t * n
+        f32 = vec32 /= 0
+        bytes = merge(4_c_size_t, 8_c_size_t, f32) * n
+        eflags = MDFT_DEVICE_PTRS
+        if (f32) eflags = MDFT_DEVICE_PTRS + MDFT_DF_FP32
         owned = .false.
         xn_own = c_null_ptr
         t_ready = -1.0_dp
@@ -75,14 +97,14 @@ contains
             vec(i) = c_null_ptr
         end do
         do i = 1, 2 * mm + 3
-            if (i == mm + 2) then
+            if (i == mm + 2 .and. .not. f32) then
                 vec(i) = g
             else
                 rc = mdft_device_alloc(ctx, bytes, vec(i)); if (rc /= 0) goto 900
                 owned(i) = .true.
             end if
         end do
-        rc = mdft_device_alloc(ctx, bytes, xn_own); if (rc /= 0) goto 900
+        rc = mdft_device_alloc(ctx, 8_c_size_t * n, xn_own); if (rc /= 0) goto 900
         x = xi
         xn = xn_own
         gram = 0.0_dp
@@ -92,11 +114,11 @@ contains
         iter = 0
         status = 2
         gcur = mm + 2
-        rc = mdft_energy_and_gradient(ctx, x, parts, vec(gcur), MDFT_DEVICE_PTRS); if (rc /= 0) goto 900
+        rc = mdft_energy_and_gradient(ctx, x, parts, vec(gcur), eflags); if (rc /= 0) goto 900
         neval = 1
         f = sum(parts)
         pv(1) = vec(gcur)
-        rc = mdft_vec_multi_dot(ctx, n, 1, pv, vec(gcur), dots); if (rc /= 0) goto 900
+        rc = multi_dot(1); if (rc /= 0) goto 900
         gram(gcur, gcur) = dots(1)
         g0 = sqrt(dots(1))
         gnorm = g0
@@ -155,7 +177,12 @@ contains
                 end if
             end do
             ! the direction and the line search's first trial point in one pass
-            rc = mdft_vec_combine_step(ctx, n, nb, coef, pv, vec(fd), -t, x, xn); if (rc /= 0) goto 900
+            if (f32) then
+                rc = mdft_vec_combine_step_f32(ctx, n, nb, coef, pv, vec(fd), -t, x, xn)
+            else
+                rc = mdft_vec_combine_step(ctx, n, nb, coef, pv, vec(fd), -t, x, xn)
+            end if
+            if (rc /= 0) goto 900
             t_ready = t
             do j = 1, 2 * mm + 3
                 if (j == fd) cycle
@@ -225,11 +252,24 @@ contains
 900     continue
         ! the caller's xi/g buffers must hold the result
         if (rc == 0 .and. .not. c_associated(x, xi)) rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, 0.0_dp, c_null_ptr, xi)
-        if (rc == 0 .and. .not. c_associated(vec(gcur), g)) &
+        if (rc == 0 .and. f32 .and. c_associated(vec(gcur))) then
+            rc = mdft_vec_widen_f32(ctx, n, vec(gcur), g)
+        else if (rc == 0 .and. .not. c_associated(vec(gcur), g)) then
             rc = mdft_vec_lincomb(ctx, n, 1.0_dp, vec(gcur), 0.0_dp, c_null_ptr, g)
+        end if
         call release()
 
     contains
+
+        ! dots(1:nl_) = pv(j) . pv(nl_): the last vector of pv is the new gradient itself
+        integer(c_int) function multi_dot(nl_) result(r)
+            integer, intent(in) :: nl_
+            if (f32) then
+                r = mdft_vec_multi_dot_f32(ctx, n, int(nl_, c_int), pv, pv(nl_), dots)
+            else
+                r = mdft_vec_multi_dot(ctx, n, int(nl_, c_int), pv, pv(nl_), dots)
+            end if
+        end function
 
         ! a^T G b over the slot coefficients
         real(dp) function gdot(a, b)
@@ -281,10 +321,15 @@ contains
         subroutine trial()
             integer :: a, a0
             if (t /= t_ready) then
-                rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, -t, vec(fd), xn); if (rc /= 0) return
+                if (f32) then
+                    rc = mdft_vec_axpy_f32(ctx, n, -t, vec(fd), x, xn)
+                else
+                    rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, -t, vec(fd), xn)
+                end if
+                if (rc /= 0) return
             end if
             t_ready = -1.0_dp
-            rc = mdft_energy_and_gradient(ctx, xn, pn, vec(fg), MDFT_DEVICE_PTRS); if (rc /= 0) return
+            rc = mdft_energy_and_gradient(ctx, xn, pn, vec(fg), eflags); if (rc /= 0) return
             neval = neval + 1
             fn = sum(pn)
             ! with the memory full, the oldest pair (ds(1), gs(1)) is dropped after this
@@ -308,7 +353,7 @@ contains
             do a = 1, nl
                 pv(a) = vec(lst(a))
             end do
-            rc = mdft_vec_multi_dot(ctx, n, nl, pv, vec(fg), dots); if (rc /= 0) return
+            rc = multi_dot(nl); if (rc /= 0) return
             do a = 1, nl
                 gram(fg, lst(a)) = dots(a)
                 gram(lst(a), fg) = dots(a)
@@ -392,6 +437,6 @@ contains
             if (c_associated(xn_own)) r2 = mdft_device_free(ctx, xn_own)
         end subroutine
 
-    end function mdft_minimize
+    end function mdft_minimize_ex
 
 end module mdft_lbfgs

@@ -32,6 +32,8 @@ def host_library():
         lib.mdft_minimize.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int]
         lib.mdft_minimize.restype = C.c_int
+        lib.mdft_minimize_ex.argtypes = lib.mdft_minimize.argtypes + [C.c_int]
+        lib.mdft_minimize_ex.restype = C.c_int
         _HOST = lib
     return _HOST
 
@@ -54,20 +56,22 @@ class MinResult:
 
 def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: int = 5, gtol: float = 1e-6,
              ftol: float = 0.0, fnoise: float = 1e-7, maxeval: int = 2000, ntrace: int = 2000,
-             ftarget: float = 0.0, fband: float = 0.0, lib=None) -> MinResult:
+             ftarget: float = 0.0, fband: float = 0.0, fp32_vectors: bool = False, lib=None) -> MinResult:
     """Minimise F over xi (in place).  Returns the final F and statistics.
     fnoise is the relative noise level of F used to relax the sufficient-decrease
-    test; with fband > 0 the run stops at the first iterate with |F - ftarget| <= fband."""
+    test; with fband > 0 the run stops at the first iterate with |F - ftarget| <= fband.
+    fp32_vectors stores the minimiser's directions and gradients in FP32 (half the
+    bytes per vector pass; meant for FP32 evaluations); g still receives FP64."""
     lib = lib or host_library()
     own = g is None
     if own:
         g = eng.alloc_field()
     stats = np.zeros(16)
     trace = np.zeros((max(ntrace, 1), 3))
-    rc = lib.mdft_minimize(eng._h, eng.N * eng.no, xi.ptr, g.ptr, int(m), float(gtol), float(ftol), float(fnoise),
-                           float(ftarget), float(fband), int(maxeval),
-                           stats.ctypes.data_as(C.POINTER(C.c_double)),
-                           trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace))
+    rc = lib.mdft_minimize_ex(eng._h, eng.N * eng.no, xi.ptr, g.ptr, int(m), float(gtol), float(ftol),
+                              float(fnoise), float(ftarget), float(fband), int(maxeval),
+                              stats.ctypes.data_as(C.POINTER(C.c_double)),
+                              trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace), int(bool(fp32_vectors)))
     if rc != 0:
         raise MdftError(rc, eng.lib.mdft_last_error(eng._h).decode())
     it = int(stats[6])

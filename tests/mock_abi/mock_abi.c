@@ -82,7 +82,13 @@ int mdft_set_vext(mdft_ctx *c, const double *vext, int flags)
 
 int mdft_energy_and_gradient(mdft_ctx *c, const double *xi, double *parts, double *df, int flags)
 {
-    (void)flags;
+    if ((flags & MDFT_DF_FP32) && df) {
+        double *tmp = malloc(sizeof(double) * c->nvar);
+        or_energy_and_gradient(c->o, xi, c->vext, parts, tmp);
+        for (size_t i = 0; i < c->nvar; i++) ((float *)df)[i] = (float)tmp[i];
+        free(tmp);
+        return 0;
+    }
     or_energy_and_gradient(c->o, xi, c->vext, parts, df);
     return 0;
 }
@@ -177,5 +183,44 @@ int mdft_charged_solute_corrections(const double *lengths, double n0, int natom,
 {
     (void)lengths; (void)n0; (void)natom; (void)atoms; (void)nsite; (void)sites; (void)relative_permittivity;
     out[0] = out[1] = 0.0;
+    return 0;
+}
+
+/* FP32-stored directions and gradients */
+int mdft_vec_multi_dot_f32(mdft_ctx *c, size_t n, int k, const float *const *vecs, const float *x, double *out)
+{
+    (void)c;
+    for (int j = 0; j < k; j++) {
+        double s = 0.0;
+        for (size_t i = 0; i < n; i++) s += (double)x[i] * vecs[j][i];
+        out[j] = s;
+    }
+    return 0;
+}
+
+int mdft_vec_combine_step_f32(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs, float *z,
+                              double a, const double *x, double *y)
+{
+    (void)c;
+    for (size_t i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < k; j++) s += coef[j] * vecs[j][i];
+        z[i] = (float)s;
+        y[i] = x[i] + a * (double)z[i];
+    }
+    return 0;
+}
+
+int mdft_vec_axpy_f32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y)
+{
+    (void)c;
+    for (size_t i = 0; i < n; i++) y[i] = x[i] + a * (double)z[i];
+    return 0;
+}
+
+int mdft_vec_widen_f32(mdft_ctx *c, size_t n, const float *in, double *out)
+{
+    (void)c;
+    for (size_t i = 0; i < n; i++) out[i] = (double)in[i];
     return 0;
 }

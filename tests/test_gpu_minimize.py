@@ -184,3 +184,103 @@ def test_boltzmann_and_reference_starts_reach_one_minimum():
         say(f"  {name}: F={r.F:.6f} after {r.evaluations} evaluations, |g|/|g0|={r.grad_ratio:.1e}")
     fs = [r.F for r in out.values()]
     assert max(fs) - min(fs) < 1e-4
+
+
+def test_fp32_vector_ops():
+    """The FP32-storage vector passes against numpy: n with a ragged tail (n % 4 = 3),
+    FP64 sums of the FP32 products, the stored direction rounded to FP32 and the
+    trial point built from it exactly, and the widening copy."""
+    import ctypes as C
+    from mdft.gpu import DeviceBuffer, Engine
+    eng = Engine((8, 8, 8), (8.0, 8.0, 8.0), 1, 2, 2.4, 0.0333)
+    rng = np.random.default_rng(3)
+    n = 4 * 8192 + 3
+    say(f"[vec32] n={n}")
+    dp = C.POINTER(C.c_double)
+    k = 5
+    vs = [rng.standard_normal(n).astype(np.float32) for _ in range(k)]
+    bufs = []
+    for v in vs:
+        bb = DeviceBuffer(eng, 4 * n)
+        bb.upload(v)
+        bufs.append(bb)
+    xs = vs[2]                                  # x is one of the vectors: loaded once
+    ptrs = (C.c_void_p * k)(*[bb.ptr.value for bb in bufs])
+    dots = np.zeros(k)
+    eng._check(eng.lib.mdft_vec_multi_dot_f32(eng._h, n, k, ptrs, bufs[2].ptr, dots.ctypes.data_as(dp)))
+    ref = np.array([xs.astype(np.float64) @ v.astype(np.float64) for v in vs])
+    assert np.allclose(dots, ref, rtol=1e-12, atol=1e-9)
+    coef = rng.standard_normal(k)
+    x64 = rng.standard_normal(n)
+    dx, dy, dz = DeviceBuffer(eng, 8 * n), DeviceBuffer(eng, 8 * n), DeviceBuffer(eng, 4 * n)
+    dx.upload(x64)
+    eng._check(eng.lib.mdft_vec_combine_step_f32(eng._h, n, k, coef.ctypes.data_as(dp), ptrs, dz.ptr, -0.37, dx.ptr,
+                                                 dy.ptr))
+    z = sum(c * v.astype(np.float64) for c, v in zip(coef, vs))
+    zd = dz.download(np.float32)
+    assert np.abs(zd.astype(np.float64) - z).max() <= 1e-6 * np.abs(z).max()
+    # y is built from the stored (rounded) z; the device may fuse the multiply-add (1 ulp)
+    ref = x64 - 0.37 * zd.astype(np.float64)
+    assert np.abs(dy.download() - ref).max() <= 1e-15 * np.abs(ref).max()
+    eng._check(eng.lib.mdft_vec_axpy_f32(eng._h, n, 0.25, dz.ptr, dx.ptr, dy.ptr))
+    ref = x64 + 0.25 * zd.astype(np.float64)
+    assert np.abs(dy.download() - ref).max() <= 1e-15 * np.abs(ref).max()
+    eng._check(eng.lib.mdft_vec_widen_f32(eng._h, n, dz.ptr, dy.ptr))
+    assert np.array_equal(dy.download(), zd.astype(np.float64))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gradient_rounded_to_fp32(precision):
+    """MDFT_DF_FP32: the gradient written as float is the FP64 gradient of the same
+    evaluation rounded to FP32, and F is unchanged."""
+    import ctypes as C
+    from mdft.gpu import DEVICE_PTRS, DF_FP32, DeviceBuffer, Engine
+    w = workloads.small("solute20_128", 16)
+    say(f"[df32] {w.name} 16^3 {precision}")
+    eng = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, precision)
+    eng.set_ck(w.ck_table())
+    eng.compute_vext(w.solute(), w.solvent)
+    xi = eng.alloc_field()
+    rng = np.random.default_rng(1)
+    xi.upload(workloads.guess_xi(eng.get_vext(), w.kT) * (1 + 0.05 * rng.standard_normal(eng.N * eng.no)))
+    g64 = eng.alloc_field()
+    p64, _ = eng.energy_and_gradient(xi, g64)
+    g32 = DeviceBuffer(eng, 4 * eng.N * eng.no)
+    p32 = np.zeros(3)
+    dp = C.POINTER(C.c_double)
+    eng._check(eng.lib.mdft_energy_and_gradient(eng._h, xi.ptr, p32.ctypes.data_as(dp), g32.ptr,
+                                                DEVICE_PTRS | DF_FP32))
+    assert np.array_equal(p32, p64)
+    assert np.array_equal(g32.download(np.float32), g64.download().astype(np.float32))
+    # refused without device pointers
+    assert eng.lib.mdft_energy_and_gradient(eng._h, xi.ptr, p32.ctypes.data_as(dp), g32.ptr, DF_FP32) != 0
+
+
+def test_fp32_vectors_reach_the_band():
+    """FP32 evaluations with the minimiser's vectors in FP32 reach the same 0.05 kJ/mol
+    band as with FP64 vectors (24^3, mmax=3, charged solute); evaluations reported."""
+    from mdft import warmstart as ws
+    from mdft.gpu import Engine
+    from mdft.minimize import minimize
+    w = workloads.Workload("v32", 2, (24, 24, 24), (14.0, 14.0, 14.0), 3, 512)
+    say("[v32] FP32 evaluations: FP64 against FP32 minimiser vectors, 24^3 mmax=3")
+    table = ck.synthetic_ck(3, 512, 0.05, seed=41, n0=w.n0)
+    sol = workloads.Solute(np.array([[7.0, 7.0, 7.0], [8.3, 7.2, 6.5], [6.1, 8.0, 7.4]]),
+                           np.array([0.6, -0.4, -0.2]), np.array([3.2, 2.9, 2.6]), np.array([0.6, 0.5, 0.4]))
+    e64 = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, "fp64")
+    e64.set_ck(table)
+    e64.compute_vext(sol, w.solvent)
+    x = e64.alloc_field()
+    ws.cold_start(e64, x)
+    ref = minimize(e64, x, m=1, gtol=1e-8, maxeval=3000)
+    e32 = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, "fp32")
+    e32.set_ck(table)
+    e32.compute_vext(sol, w.solvent)
+    out = {}
+    for v32 in (False, True):
+        ws.cold_start(e32, x)
+        r = minimize(e32, x, m=1, gtol=0.0, maxeval=1500, fnoise=1e-6, ftarget=ref.F, fband=0.05, fp32_vectors=v32)
+        f64 = float(np.sum(e64.energy_and_gradient(x, None)[0]))       # confirmed in FP64
+        out[v32] = (r, f64)
+        say(f"  fp32 vectors={v32}: {r.evaluations} evaluations, FP64 F - F* = {f64 - ref.F:+.4f} kJ/mol")
+        assert r.status == 4 and abs(f64 - ref.F) <= 0.1

@@ -24,6 +24,8 @@ def host(oracle_mod):
     lib.mdft_minimize.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int]
     lib.mdft_minimize.restype = C.c_int
+    lib.mdft_minimize_ex.argtypes = lib.mdft_minimize.argtypes + [C.c_int]
+    lib.mdft_minimize_ex.restype = C.c_int
     mock = C.CDLL(os.path.join(MOCK, "_build", "libmdft_abi_mock.so"))
     mock.mdft_ctx_create.argtypes = [C.POINTER(Desc), C.POINTER(C.c_void_p)]
     ip = C.POINTER(C.c_int)
@@ -47,13 +49,14 @@ def _problem(mock, w, sol, t):
     return h, vext
 
 
-def _minimize(lib, h, x, m=5, gtol=1e-6, fnoise=1e-7, maxeval=600, ftarget=0.0, fband=0.0):
+def _minimize(lib, h, x, m=5, gtol=1e-6, fnoise=1e-7, maxeval=600, ftarget=0.0, fband=0.0, vec32=None):
     g = np.zeros_like(x)
     stats = np.zeros(16)
     trace = np.zeros((maxeval, 3))
     dp = C.POINTER(C.c_double)
-    rc = lib.mdft_minimize(h, x.size, x.ctypes.data, g.ctypes.data, m, gtol, 0.0, fnoise, ftarget, fband, maxeval,
-                           stats.ctypes.data_as(dp), trace.ctypes.data_as(dp), maxeval)
+    args = (h, x.size, x.ctypes.data, g.ctypes.data, m, gtol, 0.0, fnoise, ftarget, fband, maxeval,
+            stats.ctypes.data_as(dp), trace.ctypes.data_as(dp), maxeval)
+    rc = lib.mdft_minimize(*args) if vec32 is None else lib.mdft_minimize_ex(*args, int(vec32))
     assert rc == 0
     return stats, trace[:int(stats[6])], g
 
@@ -116,4 +119,33 @@ def test_fortran_lbfgs_target_band(host):
     assert part[7] == 4 and abs(part[0] - full[0]) <= 0.05
     first = int(np.nonzero(np.abs(tr_full[:, 0] - full[0]) <= 0.05)[0][0]) + 1
     assert int(part[6]) == first and tr[-1, 2] == part[5]
+    mock.mdft_ctx_destroy(h)
+
+
+def test_fortran_lbfgs_fp32_vectors(host):
+    """With the directions and gradients stored in FP32 (mdft_minimize_ex, vec32 = 1)
+    the minimiser reaches the FP64-vector minimum to FP32 accuracy, returns its
+    gradient widened to FP64, and mdft_minimize_ex(vec32 = 0) agrees with mdft_minimize."""
+    lib, mock = host
+    w = workloads.Workload("probe", 0, (9, 9, 9), (10.0, 10.0, 10.0), 1, 256)
+    sol = workloads.Solute(np.array([[5.0, 5.0, 5.0]]), np.array([-0.4]), np.array([3.2]), np.array([0.6]))
+    t = ck.synthetic_ck(1, 256, 0.05, seed=5, n0=w.n0)
+    h, vext = _problem(mock, w, sol, t)
+    x0 = workloads.guess_xi(vext, w.kT)
+    xa, xb, xc = x0.copy(), x0.copy(), x0.copy()
+    sa, _, _ = _minimize(lib, h, xa, m=1, maxeval=1500)
+    sb, trb, gb = _minimize(lib, h, xb, m=1, maxeval=1500, vec32=0)
+    # (the oracle's threaded sums are not bit-reproducible from run to run)
+    assert abs(sa[0] - sb[0]) < 1e-7 * abs(sa[0]) and np.abs(xa - xb).max() < 1e-5
+    sc, trc, gc = _minimize(lib, h, xc, m=1, gtol=1e-5, maxeval=1500, vec32=1)
+    assert sc[7] in (0, 1)
+    assert abs(sc[0] - sa[0]) < 1e-6 * abs(sa[0])
+    assert np.abs(xc - xa).max() < 1e-3 * np.abs(xa).max()
+    # the returned gradient is the final iterate's, widened from FP32
+    parts = np.zeros(3)
+    gref = np.zeros_like(xc)
+    dp = C.POINTER(C.c_double)
+    mock.mdft_energy_and_gradient.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.c_int]
+    assert mock.mdft_energy_and_gradient(h, xc.ctypes.data, parts.ctypes.data_as(dp), gref.ctypes.data, 0) == 0
+    assert np.array_equal(gc, gref.astype(np.float32).astype(np.float64))
     mock.mdft_ctx_destroy(h)
