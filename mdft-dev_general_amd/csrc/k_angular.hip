@@ -521,11 +521,40 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         T vt = hr[0].x;
 #pragma unroll
         for (int mu2 = 1; mu2 < H1; mu2++) vt += hr[mu2].x * tp[mu2].x - hr[mu2].y * tp[mu2].y;
+        const size_t g = r0 * NO + i;
+        if constexpr (sizeof(T) == 4) {
+            if (mode == 0) {
+                // FP32 path: the per-orientation terms in FP32 (the transforms feeding them
+                // are FP32 already), summed in FP64; rho/rho0 - 1 is formed in FP64 first so
+                // that the excess term keeps its small deviations from the bulk
+                const double x = xr[q];
+                const double xx = x * x;
+                const float wf = (float)wq[io];
+                const float vexcf = (float)k.prefactor * wf * vt;
+                const float drf = (float)(xx - 1.0);
+                const float rho0f = (float)rho0, x2 = (float)xx, xf = (float)x;
+                fexc += (double)(rho0f * drf * vexcf);
+                float gid = 0.0f;
+                if (xx * rho0 > DBL_EPSILON) {
+                    const float lg = logf(x2);  // ln(rho / rho0)
+                    const float ve = (float)vr[q];
+                    const float kTf = (float)kT, dvf = (float)dv;
+                    fid += (double)(kTf * dvf * wf * rho0f * (x2 * lg - drf));
+                    fext += (double)(dvf * wf * rho0f * x2 * ve);
+                    gid = wf * 2.0f * rho0f * xf * (kTf * lg + ve);
+                } else {
+                    fid += kT * dv * wq[io] * rho0;
+                }
+                const float gv = gid + 2.0f * rho0f * xf * vexcf;
+                if (df32) df32[g] = gv;
+                else if (df) df[g] = (double)gv;
+                continue;
+            }
+        }
         // from here on FP64: free-energy reductions, rho ln rho and the gradient
         const double v = (double)vt;
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
-        const size_t g = r0 * NO + i;
         const double x = xr[q];
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
