@@ -10,8 +10,11 @@
 // so every channel of every tap is a contiguous, coalesced stream; the box is
 // periodic, so the convolutions wrap.
 //
-//   forward      one thread per grid point, KG output channels in registers,
-//                the workgroup's weight slice [KG][Cin][27] in LDS (broadcast)
+//   forward      one thread per grid point, 32 output channels in registers, the
+//                workgroup's weight slice [Cin][taps][32] in LDS read as 16-byte
+//                groups (broadcast): 32 FMAs per gathered input value and 4 per
+//                LDS read (KG = 8 channels and single-float reads where the slice
+//                does not fit the device's LDS)
 //   data grad    the same shape with input channels in registers
 //   weight grad  one thread per (input channel, tap) with all output channels
 //                in registers, over a chunk of points whose output gradients
@@ -97,6 +100,58 @@ __global__ __launch_bounds__(kThreads) void conv_fwd(const float *__restrict__ i
     }
 #pragma unroll
     for (int k = 0; k < KG; k++) {
+        if (k < kg) {
+            float v = acc[k] + b[co0 + k];
+            if (relu && v < 0.0f) v *= kLeak;
+            out[(size_t)(co0 + k) * N + p] = v;
+        }
+    }
+}
+
+// The same with KW output channels per thread and the weights stored [cin][TAPS][KW]
+// (zero past cout), so each LDS read brings four weights of one (ci, t).
+constexpr int KW = 32;
+constexpr int kWideThreads = 1024;
+
+template <int TAPS>
+__global__ __launch_bounds__(kWideThreads) void conv_fwd_wide(const float *__restrict__ in, const float *__restrict__ W,
+                                                             const float *__restrict__ b, float *__restrict__ out,
+                                                             int cin, int cout, Geo g, int d, int relu) {
+    extern __shared__ float4 ws4[];  // [cin][TAPS][KW / 4]
+    float *ws = reinterpret_cast<float *>(ws4);
+    const int N = g.nx * g.ny * g.nz;
+    const int co0 = blockIdx.y * KW;
+    const int kg = min(KW, cout - co0);
+    for (int i = threadIdx.x; i < cin * TAPS * KW; i += kWideThreads) {
+        const int k = i % KW, r = i / KW;  // r = ci * TAPS + t
+        ws[i] = (k < kg) ? W[(size_t)(co0 + k) * cin * TAPS + r] : 0.0f;
+    }
+    __syncthreads();
+    const int p = blockIdx.x * kWideThreads + threadIdx.x;
+    if (p >= N) return;
+    int idx[TAPS];
+    taps<TAPS>(p, g, d, 1, idx);
+    float acc[KW];
+#pragma unroll
+    for (int k = 0; k < KW; k++) acc[k] = 0.0f;
+    for (int ci = 0; ci < cin; ci++) {
+        const float *src = in + (size_t)ci * N;
+        const float4 *wr = ws4 + (size_t)ci * TAPS * (KW / 4);
+#pragma unroll
+        for (int t = 0; t < TAPS; t++) {
+            const float v = src[idx[t]];
+#pragma unroll
+            for (int q = 0; q < KW / 4; q++) {
+                const float4 w = wr[t * (KW / 4) + q];
+                acc[4 * q + 0] += w.x * v;
+                acc[4 * q + 1] += w.y * v;
+                acc[4 * q + 2] += w.z * v;
+                acc[4 * q + 3] += w.w * v;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KW; k++) {
         if (k < kg) {
             float v = acc[k] + b[co0 + k];
             if (relu && v < 0.0f) v *= kLeak;
@@ -299,6 +354,7 @@ unsigned grid1(size_t n) {
 
 struct mdft_wsnet {
     int device = 0;
+    size_t lds_limit = 65536;  // the device's LDS per workgroup
     int nl = 0;
     std::vector<int> ch, dil, taps;  // taps: 27 (3x3x3) or 1 (pointwise)
     std::vector<size_t> woff, boff;  // offsets of W_l and b_l in the parameter vector
@@ -365,6 +421,21 @@ int forward(mdft_wsnet *n, const float *in) {
     const float *src = in;
     for (int l = 1; l <= n->nl; l++) {
         const int cin = n->ch[l - 1], cout = n->ch[l], T = n->taps[l - 1];
+        const size_t wide_lds = sizeof(float) * KW * cin * T;
+        if (wide_lds <= n->lds_limit) {
+            dim3 gw((unsigned)((N + kWideThreads - 1) / kWideThreads), (unsigned)((cout + KW - 1) / KW));
+            if (T == 27)
+                hipLaunchKernelGGL(conv_fwd_wide<27>, gw, dim3(kWideThreads), wide_lds, n->stream, src,
+                                   n->params + n->woff[l - 1], n->params + n->boff[l - 1], n->act[l], cin, cout, g,
+                                   n->dil[l - 1], l < n->nl ? 1 : 0);
+            else
+                hipLaunchKernelGGL(conv_fwd_wide<1>, gw, dim3(kWideThreads), wide_lds, n->stream, src,
+                                   n->params + n->woff[l - 1], n->params + n->boff[l - 1], n->act[l], cin, cout, g, 1,
+                                   l < n->nl ? 1 : 0);
+            WSCHK(n, hipGetLastError());
+            src = n->act[l];
+            continue;
+        }
         dim3 grid((unsigned)((N + kThreads - 1) / kThreads), (unsigned)((cout + KG - 1) / KG));
         const size_t lds = sizeof(float) * KG * cin * T;
         if (T == 27)
@@ -469,6 +540,11 @@ int mdft_wsnet_create(int nlayers, const int *channels, const int *ksize, const 
     }
     mdft_wsnet *n = new mdft_wsnet();
     n->device = device;
+    {
+        int lim = 0;
+        if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lim > 0)
+            n->lds_limit = (size_t)lim;
+    }
     n->nl = nlayers;
     n->ch.assign(channels, channels + nlayers + 1);
     n->dil.assign(dilations, dilations + nlayers);
