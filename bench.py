@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--memory", type=int, default=1, help="L-BFGS memory m (both paths; 1 is the reference's)")
     ap.add_argument("--fp64-vectors", action="store_true",
                     help="fast path: keep the minimiser's directions and gradients in FP64 (default FP32)")
+    ap.add_argument("--unscaled", action="store_true",
+                    help="fast path: minimise over xi itself (default: over u = sqrt(w / mean w) xi)")
     ap.add_argument("--maxeval", type=int, default=1500)
     ap.add_argument("--mode", choices=["converge", "eval", "copy"], default="converge",
                     help="copy: a plain device copy of 1 GiB (z = x through mdft_vec_lincomb), the "
@@ -258,6 +260,15 @@ def main():
                          ftarget=dg_ref, fband=band)
         eng.synchronize()
         t_plain = time.perf_counter() - t0
+        # ---- the FP64 method over the scaled variables, to the band (what the change of
+        # variables alone gives, without the mixed precision or the network)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        wsm.cold_start(eng, xi)
+        res_s = minimize(eng, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-7,
+                         ftarget=dg_ref, fband=band, scaled=True)
+        eng.synchronize()
+        t_scaled64 = time.perf_counter() - t0
         # ---- fast path: learned warm start + FP32 evaluations to the band,
         # confirmed by one FP64 evaluation of the final iterate
         e32 = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, "fp32")
@@ -277,7 +288,7 @@ def main():
             else:
                 wsm.cold_start(e32, xi)
             r = minimize(e32, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref,
-                         fp32_vectors=not args.fp64_vectors,
+                         fp32_vectors=not args.fp64_vectors, scaled=not args.unscaled,
                          fband=band)
             c, _ = eng.energy_and_gradient(xi, None)
             eng.synchronize()
@@ -323,18 +334,21 @@ def main():
                                             "vector_bytes_per_element_per_iteration": per_elem,
                                             "minimiser_vectors": "fp64" if args.fp64_vectors else "fp32"}
         vname = "FP64" if args.fp64_vectors else "FP32"
+        sname = "xi" if args.unscaled else "u = sqrt(w / mean w) xi"
         line.update({
             "value": t_fast, "unit": "s", "steps": res_f.evaluations,
             "ms_per_step": t_fast / max(res_f.evaluations, 1) * 1e3,
             "t_fast_s": t_fast, "t_plain_fp64_s": t_conv, "speedup_vs_plain": t_conv / t_fast,
             "t_plain_fp64_to_band_s": t_plain, "speedup_band_to_band": t_plain / t_fast,
+            "t_fp64_scaled_to_band_s": t_scaled64, "fp64_scaled_evaluations": res_s.evaluations,
             "t_fp32_cold_start_s": t_cold32, "t_fp32_boltzmann_start_s": t_boltz32,
             "fp32_cold_start_evaluations": res_c32.evaluations, "fp32_boltzmann_start_evaluations": res_b32.evaluations,
             "dG_fp32_boltzmann_error_kJmol": dg_b32 - dg_ref,
             "fast_path": ("learned warm start (network inference inside the timed region) + FP32 evaluations + "
-                          f"L-BFGS (FP64 scalars, {vname} vectors), FP64 confirmation") if net is not None else
-                         f"FP32 evaluations + L-BFGS (FP64 scalars, {vname} vectors) from the reference cold start, "
-                         "FP64 confirmation (no trained warm-start weights found)",
+                          f"L-BFGS (FP64 scalars, {vname} vectors, {sname} variables), FP64 confirmation")
+                         if net is not None else
+                         f"FP32 evaluations + L-BFGS (FP64 scalars, {vname} vectors, {sname} variables) from the "
+                         "reference cold start, FP64 confirmation (no trained warm-start weights found)",
             "dtype": f"f32 evaluations (FP64 reductions and ideal term), L-BFGS with FP64 scalars and {vname} "
                      "vectors, f64 confirmation; the plain path all f64",
             "dG_ref_kJmol": dg_ref, "dG_fast_confirmed_kJmol": dg_fast,

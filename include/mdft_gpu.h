@@ -49,9 +49,14 @@ enum { MDFT_PREC_FP64 = 0, MDFT_PREC_FP32 = 1, MDFT_PREC_BF16 = 2 };
 enum {
     MDFT_DEVICE_PTRS = 1, /* xi / df / vext arguments are device pointers */
     MDFT_TIMERS = 2,      /* record per-phase HIP event timings (mdft_phase_times) */
-    MDFT_DF_FP32 = 4      /* mdft_energy_and_gradient only, with MDFT_DEVICE_PTRS and molecular
+    MDFT_DF_FP32 = 4,     /* mdft_energy_and_gradient only, with MDFT_DEVICE_PTRS and molecular
                              symmetry order 2, FP64 or FP32 contexts: df is a float array that
                              receives dF rounded to FP32 (the minimiser's FP32 vector storage) */
+    MDFT_SCALED_VARS = 8  /* mdft_energy_and_gradient only, molecular symmetry order 2, FP64 or FP32
+                             contexts: xi holds u(io, r) = s(io) xi(io, r) with s = sqrt(w / mean w)
+                             (w the quadrature weights) and df receives dF/du = dF/dxi / s -- the
+                             minimiser's change of variables, which takes the quadrature weight out
+                             of the curvature of every term (no reference counterpart) */
 };
 
 typedef struct mdft_desc {
@@ -202,6 +207,11 @@ int mdft_vec_combine_step_f32(mdft_ctx *ctx, size_t n, int k, const double *coef
 int mdft_vec_axpy_f32(mdft_ctx *ctx, size_t n, double a, const float *z, const double *x, double *y);
 /* out = (double) in. */
 int mdft_vec_widen_f32(mdft_ctx *ctx, size_t n, const float *in, double *out);
+/* The change of variables of MDFT_SCALED_VARS over n = (points) x no values: dir = 1,
+ * out = s in (xi -> u, or dF/du -> dF/dxi); dir = -1, out = in / s (u -> xi).  in is
+ * FP32 when in_fp32 (a gradient stored in FP32), else FP64; out may alias an FP64 in.
+ * No reference counterpart. */
+int mdft_scale_variables(mdft_ctx *ctx, size_t n, int dir, int in_fp32, const void *in, double *out);
 
 /* ---- restart (density.bin) support, all device pointers ----
  * angl2proj / proj2angl of src/module_orientation_projection_transform.f90:159-199 and

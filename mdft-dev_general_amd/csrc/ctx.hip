@@ -476,7 +476,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
-                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale};
+                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);
@@ -614,6 +614,23 @@ int mdft_excess_stage(mdft_ctx *c, const double *xi, int stage, double *out, int
 }
 
 // mode 0: full energy_and_gradient, 1: excess only
+// s(io) = sqrt(w(io) / mean w) and 1 / s(io), on the device (scaled variables u = s xi)
+static int ensure_vscale(mdft_ctx *c) {
+    if (c->d_vscale) return MDFT_OK;
+    const int no = c->no;
+    double wm = 0.0;
+    for (int io = 0; io < no; io++) wm += c->w[io];
+    wm /= no;
+    std::vector<double> h(2 * (size_t)no);
+    for (int io = 0; io < no; io++) {
+        h[io] = sqrt(c->w[io] / wm);
+        h[no + io] = 1.0 / h[io];
+    }
+    HIPCHK(c, hipMalloc(&c->d_vscale, sizeof(double) * h.size()));
+    HIPCHK(c, hipMemcpy(c->d_vscale, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    return MDFT_OK;
+}
+
 static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mode, double *sums) {
     int rc = ready(c);
     if (rc) return rc;
@@ -624,6 +641,11 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     if (df32 && (!dev || mode != 0 || !df || c->d.mrso != 2 || c->d.precision == MDFT_PREC_BF16))
         return fail(c, MDFT_EINVAL, "MDFT_DF_FP32 needs device pointers, a gradient, molecular symmetry order 2 "
                                     "and an FP64 or FP32 context");
+    const bool scaled = flags & MDFT_SCALED_VARS;
+    if (scaled && (mode != 0 || c->d.mrso != 2 || c->d.precision == MDFT_PREC_BF16))
+        return fail(c, MDFT_EINVAL, "MDFT_SCALED_VARS needs mdft_energy_and_gradient, molecular symmetry order 2 "
+                                    "and an FP64 or FP32 context");
+    if (scaled && (rc = ensure_vscale(c))) return rc;
     if (timers && (rc = ensure_events(c))) return rc;
     const size_t nbytes = sizeof(double) * c->N * c->no;
     const double *xd = xi;
@@ -639,12 +661,17 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
             dd = c->d_df_stage;
         }
     }
+    c->vsinv = scaled ? c->d_vscale + c->no : nullptr;
     rc = run_excess_pipeline(c, xd, 4, timers);
-    if (rc) return rc;
+    if (rc) {
+        c->vsinv = nullptr;
+        return rc;
+    }
     int nb = 0;
     c->df32 = df32 ? reinterpret_cast<float *>(df) : nullptr;
     const hipError_t fe = launch_final(c, xd, dd, mode, &nb);
     c->df32 = nullptr;
+    c->vsinv = nullptr;
     HIPCHK(c, fe);
     if (timers) HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     rc = read_sums(c, nb, sums);
@@ -860,6 +887,15 @@ int mdft_vec_axpy_f32(mdft_ctx *c, size_t n, double a, const float *z, const dou
     if (!c || !z || !x || !y) return MDFT_EINVAL;
     if (!aligned16(z) || !aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
     HIPCHK(c, vec_axpy32(c, n, a, z, x, y));
+    return MDFT_OK;
+}
+
+int mdft_scale_variables(mdft_ctx *c, size_t n, int dir, int in_fp32, const void *in, double *out) {
+    if (!c || !in || !out || (dir != 1 && dir != -1)) return MDFT_EINVAL;
+    if (n % (size_t)c->no) return fail(c, MDFT_EINVAL, "n must be a whole number of grid points times no");
+    int rc = ensure_vscale(c);
+    if (rc) return rc;
+    HIPCHK(c, vec_scale_orient(c, n, c->no, dir > 0 ? c->d_vscale : c->d_vscale + c->no, in, in_fp32 != 0, out));
     return MDFT_OK;
 }
 

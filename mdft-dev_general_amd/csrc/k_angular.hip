@@ -320,7 +320,8 @@ struct Dims {
 template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
                                                        size_t N, AngTables t, double k0, int feat, double beta,
-                                                       const double *__restrict__ aux) {
+                                                       const double *__restrict__ aux,
+                                                       const double *__restrict__ vsinv) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     for (int q = 0; q < NIT; q++) {
         const int i = tid + q * kThreads;
         if (i >= npts * NO) break;
-        const double x = xr[q];
+        const double x = vsinv ? xr[q] * vsinv[i % NO] : xr[q];  // scaled variables: xi = u / s(io)
         if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
         else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
         else if (feat == 2) o[i] = (T)(x * x - 1.0);
@@ -421,7 +422,8 @@ template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
-                                                   FinalConsts k, int mode, float *__restrict__ df32) {
+                                                   FinalConsts k, int mode, float *__restrict__ df32,
+                                                   const double *__restrict__ vsinv) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -527,7 +529,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                 // FP32 path: the per-orientation terms in FP32 (the transforms feeding them
                 // are FP32 already), summed in FP64; rho/rho0 - 1 is formed in FP64 first so
                 // that the excess term keeps its small deviations from the bulk
-                const double x = xr[q];
+                const double si = vsinv ? vsinv[io] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
+                const double x = xr[q] * si;
                 const double xx = x * x;
                 const float wf = (float)wq[io];
                 const float vexcf = (float)k.prefactor * wf * vt;
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                 } else {
                     fid += kT * dv * wq[io] * rho0;
                 }
-                const float gv = gid + 2.0f * rho0f * xf * vexcf;
+                const float gv = (gid + 2.0f * rho0f * xf * vexcf) * (float)si;
                 if (df32) df32[g] = gv;
                 else if (df) df[g] = (double)gv;
                 continue;
@@ -555,7 +558,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         const double v = (double)vt;
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
-        const double x = xr[q];
+        const double si = (mode == 0 && vsinv) ? vsinv[io] : 1.0;  // scaled variables (mode 0)
+        const double x = xr[q] * si;
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
         if (mode == 0) {
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
             } else {
                 fid += kT * dv * wio * rho0;
             }
-            const double gv = gid + 2.0 * rho0 * x * vexc;
+            const double gv = (gid + 2.0 * rho0 * x * vexc) * si;
             if (df32) df32[g] = (float)gv;
             else if (df) df[g] = gv;
         } else if (mode == 1) {
@@ -619,7 +623,7 @@ bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int fe
     const double k0 = (feat == 3) ? 0.5 * c->dv : c->rho0;
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, npts, c->ang, k0, feat, beta, aux); return true;
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, npts, c->ang, k0, feat, beta, aux, feat == 0 ? c->vsinv : nullptr); return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;
@@ -633,7 +637,7 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
     const size_t nb = (npts + PT - 1) / PT;
     switch (c->L) {
 #define FIN_CASE(LL) \
-    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr); return true;
+    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr); return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
         default: return false;

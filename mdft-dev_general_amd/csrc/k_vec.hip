@@ -339,6 +339,16 @@ __global__ __launch_bounds__(kThreads) void widen1(size_t n, const float *__rest
         out[i] = (double)in[i];
 }
 
+// out[i] = in[i] fac[i % no] (in FP64 or FP32): the per-orientation change of variables
+__global__ __launch_bounds__(kThreads) void scale_orient(size_t n, int no, const double *__restrict__ fac,
+                                                        const double *__restrict__ in64, const float *__restrict__ in32,
+                                                        double *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const double v = in32 ? (double)in32[i] : in64[i];
+        out[i] = v * fac[i % (size_t)no];
+    }
+}
+
 // Per grid point (one wave each): n(r) = rho0 sum_io w_io xi^2; block partials of
 // sum_r n(r) and of the number of points with n(r) / n0 < cav (cavity points).
 __global__ __launch_bounds__(kThreads) void density_integrals(const double *__restrict__ xi,
@@ -526,6 +536,15 @@ hipError_t vec_axpy32(mdft_ctx *c, size_t n, double a, const float *z, const dou
 
 hipError_t vec_widen(mdft_ctx *c, size_t n, const float *in, double *out) {
     hipLaunchKernelGGL(widen1, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_scale_orient(mdft_ctx *c, size_t n, int no, const double *fac, const void *in, bool in_f32, double *out) {
+    hipLaunchKernelGGL(scale_orient, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, no, fac,
+                       in_f32 ? nullptr : static_cast<const double *>(in), in_f32 ? static_cast<const float *>(in) : nullptr,
+                       out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);

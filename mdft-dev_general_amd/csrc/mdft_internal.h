@@ -93,6 +93,8 @@ struct mdft_ctx {
     double *d_vpart = nullptr;    // vector-op block partials
     double *d_mpart = nullptr;    // multi-dot block partials [blocks][MDFT_VEC_MAX] + result
     float *df32 = nullptr;        // set during an evaluation with MDFT_DF_FP32: dF goes here, rounded to FP32
+    double *d_vscale = nullptr;   // [2 no]: s(io) = sqrt(w(io) / mean w) and 1 / s(io) (scaled variables)
+    const double *vsinv = nullptr;  // set during an evaluation with MDFT_SCALED_VARS: 1 / s
     float *d_chscale = nullptr;   // per-channel input scale of the warm-start features [2 np]
     float *d_adjscale = nullptr;  // per-channel factor of the adjoint of proj2angl [2 np]
     double *h_sums = nullptr;     // pinned mirror of d_sums
@@ -151,6 +153,7 @@ hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, 
                               double a, const double *x, double *y);
 hipError_t vec_axpy32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y);
 hipError_t vec_widen(mdft_ctx *c, size_t n, const float *in, double *out);
+hipError_t vec_scale_orient(mdft_ctx *c, size_t n, int no, const double *fac, const void *in, bool in_f32, double *out);
 hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,
                        const double *d_site_off, double rcut, double qfact, double clip, bool coulomb = true);
 hipError_t launch_poisson_vext(mdft_ctx *c, int natom, const double *h_atoms, int nsite, const double *h_site_q,

@@ -15,7 +15,7 @@ module mdft_gpu
 
     integer, parameter :: dp = c_double
     integer(c_int), parameter, public :: MDFT_PREC_FP64 = 0, MDFT_PREC_FP32 = 1, MDFT_PREC_BF16 = 2
-    integer(c_int), parameter, public :: MDFT_DEVICE_PTRS = 1, MDFT_TIMERS = 2, MDFT_DF_FP32 = 4
+    integer(c_int), parameter, public :: MDFT_DEVICE_PTRS = 1, MDFT_TIMERS = 2, MDFT_DF_FP32 = 4, MDFT_SCALED_VARS = 8
 
     type, bind(C), public :: mdft_desc
         integer(c_int) :: nx, ny, nz
@@ -183,6 +183,13 @@ module mdft_gpu
             type(c_ptr), value :: ctx, in, out
             integer(c_size_t), value :: n
         end function
+        ! u = s xi (dir = 1) or xi = u / s (dir = -1), s(io) = sqrt(w(io) / mean w)
+        integer(c_int) function mdft_scale_variables(ctx, n, dir, in_fp32, in, out) bind(C, name="mdft_scale_variables")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: ctx, in, out
+            integer(c_size_t), value :: n
+            integer(c_int), value :: dir, in_fp32
+        end function
         integer(c_int) function mdft_charged_solute_corrections(lengths, n0, natom, atoms, nsite, sites, eps_r, out) &
                 bind(C, name="mdft_charged_solute_corrections")
             import :: c_int, c_double
@@ -198,6 +205,7 @@ module mdft_gpu
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
     public :: mdft_vec_multi_dot, mdft_vec_combine, mdft_vec_combine_step
     public :: mdft_vec_multi_dot_f32, mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32
+    public :: mdft_scale_variables
     public :: mdft_charged_solute_corrections
     public :: check, last_error_text
 

@@ -21,7 +21,7 @@ module mdft_lbfgs
     use mdft_gpu, only: mdft_energy_and_gradient, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
                         mdft_vec_multi_dot, mdft_vec_combine_step, mdft_vec_multi_dot_f32, &
                         mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32, &
-                        MDFT_DEVICE_PTRS, MDFT_DF_FP32
+                        mdft_scale_variables, MDFT_DEVICE_PTRS, MDFT_DF_FP32, MDFT_SCALED_VARS
     implicit none
     private
     integer, parameter :: dp = c_double
@@ -56,15 +56,19 @@ contains
     ! fnoise*|F| so that steps are judged by the curvature condition once F stops
     ! resolving the decrease (the evaluation's single-precision chops put ~1e-8
     ! relative noise on F in FP64, more in the mixed-precision modes)
-    ! vec32 /= 0: the stored directions and gradients are FP32 (half the bytes of every
+    ! opts bit 0 (1): the stored directions and gradients are FP32 (half the bytes of every
     ! vector pass; FP64 dot products, iterate and Gram matrix) -- for the mixed-precision
-    ! evaluations, whose gradients carry FP32-level errors anyway.  g still receives the
-    ! final gradient in FP64.
+    ! evaluations, whose gradients carry FP32-level errors anyway.
+    ! opts bit 1 (2): minimise over u = s xi, s(io) = sqrt(w(io) / mean w) (MDFT_SCALED_VARS):
+    ! every term's curvature in xi carries the quadrature weight w(io), which spans a factor
+    ! of about 4 over the theta nodes at mmax = 5; in u it does not.  xi is converted in
+    ! place on entry and back on return.
+    ! xi and g (FP64) receive the last iterate and its gradient dF/dxi in every mode.
     integer(c_int) function mdft_minimize_ex(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, &
-                                             stats, trace, ntrace, vec32) bind(C, name="mdft_minimize_ex") result(rc)
+                                             stats, trace, ntrace, opts) bind(C, name="mdft_minimize_ex") result(rc)
         type(c_ptr), value :: ctx, xi, g
         integer(c_size_t), value :: n
-        integer(c_int), value :: m, maxeval, ntrace, vec32
+        integer(c_int), value :: m, maxeval, ntrace, opts
         real(c_double), value :: gtol, ftol, fnoise, ftarget, fband
         real(c_double), intent(out) :: stats(16)
         real(c_double), intent(out) :: trace(3, *)
@@ -80,16 +84,20 @@ contains
         real(dp) :: t_ready   ! step for which xn already holds x - t d (negative: none)
         integer :: i, j, k, mm, nb, nl, fd, fg, gcur, neval, iter, status
         integer(c_size_t) :: bytes
-        integer(c_int) :: eflags
-        logical :: ok, f32
+        integer(c_int) :: eflags, r2
+        logical :: ok, f32, sc, scaled_in
 
         rc = 0
         stats = 0
         mm = max(1, min(int(m), mmem_max))
-        f32 = vec32 /= 0
+        gcur = mm + 2
+        f32 = iand(opts, 1_c_int) /= 0
+        sc = iand(opts, 2_c_int) /= 0
+        scaled_in = .false.
         bytes = merge(4_c_size_t, 8_c_size_t, f32) * n
         eflags = MDFT_DEVICE_PTRS
-        if (f32) eflags = MDFT_DEVICE_PTRS + MDFT_DF_FP32
+        if (f32) eflags = eflags + MDFT_DF_FP32
+        if (sc) eflags = eflags + MDFT_SCALED_VARS
         owned = .false.
         xn_own = c_null_ptr
         t_ready = -1.0_dp
@@ -105,6 +113,10 @@ contains
             end if
         end do
         rc = mdft_device_alloc(ctx, 8_c_size_t * n, xn_own); if (rc /= 0) goto 900
+        if (sc) then
+            rc = mdft_scale_variables(ctx, n, 1_c_int, 0_c_int, xi, xi); if (rc /= 0) goto 900
+            scaled_in = .true.
+        end if
         x = xi
         xn = xn_own
         gram = 0.0_dp
@@ -252,10 +264,17 @@ contains
 900     continue
         ! the caller's xi/g buffers must hold the result
         if (rc == 0 .and. .not. c_associated(x, xi)) rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, 0.0_dp, c_null_ptr, xi)
-        if (rc == 0 .and. f32 .and. c_associated(vec(gcur))) then
+        if (rc == 0 .and. sc .and. c_associated(vec(gcur))) then
+            ! dF/dxi = s dF/du
+            rc = mdft_scale_variables(ctx, n, 1_c_int, merge(1_c_int, 0_c_int, f32), vec(gcur), g)
+        else if (rc == 0 .and. f32 .and. c_associated(vec(gcur))) then
             rc = mdft_vec_widen_f32(ctx, n, vec(gcur), g)
         else if (rc == 0 .and. .not. c_associated(vec(gcur), g)) then
             rc = mdft_vec_lincomb(ctx, n, 1.0_dp, vec(gcur), 0.0_dp, c_null_ptr, g)
+        end if
+        if (scaled_in) then                      ! xi = u / s, also after an error
+            r2 = mdft_scale_variables(ctx, n, -1_c_int, 0_c_int, xi, xi)
+            if (rc == 0) rc = r2
         end if
         call release()
 

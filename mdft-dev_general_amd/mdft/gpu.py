@@ -22,6 +22,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
 DEVICE_PTRS = 1
 TIMERS = 2
 DF_FP32 = 4
+SCALED_VARS = 8
 
 MDFT_EXPORTS = (
     "mdft_ctx_create", "mdft_ctx_destroy", "mdft_last_error", "mdft_get_info", "mdft_get_quadrature",
@@ -32,7 +33,7 @@ MDFT_EXPORTS = (
     "mdft_vec_combine", "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj",
     "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_vec_combine_step",
     "mdft_charged_solute_corrections", "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32",
-    "mdft_vec_widen_f32",
+    "mdft_vec_widen_f32", "mdft_scale_variables",
 )
 
 
@@ -97,6 +98,7 @@ def load_library(path: str = LIBPATH):
     lib.mdft_vec_combine_step_f32.argtypes = [vp, C.c_size_t, C.c_int, _dp, C.POINTER(vp), vp, C.c_double, vp, vp]
     lib.mdft_vec_axpy_f32.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp]
     lib.mdft_vec_widen_f32.argtypes = [vp, C.c_size_t, vp, vp]
+    lib.mdft_scale_variables.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp]
     lib.mdft_angl2proj.argtypes = [vp, vp, C.c_size_t, vp]
     lib.mdft_proj2angl.argtypes = [vp, vp, C.c_size_t, vp]
     lib.mdft_regrid_trilinear.argtypes = [vp, vp, _ip, _dp, C.c_double, vp]
@@ -109,7 +111,8 @@ def load_library(path: str = LIBPATH):
                  "mdft_vec_axpy_dot", "mdft_vec_lincomb", "mdft_vec_multi_dot", "mdft_vec_combine", "mdft_vec_combine_step",
                  "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj", "mdft_proj2angl",
                  "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_charged_solute_corrections",
-                 "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32", "mdft_vec_widen_f32"):
+                 "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32", "mdft_vec_widen_f32",
+                 "mdft_scale_variables"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib

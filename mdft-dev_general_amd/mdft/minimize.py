@@ -56,12 +56,16 @@ class MinResult:
 
 def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: int = 5, gtol: float = 1e-6,
              ftol: float = 0.0, fnoise: float = 1e-7, maxeval: int = 2000, ntrace: int = 2000,
-             ftarget: float = 0.0, fband: float = 0.0, fp32_vectors: bool = False, lib=None) -> MinResult:
+             ftarget: float = 0.0, fband: float = 0.0, fp32_vectors: bool = False, scaled: bool = False,
+             lib=None) -> MinResult:
     """Minimise F over xi (in place).  Returns the final F and statistics.
     fnoise is the relative noise level of F used to relax the sufficient-decrease
     test; with fband > 0 the run stops at the first iterate with |F - ftarget| <= fband.
     fp32_vectors stores the minimiser's directions and gradients in FP32 (half the
-    bytes per vector pass; meant for FP32 evaluations); g still receives FP64."""
+    bytes per vector pass; meant for FP32 evaluations); g still receives FP64.
+    scaled minimises over u = s xi with s(io) = sqrt(w(io) / mean w), which takes the
+    quadrature weight out of the curvature (fewer iterations at the same minimum);
+    xi and g are returned in the caller's variables."""
     lib = lib or host_library()
     own = g is None
     if own:
@@ -71,7 +75,8 @@ def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: in
     rc = lib.mdft_minimize_ex(eng._h, eng.N * eng.no, xi.ptr, g.ptr, int(m), float(gtol), float(ftol),
                               float(fnoise), float(ftarget), float(fband), int(maxeval),
                               stats.ctypes.data_as(C.POINTER(C.c_double)),
-                              trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace), int(bool(fp32_vectors)))
+                              trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace),
+                              int(bool(fp32_vectors)) | (2 if scaled else 0))
     if rc != 0:
         raise MdftError(rc, eng.lib.mdft_last_error(eng._h).decode())
     it = int(stats[6])

@@ -19,6 +19,16 @@ iterate within 0.05 kJ/mol of the solution, cold start against the network's
 start.
 
     python -m mdft.train_warmstart --train 160 --val 16 --epochs 40 --energy-epochs 40 --weights PATH
+
+Calibration of the start (run after training, on the target workload):
+
+    python -m mdft.train_warmstart --calibrate-mix --weights PATH
+
+solves 128^3 solutes of seeds 8000-8003 (not the benchmark's solute) from the
+starts (1 - lam) rho_cold + lam rho_pred for lam in 0.5 .. 1.0 with the fast
+path's minimiser (FP32 evaluations and vectors, scaled variables) and stores the
+lam of the fewest mean evaluations to the 0.05 kJ/mol band with the weights
+(`start_mix`), with the table it came from.
 """
 from __future__ import annotations
 
@@ -32,6 +42,36 @@ import numpy as np
 
 from . import warmstart as ws
 from . import workloads
+
+
+def calibrate_mix(path: str, seeds=(8000, 8001, 8002, 8003), lams=(0.5, 0.6, 0.7, 0.8, 0.9, 1.0), log=print):
+    """Choose the net's start mixture on solutes of the 128^3 workload (see the module text)."""
+    from .gpu import Engine
+    from .minimize import minimize
+    head = workloads.WORKLOADS["solute20_128"]
+    eng = Engine(head.n, head.length, head.mmax, head.mrso, head.kT, head.n0, "fp32")
+    eng.set_ck(head.ck_table())
+    net = ws.WarmStartNet.load(path)
+    xi, g = eng.alloc_field(), eng.alloc_field()
+    evals = np.zeros((len(seeds), len(lams)), dtype=np.int64)
+    for i, seed in enumerate(seeds):
+        eng.compute_vext(workloads.make_solute(head, seed), head.solvent)
+        ws.cold_start(eng, xi)
+        ref = minimize(eng, xi, g=g, m=1, gtol=1e-6, maxeval=1500, fnoise=1e-6, fp32_vectors=True, scaled=True)
+        for j, lam in enumerate(lams):
+            ws.warm_start(eng, net, xi, start_mix=lam)
+            r = minimize(eng, xi, g=g, m=1, gtol=0.0, maxeval=400, fnoise=1e-6, ftarget=ref.F, fband=0.05,
+                         fp32_vectors=True, scaled=True)
+            evals[i, j] = r.evaluations
+        log(f"  solute {seed}: F*={ref.F:.3f}; evaluations to the band for lam {list(lams)}: {list(evals[i])}")
+    best = float(lams[int(np.argmin(evals.mean(axis=0)))])
+    with np.load(path, allow_pickle=False) as z:
+        data = {k: z[k] for k in z.files}
+    data.update(start_mix=np.asarray(best), mix_calibration_lams=np.asarray(lams),
+                mix_calibration_seeds=np.asarray(seeds), mix_calibration_evaluations=evals)
+    np.savez(path, **data)
+    log(f"[train_warmstart] start_mix = {best} (mean evaluations {evals.mean(axis=0).round(2).tolist()})")
+    return best, evals
 
 
 def main(argv=None):
@@ -49,10 +89,16 @@ def main(argv=None):
     ap.add_argument("--weights", default=ws.DEFAULT_WEIGHTS)
     ap.add_argument("--log", default=None, help="JSON summary file")
     ap.add_argument("--gain-solutes", type=int, default=8, help="held-out solutes for the evaluation count")
+    ap.add_argument("--calibrate-mix", action="store_true",
+                    help="only calibrate the start mixture of the weights at --weights")
     args = ap.parse_args(argv)
 
     def say(s):
         print(s, flush=True)
+
+    if args.calibrate_mix:
+        calibrate_mix(args.weights, log=say)
+        return 0
 
     head = workloads.WORKLOADS["solute20_128"]
     w = workloads.Workload("warmstart_train", 3, (args.n,) * 3, (args.box,) * 3, head.mmax, head.nk, head.dk)

@@ -98,6 +98,7 @@ class WarmStartNet:
     def __init__(self, channels, ksize, dilations, seed: int = 0, device: int = 0):
         self.lib = _lib()
         self.input_scale = None   # per-channel feature scale [2 np] (float32) or None
+        self.start_mix = 1.0      # lam of the start (1 - lam) rho_cold + lam rho_pred (calibrated)
         self.channels = tuple(int(c) for c in channels)
         self.ksize = tuple(int(k) for k in ksize)
         self.dilations = tuple(int(d) for d in dilations)
@@ -175,6 +176,7 @@ class WarmStartNet:
     def save(self, path: str, **meta) -> None:
         os.makedirs(os.path.dirname(path), exist_ok=True)
         extra = {} if self.input_scale is None else {"input_scale": np.asarray(self.input_scale, np.float32)}
+        extra["start_mix"] = np.asarray(float(self.start_mix))
         np.savez(path, params=self.get_params(), channels=np.array(self.channels), ksize=np.array(self.ksize),
                  dilations=np.array(self.dilations), **extra, **{k: np.asarray(v) for k, v in meta.items()})
 
@@ -185,6 +187,8 @@ class WarmStartNet:
             net.set_params(z["params"])
             if "input_scale" in z.files:
                 net.input_scale = np.ascontiguousarray(z["input_scale"], dtype=np.float32)
+            if "start_mix" in z.files:
+                net.start_mix = float(z["start_mix"])
         return net
 
     @classmethod
@@ -273,19 +277,32 @@ def _F(eng, xi):
 
 
 def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: DeviceBuffer | None = None,
-               kind: int = WS_DENSITY, lams=()) -> dict:
-    """xi <- the network's starting density for the engine's vext.  With a
-    scratch field `fallback`, F is evaluated at the prediction (lam = 1), at
-    the cold start (lam = 0) and at the density mixtures `lams`, and the lowest
-    is kept: the selection costs len(lams) + 2 F evaluations and never gives a
-    worse start than the cold one."""
+               kind: int = WS_DENSITY, lams=(), start_mix: float | None = None) -> dict:
+    """xi <- the network's starting density for the engine's vext: the density
+    mixture (1 - lam) rho_cold + lam rho_pred of the reference cold start and
+    the prediction, lam = start_mix if given, else net.start_mix (calibrated with
+    the weights); a scratch field for the cold start is allocated when lam < 1
+    and no `fallback` is given.  With `fallback` and `lams`, F is evaluated at the
+    prediction (lam = 1), at the cold start (lam = 0) and at the mixtures `lams`
+    instead, and the lowest is kept: that selection costs len(lams) + 2 F
+    evaluations and never gives a start of higher F than the cold one."""
     assert net.channels[-1] == 2 * eng.np_ and net.channels[0] == 2 * eng.np_
+    lam = float(net.start_mix if start_mix is None else start_mix)
     f = features(eng, net.input_scale)
     pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
     net.forward(eng.n, f, pred)
     f.free()
     guess(eng, pred, xi, kind)
     info = {"start": "learned", "kind": "density" if kind == WS_DENSITY else "gamma"}
+    if not lams and lam < 1.0 and kind == WS_DENSITY:
+        scratch = fallback if fallback is not None else eng.alloc_field()
+        cold_start(eng, scratch)
+        mix(eng, lam, scratch, xi, xi)
+        if fallback is None:
+            scratch.free()
+        pred.free()
+        info.update(start=f"learned, mixed with the cold start (lam={lam})", lam=lam)
+        return info
     if fallback is not None:
         trials = {1.0: _F(eng, xi)}
         cold_start(eng, fallback)

@@ -6,6 +6,7 @@
  * pointers are plain host memory here.
  */
 #include <complex.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -80,16 +81,55 @@ int mdft_set_vext(mdft_ctx *c, const double *vext, int flags)
     return 0;
 }
 
+/* s(io) = sqrt(w(io) / mean w), the scaled-variable factor */
+static double *scale_table(mdft_ctx *c)
+{
+    int s[7];
+    or_sizes(c->o, s);
+    double *th = malloc(sizeof(double) * s[0]), *wt = malloc(sizeof(double) * s[0]);
+    double *w = malloc(sizeof(double) * c->no);
+    or_tables(c->o, th, wt, w);
+    double wm = 0.0;
+    for (int io = 0; io < c->no; io++) wm += w[io];
+    wm /= c->no;
+    for (int io = 0; io < c->no; io++) w[io] = sqrt(w[io] / wm);
+    free(th);
+    free(wt);
+    return w;
+}
+
 int mdft_energy_and_gradient(mdft_ctx *c, const double *xi, double *parts, double *df, int flags)
 {
-    if ((flags & MDFT_DF_FP32) && df) {
-        double *tmp = malloc(sizeof(double) * c->nvar);
-        or_energy_and_gradient(c->o, xi, c->vext, parts, tmp);
-        for (size_t i = 0; i < c->nvar; i++) ((float *)df)[i] = (float)tmp[i];
-        free(tmp);
-        return 0;
+    double *sc = (flags & MDFT_SCALED_VARS) ? scale_table(c) : NULL;
+    const double *x = xi;
+    double *xs = NULL;
+    if (sc) {
+        xs = malloc(sizeof(double) * c->nvar);
+        for (size_t i = 0; i < c->nvar; i++) xs[i] = xi[i] / sc[i % c->no];
+        x = xs;
     }
-    or_energy_and_gradient(c->o, xi, c->vext, parts, df);
+    double *tmp = df ? malloc(sizeof(double) * c->nvar) : NULL;
+    or_energy_and_gradient(c->o, x, c->vext, parts, tmp);
+    if (df)
+        for (size_t i = 0; i < c->nvar; i++) {
+            const double g = sc ? tmp[i] / sc[i % c->no] : tmp[i];
+            if (flags & MDFT_DF_FP32) ((float *)df)[i] = (float)g;
+            else df[i] = g;
+        }
+    free(tmp);
+    free(xs);
+    free(sc);
+    return 0;
+}
+
+int mdft_scale_variables(mdft_ctx *c, size_t n, int dir, int in_fp32, const void *in, double *out)
+{
+    double *sc = scale_table(c);
+    for (size_t i = 0; i < n; i++) {
+        const double v = in_fp32 ? (double)((const float *)in)[i] : ((const double *)in)[i];
+        out[i] = dir > 0 ? v * sc[i % c->no] : v / sc[i % c->no];
+    }
+    free(sc);
     return 0;
 }
 

@@ -284,3 +284,78 @@ def test_fp32_vectors_reach_the_band():
         out[v32] = (r, f64)
         say(f"  fp32 vectors={v32}: {r.evaluations} evaluations, FP64 F - F* = {f64 - ref.F:+.4f} kJ/mol")
         assert r.status == 4 and abs(f64 - ref.F) <= 0.1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_scaled_variables_evaluation(precision):
+    """MDFT_SCALED_VARS: evaluating at u = s xi gives F(xi) and dF/du = dF/dxi / s, s(io) =
+    sqrt(w(io) / mean w); mdft_scale_variables maps xi -> u -> xi (FP64) and an FP32
+    gradient back to dF/dxi."""
+    import ctypes as C
+    from mdft.gpu import DEVICE_PTRS, DF_FP32, SCALED_VARS, DeviceBuffer, Engine
+    from mdft.grid import angular_grid
+    w = workloads.small("solute20_128", 16)
+    say(f"[scaled] {w.name} 16^3 {precision}")
+    eng = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, precision)
+    eng.set_ck(w.ck_table())
+    eng.compute_vext(w.solute(), w.solvent)
+    rng = np.random.default_rng(2)
+    x = workloads.guess_xi(eng.get_vext(), w.kT) * (1 + 0.05 * rng.standard_normal(eng.N * eng.no))
+    wq = angular_grid(w.mmax, 2)["w"]
+    s = np.tile(np.sqrt(wq / wq.mean()), eng.N)
+    bx, bu, g, gu = eng.alloc_field(), eng.alloc_field(), eng.alloc_field(), eng.alloc_field()
+    bx.upload(x)
+    p0, _ = eng.energy_and_gradient(bx, g)
+    eng._check(eng.lib.mdft_scale_variables(eng._h, eng.N * eng.no, 1, 0, bx.ptr, bu.ptr))
+    u = bu.download()
+    assert np.abs(u - s * x).max() <= 4e-15 * np.abs(u).max()       # s from the library's own mean of w: a few ulp
+    dp = C.POINTER(C.c_double)
+    p1 = np.zeros(3)
+    eng._check(eng.lib.mdft_energy_and_gradient(eng._h, bu.ptr, p1.ctypes.data_as(dp), gu.ptr,
+                                                DEVICE_PTRS | SCALED_VARS))
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert np.abs(p1 - p0).max() <= tol * np.abs(p0).max()
+    gx, gux = g.download(), gu.download()
+    assert np.abs(gux - gx / s).max() <= tol * np.abs(gx / s).max()
+    # FP32 gradient storage back to dF/dxi, and u back to xi in place
+    g32 = DeviceBuffer(eng, 4 * eng.N * eng.no)
+    eng._check(eng.lib.mdft_energy_and_gradient(eng._h, bu.ptr, p1.ctypes.data_as(dp), g32.ptr,
+                                                DEVICE_PTRS | SCALED_VARS | DF_FP32))
+    eng._check(eng.lib.mdft_scale_variables(eng._h, eng.N * eng.no, 1, 1, g32.ptr, gu.ptr))
+    assert np.abs(gu.download() - gx).max() <= max(tol, 2e-7) * np.abs(gx).max()
+    eng._check(eng.lib.mdft_scale_variables(eng._h, eng.N * eng.no, -1, 0, bu.ptr, bu.ptr))
+    assert np.abs(bu.download() - x).max() <= 4e-15 * np.abs(x).max()
+
+
+def test_scaled_variables_minimise_faster():
+    """The minimiser over u = s xi reaches the 0.05 kJ/mol band of the same minimum in
+    fewer evaluations than over xi (24^3, mmax=5, charged solute, FP64 and FP32 with
+    FP32 vectors); evaluations reported."""
+    from mdft import warmstart as ws
+    from mdft.gpu import Engine
+    from mdft.minimize import minimize
+    w = workloads.Workload("sv", 2, (24, 24, 24), (14.0, 14.0, 14.0), 5, 512)
+    say("[scaled] evaluations to the band over xi and over u, 24^3 mmax=5")
+    table = ck.synthetic_ck(5, 512, 0.05, seed=43, n0=w.n0)
+    sol = workloads.Solute(np.array([[7.0, 7.0, 7.0], [8.3, 7.2, 6.5], [6.1, 8.0, 7.4]]),
+                           np.array([0.6, -0.4, -0.2]), np.array([3.2, 2.9, 2.6]), np.array([0.6, 0.5, 0.4]))
+    e64 = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, "fp64")
+    e64.set_ck(table)
+    e64.compute_vext(sol, w.solvent)
+    x = e64.alloc_field()
+    ws.cold_start(e64, x)
+    ref = minimize(e64, x, m=1, gtol=1e-8, maxeval=3000)
+    e32 = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, "fp32")
+    e32.set_ck(table)
+    e32.compute_vext(sol, w.solvent)
+    out = {}
+    for name, eng, kw in (("fp64", e64, {}), ("fp64 scaled", e64, {"scaled": True}),
+                          ("fp32", e32, {"fp32_vectors": True, "fnoise": 1e-6}),
+                          ("fp32 scaled", e32, {"fp32_vectors": True, "fnoise": 1e-6, "scaled": True})):
+        ws.cold_start(eng, x)
+        r = minimize(eng, x, m=1, gtol=0.0, maxeval=1500, ftarget=ref.F, fband=0.05, **kw)
+        f64 = float(np.sum(e64.energy_and_gradient(x, None)[0]))
+        out[name] = r.evaluations
+        say(f"  {name}: {r.evaluations} evaluations, FP64 F - F* = {f64 - ref.F:+.4f}")
+        assert r.status == 4 and abs(f64 - ref.F) <= 0.1
+    assert out["fp64 scaled"] < out["fp64"] and out["fp32 scaled"] < out["fp32"]
