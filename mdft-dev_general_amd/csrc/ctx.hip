@@ -476,7 +476,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
-                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale};
+                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);
@@ -631,7 +631,8 @@ static int ensure_vscale(mdft_ctx *c) {
     return MDFT_OK;
 }
 
-static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mode, double *sums) {
+static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mode, double *sums, int dot_k = 0,
+                    const void *const *dot_vecs = nullptr, double *dots = nullptr) {
     int rc = ready(c);
     if (rc) return rc;
     if (!xi) return fail(c, MDFT_EINVAL, "xi is null");
@@ -646,6 +647,26 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
         return fail(c, MDFT_EINVAL, "MDFT_SCALED_VARS needs mdft_energy_and_gradient, molecular symmetry order 2 "
                                     "and an FP64 or FP32 context");
     if (scaled && (rc = ensure_vscale(c))) return rc;
+    if (dot_k > 0) {
+        if (dot_k > 3 || !dot_vecs || !dots || !dev || !df || mode != 0 || c->d.mrso != 2 ||
+            c->d.precision == MDFT_PREC_BF16)
+            return fail(c, MDFT_EINVAL, "gradient products need 1-3 vectors, device pointers, a gradient, "
+                                        "molecular symmetry order 2 and an FP64 or FP32 context");
+        const size_t need = 3 * ((c->N + 3) / 4);
+        if (c->dpart_cap < need) {
+            if (c->d_dpart) HIPCHK(c, hipFree(c->d_dpart));
+            c->d_dpart = nullptr;
+            c->dpart_cap = 0;
+            HIPCHK(c, hipMalloc(&c->d_dpart, sizeof(double) * need));
+            c->dpart_cap = need;
+        }
+        c->dot_self = 0;
+        for (int j = 0; j < dot_k; j++) {
+            if (!dot_vecs[j]) return fail(c, MDFT_EINVAL, "null vector");
+            c->dot_vec[j] = dot_vecs[j];
+            if (dot_vecs[j] == static_cast<const void *>(df)) c->dot_self |= 1 << j;
+        }
+    }
     if (timers && (rc = ensure_events(c))) return rc;
     const size_t nbytes = sizeof(double) * c->N * c->no;
     const double *xd = xi;
@@ -669,10 +690,16 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     }
     int nb = 0;
     c->df32 = df32 ? reinterpret_cast<float *>(df) : nullptr;
+    c->dot_k = dot_k;
     const hipError_t fe = launch_final(c, xd, dd, mode, &nb);
     c->df32 = nullptr;
     c->vsinv = nullptr;
+    c->dot_k = 0;
     HIPCHK(c, fe);
+    if (dot_k > 0) {
+        HIPCHK(c, launch_reduce3_buf(c, c->d_dpart, nb, c->d_sums + 5));
+        HIPCHK(c, hipMemcpyAsync(c->h_sums + 5, c->d_sums + 5, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    }
     if (timers) HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     rc = read_sums(c, nb, sums);
     if (rc) return rc;
@@ -686,6 +713,7 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     sums[2] *= 0.5 * c->dv;  // ff = ff * 0.5 * dv (cproj_mrso.f90:745)
+    for (int j = 0; j < dot_k; j++) dots[j] = c->h_sums[5 + j];  // read_sums synchronised the stream
     return MDFT_OK;
 }
 
@@ -693,6 +721,18 @@ int mdft_energy_and_gradient(mdft_ctx *c, const double *xi, double *f_parts, dou
     if (!f_parts) return c ? fail(c, MDFT_EINVAL, "f_parts is null") : MDFT_EINVAL;
     double s[3];
     const int rc = evaluate(c, xi, df, flags, 0, s);
+    if (rc) return rc;
+    f_parts[0] = s[0];
+    f_parts[1] = s[1];
+    f_parts[2] = s[2];
+    return MDFT_OK;
+}
+
+int mdft_energy_and_gradient_dots(mdft_ctx *c, const double *xi, double *f_parts, void *df, int flags, int k,
+                                  const void *const *vecs, double *dots) {
+    if (!f_parts || k < 1) return c ? fail(c, MDFT_EINVAL, "f_parts is null or no vectors") : MDFT_EINVAL;
+    double s[3];
+    const int rc = evaluate(c, xi, static_cast<double *>(df), flags, 0, s, k, vecs, dots);
     if (rc) return rc;
     f_parts[0] = s[0];
     f_parts[1] = s[1];

@@ -100,6 +100,18 @@ __global__ __launch_bounds__(kThreads) void angl2proj_f64(const double *__restri
     }
 }
 
+// products of the gradient with up to 3 vectors (FP32 or FP64, bit j of self: the gradient
+// itself), accumulated in the final pass (mdft_energy_and_gradient_dots)
+struct DotArgs {
+    int k, self, f32;
+    const void *p[3];
+};
+
+__device__ inline double dot_operand(const DotArgs &da, int j, size_t g, double gv) {
+    if ((da.self >> j) & 1) return gv;
+    return da.f32 ? (double)static_cast<const float *>(da.p[j])[g] : static_cast<const double *>(da.p[j])[g];
+}
+
 __device__ inline void block_reduce3(double &a, double &b, double &c, double *partials) {
     __shared__ double red[3][kThreads / 64];
     for (int off = 32; off > 0; off >>= 1) {
@@ -423,7 +435,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode, float *__restrict__ df32,
-                                                   const double *__restrict__ vsinv) {
+                                                   const double *__restrict__ vsinv, DotArgs da,
+                                                   double *__restrict__ dpart) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -511,6 +524,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     }
     __syncthreads();
     double fid = 0.0, fext = 0.0, fexc = 0.0;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;  // products of dF with the DotArgs vectors
     const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
@@ -551,6 +565,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                 const float gv = (gid + 2.0f * rho0f * xf * vexcf) * (float)si;
                 if (df32) df32[g] = gv;
                 else if (df) df[g] = (double)gv;
+                if (da.k > 0) {
+                    d0 += (double)gv * dot_operand(da, 0, g, (double)gv);
+                    if (da.k > 1) d1 += (double)gv * dot_operand(da, 1, g, (double)gv);
+                    if (da.k > 2) d2 += (double)gv * dot_operand(da, 2, g, (double)gv);
+                }
                 continue;
             }
         }
@@ -576,6 +595,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
             const double gv = (gid + 2.0 * rho0 * x * vexc) * si;
             if (df32) df32[g] = (float)gv;
             else if (df) df[g] = gv;
+            if (da.k > 0) {
+                // the products see the gradient as stored
+                const double gs = df32 ? (double)(float)gv : gv;
+                d0 += gs * dot_operand(da, 0, g, gs);
+                if (da.k > 1) d1 += gs * dot_operand(da, 1, g, gs);
+                if (da.k > 2) d2 += gs * dot_operand(da, 2, g, gs);
+            }
         } else if (mode == 1) {
             if (df) df[g] += 2.0 * rho0 * x * vexc;
         } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
@@ -590,6 +616,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         }
     }
     block_reduce3(fid, fext, fexc, partials);
+    if (da.k > 0) {
+        __syncthreads();
+        block_reduce3(d0, d1, d2, dpart);
+    }
 }
 
 constexpr int kPT = 4;  // points per workgroup tile (keeps the LDS tile under 64 KB at mmax = 5)
@@ -635,9 +665,16 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
                     size_t npts = 0) {
     if (!npts) npts = c->N;
     const size_t nb = (npts + PT - 1) / PT;
+    DotArgs da{};
+    if (mode == 0 && c->dot_k > 0) {
+        da.k = c->dot_k;
+        da.self = c->dot_self;
+        da.f32 = c->df32 != nullptr;
+        for (int j = 0; j < 3; j++) da.p[j] = c->dot_vec[j];
+    }
     switch (c->L) {
 #define FIN_CASE(LL) \
-    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr); return true;
+    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr, da, c->d_dpart); return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
         default: return false;
@@ -700,20 +737,22 @@ hipError_t launch_ideal_f64(mdft_ctx *c, const double *xi, double *df, int *nblo
     return hipGetLastError();
 }
 
-hipError_t launch_reduce3(mdft_ctx *c, int nblocks) {
+hipError_t launch_reduce3_buf(mdft_ctx *c, const double *part, int nblocks, double *out) {
     constexpr int kSlices = 512;
     if (nblocks <= 4 * kThreads) {
-        hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_partials, nblocks, c->d_sums);
+        hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, part, nblocks, out);
         return hipGetLastError();
     }
     if (!c->d_slices) {
         hipError_t e = hipMalloc(&c->d_slices, sizeof(double) * 3 * kSlices);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(reduce3, dim3(kSlices), dim3(kThreads), 0, c->stream, c->d_partials, nblocks, c->d_slices);
-    hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_slices, kSlices, c->d_sums);
+    hipLaunchKernelGGL(reduce3, dim3(kSlices), dim3(kThreads), 0, c->stream, part, nblocks, c->d_slices);
+    hipLaunchKernelGGL(reduce3, dim3(1), dim3(kThreads), 0, c->stream, c->d_slices, kSlices, out);
     return hipGetLastError();
 }
+
+hipError_t launch_reduce3(mdft_ctx *c, int nblocks) { return launch_reduce3_buf(c, c->d_partials, nblocks, c->d_sums); }
 
 FinalConsts final_consts(mdft_ctx *c) {
     FinalConsts k;

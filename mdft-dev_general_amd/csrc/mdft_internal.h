@@ -95,6 +95,11 @@ struct mdft_ctx {
     float *df32 = nullptr;        // set during an evaluation with MDFT_DF_FP32: dF goes here, rounded to FP32
     double *d_vscale = nullptr;   // [2 no]: s(io) = sqrt(w(io) / mean w) and 1 / s(io) (scaled variables)
     const double *vsinv = nullptr;  // set during an evaluation with MDFT_SCALED_VARS: 1 / s
+    // set during mdft_energy_and_gradient_dots: products of dF with up to 3 vectors in the final pass
+    int dot_k = 0, dot_self = 0;
+    const void *dot_vec[3] = {nullptr, nullptr, nullptr};
+    double *d_dpart = nullptr;    // [blocks][3] their block partials
+    size_t dpart_cap = 0;
     float *d_chscale = nullptr;   // per-channel input scale of the warm-start features [2 np]
     float *d_adjscale = nullptr;  // per-channel factor of the adjoint of proj2angl [2 np]
     double *h_sums = nullptr;     // pinned mirror of d_sums
@@ -132,6 +137,7 @@ hipError_t launch_regrid(mdft_ctx *c, const double *prev, int pnx, int pny, cons
                          double outside, double *xi);
 hipError_t launch_proj_to_channels(mdft_ctx *c, float *out, float scale, const float *chscale = nullptr);
 hipError_t launch_channels_to_proj(mdft_ctx *c, const float *in);
+hipError_t launch_reduce3_buf(mdft_ctx *c, const double *part, int nblocks, double *out);
 hipError_t launch_cold_start(mdft_ctx *c, double *xi, bool boltzmann = false);
 hipError_t launch_density_mix(mdft_ctx *c, size_t total, double lam, const double *xa, const double *xb,
                               double *out);

@@ -81,6 +81,16 @@ module mdft_gpu
             real(c_double), intent(out) :: f_parts(3)
             integer(c_int), value :: flags
         end function
+        ! the same, with dots(j) = df . vecs(j), j <= k <= 3, formed in the final pass
+        integer(c_int) function mdft_energy_and_gradient_dots(ctx, xi, f_parts, df, flags, k, vecs, dots) &
+                bind(C, name="mdft_energy_and_gradient_dots")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, xi, df
+            real(c_double), intent(out) :: f_parts(3)
+            integer(c_int), value :: flags, k
+            type(c_ptr), intent(in) :: vecs(*)
+            real(c_double), intent(out) :: dots(*)
+        end function
         integer(c_int) function mdft_excess_energy(ctx, xi, fexc, df, flags) bind(C, name="mdft_excess_energy")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx, xi, df
@@ -201,7 +211,8 @@ module mdft_gpu
     end interface
 
     public :: mdft_ctx_create, mdft_ctx_destroy, mdft_get_info, mdft_set_ck, mdft_compute_vext, mdft_compute_vext_ex, mdft_set_vext
-    public :: mdft_energy_and_gradient, mdft_excess_energy, mdft_device_alloc, mdft_device_free
+    public :: mdft_energy_and_gradient, mdft_energy_and_gradient_dots, mdft_excess_energy, mdft_device_alloc
+    public :: mdft_device_free
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
     public :: mdft_vec_multi_dot, mdft_vec_combine, mdft_vec_combine_step
     public :: mdft_vec_multi_dot_f32, mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32

@@ -18,7 +18,7 @@
 ! or a relative decrease of F below ftol, or maxeval evaluations.
 module mdft_lbfgs
     use iso_c_binding
-    use mdft_gpu, only: mdft_energy_and_gradient, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
+    use mdft_gpu, only: mdft_energy_and_gradient, mdft_energy_and_gradient_dots, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
                         mdft_vec_multi_dot, mdft_vec_combine_step, mdft_vec_multi_dot_f32, &
                         mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32, &
                         mdft_scale_variables, MDFT_DEVICE_PTRS, MDFT_DF_FP32, MDFT_SCALED_VARS
@@ -126,11 +126,16 @@ contains
         iter = 0
         status = 2
         gcur = mm + 2
-        rc = mdft_energy_and_gradient(ctx, x, parts, vec(gcur), eflags); if (rc /= 0) goto 900
+        pv(1) = vec(gcur)
+        if (f32) then
+            rc = mdft_energy_and_gradient_dots(ctx, x, parts, vec(gcur), eflags, 1_c_int, pv, dots)
+        else
+            rc = mdft_energy_and_gradient(ctx, x, parts, vec(gcur), eflags)
+            if (rc == 0) rc = multi_dot(1)
+        end if
+        if (rc /= 0) goto 900
         neval = 1
         f = sum(parts)
-        pv(1) = vec(gcur)
-        rc = multi_dot(1); if (rc /= 0) goto 900
         gram(gcur, gcur) = dots(1)
         g0 = sqrt(dots(1))
         gnorm = g0
@@ -348,9 +353,6 @@ contains
                 if (rc /= 0) return
             end if
             t_ready = -1.0_dp
-            rc = mdft_energy_and_gradient(ctx, xn, pn, vec(fg), eflags); if (rc /= 0) return
-            neval = neval + 1
-            fn = sum(pn)
             ! with the memory full, the oldest pair (ds(1), gs(1)) is dropped after this
             ! iteration whatever the line search gives: its products with the new
             ! gradient are never used, so they are not computed
@@ -372,7 +374,17 @@ contains
             do a = 1, nl
                 pv(a) = vec(lst(a))
             end do
-            rc = multi_dot(nl); if (rc /= 0) return
+            if (nl <= 3 .and. f32) then
+                ! up to 3 products (m = 1) are formed by the evaluation as it writes the FP32
+                ! gradient (with FP64 vectors the separate pass is the faster one)
+                rc = mdft_energy_and_gradient_dots(ctx, xn, pn, vec(fg), eflags, int(nl, c_int), pv, dots)
+                if (rc /= 0) return
+            else
+                rc = mdft_energy_and_gradient(ctx, xn, pn, vec(fg), eflags); if (rc /= 0) return
+                rc = multi_dot(nl); if (rc /= 0) return
+            end if
+            neval = neval + 1
+            fn = sum(pn)
             do a = 1, nl
                 gram(fg, lst(a)) = dots(a)
                 gram(lst(a), fg) = dots(a)

@@ -33,7 +33,7 @@ MDFT_EXPORTS = (
     "mdft_vec_combine", "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj",
     "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_vec_combine_step",
     "mdft_charged_solute_corrections", "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32",
-    "mdft_vec_widen_f32", "mdft_scale_variables",
+    "mdft_vec_widen_f32", "mdft_scale_variables", "mdft_energy_and_gradient_dots",
 )
 
 
@@ -99,6 +99,7 @@ def load_library(path: str = LIBPATH):
     lib.mdft_vec_axpy_f32.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp]
     lib.mdft_vec_widen_f32.argtypes = [vp, C.c_size_t, vp, vp]
     lib.mdft_scale_variables.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp]
+    lib.mdft_energy_and_gradient_dots.argtypes = [vp, vp, _dp, vp, C.c_int, C.c_int, C.POINTER(vp), _dp]
     lib.mdft_angl2proj.argtypes = [vp, vp, C.c_size_t, vp]
     lib.mdft_proj2angl.argtypes = [vp, vp, C.c_size_t, vp]
     lib.mdft_regrid_trilinear.argtypes = [vp, vp, _ip, _dp, C.c_double, vp]
@@ -112,7 +113,7 @@ def load_library(path: str = LIBPATH):
                  "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj", "mdft_proj2angl",
                  "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_charged_solute_corrections",
                  "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32", "mdft_vec_widen_f32",
-                 "mdft_scale_variables"):
+                 "mdft_scale_variables", "mdft_energy_and_gradient_dots"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib

@@ -122,6 +122,22 @@ int mdft_energy_and_gradient(mdft_ctx *c, const double *xi, double *parts, doubl
     return 0;
 }
 
+int mdft_energy_and_gradient_dots(mdft_ctx *c, const double *xi, double *parts, void *df, int flags, int k,
+                                  const void *const *vecs, double *dots)
+{
+    int rc = mdft_energy_and_gradient(c, xi, parts, (double *)df, flags);
+    if (rc) return rc;
+    for (int j = 0; j < k; j++) {
+        double s = 0.0;
+        for (size_t i = 0; i < c->nvar; i++) {
+            if (flags & MDFT_DF_FP32) s += (double)((const float *)df)[i] * ((const float *)vecs[j])[i];
+            else s += ((const double *)df)[i] * ((const double *)vecs[j])[i];
+        }
+        dots[j] = s;
+    }
+    return 0;
+}
+
 int mdft_scale_variables(mdft_ctx *c, size_t n, int dir, int in_fp32, const void *in, double *out)
 {
     double *sc = scale_table(c);

@@ -359,3 +359,45 @@ def test_scaled_variables_minimise_faster():
         say(f"  {name}: {r.evaluations} evaluations, FP64 F - F* = {f64 - ref.F:+.4f}")
         assert r.status == 4 and abs(f64 - ref.F) <= 0.1
     assert out["fp64 scaled"] < out["fp64"] and out["fp32 scaled"] < out["fp32"]
+
+
+@pytest.mark.parametrize("precision,df32", [("fp64", False), ("fp32", False), ("fp32", True)])
+def test_gradient_products_in_the_evaluation(precision, df32):
+    """mdft_energy_and_gradient_dots: F and the gradient as mdft_energy_and_gradient
+    gives them, and dots[j] = dF . v_j formed in the final pass against numpy (FP64
+    sums of the stored gradient), the gradient itself among the vectors."""
+    import ctypes as C
+    from mdft.gpu import DEVICE_PTRS, DF_FP32, DeviceBuffer, Engine
+    w = workloads.small("solute20_128", 16)
+    say(f"[dots] {w.name} 16^3 {precision} df32={df32}")
+    eng = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, precision)
+    eng.set_ck(w.ck_table())
+    eng.compute_vext(w.solute(), w.solvent)
+    rng = np.random.default_rng(4)
+    n = eng.N * eng.no
+    xi = eng.alloc_field()
+    xi.upload(workloads.guess_xi(eng.get_vext(), w.kT) * (1 + 0.05 * rng.standard_normal(n)))
+    dtype = np.float32 if df32 else np.float64
+    flags = DEVICE_PTRS | (DF_FP32 if df32 else 0)
+    dp = C.POINTER(C.c_double)
+    g0 = DeviceBuffer(eng, np.dtype(dtype).itemsize * n)
+    p0 = np.zeros(3)
+    eng._check(eng.lib.mdft_energy_and_gradient(eng._h, xi.ptr, p0.ctypes.data_as(dp), g0.ptr, flags))
+    vs = [rng.standard_normal(n).astype(dtype) for _ in range(2)]
+    bv = []
+    for v in vs:
+        b = DeviceBuffer(eng, v.nbytes)
+        b.upload(v)
+        bv.append(b)
+    g1 = DeviceBuffer(eng, np.dtype(dtype).itemsize * n)
+    ptrs = (C.c_void_p * 3)(bv[0].ptr.value, g1.ptr.value, bv[1].ptr.value)
+    p1, dots = np.zeros(3), np.zeros(3)
+    eng._check(eng.lib.mdft_energy_and_gradient_dots(eng._h, xi.ptr, p1.ctypes.data_as(dp), g1.ptr, flags, 3, ptrs,
+                                                     dots.ctypes.data_as(dp)))
+    assert np.array_equal(p1, p0)
+    ga, gb = g0.download(dtype), g1.download(dtype)
+    assert np.array_equal(ga, gb)
+    g = gb.astype(np.float64)
+    ref = np.array([g @ vs[0].astype(np.float64), g @ g, g @ vs[1].astype(np.float64)])
+    say(f"  dots {dots}, numpy {ref}")
+    assert np.allclose(dots, ref, rtol=1e-12, atol=1e-12 * np.abs(g).max() * np.sqrt(n))
