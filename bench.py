@@ -1,10 +1,9 @@
 #!/usr/bin/env python3
 """Benchmark of the MDFT solver on one MI355X.
 
-Headline (BASELINE.json metric): seconds until the solvation free energy is
-within 0.05 kJ/mol of the converged FP64 value, for the 20-atom synthetic
-solute on the 128^3 grid (40 A box), mmax = 5 (330 orientations, 88
-projections), one GPU.
+Headline: seconds until the solvation free energy is within 0.05 kJ/mol of
+the converged FP64 value, for the 20-atom synthetic solute on the 128^3 grid
+(40 A box), mmax = 5 (330 orientations, 88 projections), one GPU.
 
 * plain FP64 path: reference cold start (xi = 0 inside the solute, 1
   elsewhere), FP64 evaluations, device-resident L-BFGS to |dF|/|dF0| < 1e-6;
@@ -12,12 +11,15 @@ projections), one GPU.
   Also reported: the same FP64 run timed only up to the first iterate within
   0.05 kJ/mol of Delta G_ref (band-to-band comparison).
 * fast path: the learned warm start (features of vext, network inference and
-  the guess, all inside the timed region) followed by FP32 evaluations
-  (angular transforms, FFTs, OZ in FP32; ideal term, reductions and L-BFGS in
-  FP64), timed up to the first iterate within 0.05 kJ/mol of Delta G_ref plus
-  one FP64 evaluation of that iterate, which must itself be within the band.
-  `value` is that time; speedup_vs_plain = T_plain / value.  Also reported:
-  the FP32 path from the cold start.
+  the guess mixed with the cold start, all inside the timed region) followed
+  by FP32 evaluations (angular transforms, FFTs, OZ and the per-orientation
+  terms in FP32; reductions in FP64) and the L-BFGS with FP64 scalars, FP32
+  vectors and the weight-scaled variables, timed up to the first iterate
+  within 0.05 kJ/mol of Delta G_ref plus one FP64 evaluation of that iterate,
+  which must itself be within the band.  `value` is that time;
+  speedup_vs_plain = T_plain / value.  Also reported: the FP64 method over the
+  scaled variables and the FP32 path from the cold start and from the
+  Boltzmann start, to the same band.
 
 Also reported: F+dF evaluations per second (one step = one evaluation over the
 whole grid, inputs resident in HBM), per-phase HIP-event timings on the
