@@ -325,11 +325,13 @@ def main():
         # seconds-to-band roofline of the fast path: each FP32 evaluation at its fused
         # algorithmic bytes and each L-BFGS iteration's vector passes at their bytes per
         # element -- the direction from 2m + 1 stored vectors with the trial point (x read,
-        # y written, FP64) and the multi-dot of the new gradient with 2m + 1 vectors -- all
-        # at the HBM peak
+        # y written, FP64), and the products of the new gradient with the 2m + 1 live
+        # vectors (with FP32 vectors at m = 1 formed in the final pass: the 2 vectors other
+        # than the gradient itself are read there) -- all at the HBM peak
         nvec = eng.N * eng.no
         vb = 4 if not args.fp64_vectors else 8
-        per_elem = 2 * (2 * args.memory + 1) * vb + vb + 16
+        dot_bytes = 2 * args.memory * vb if (vb == 4 and args.memory == 1) else (2 * args.memory + 1) * vb
+        per_elem = (2 * args.memory + 1) * vb + vb + 16 + dot_bytes
         t_roof = (res_f.evaluations * algorithmic_bytes(eng.N, eng.no, eng.np_, 8)
                   + res_f.iterations * per_elem * nvec) / (HBM_PEAK_GBS * 1e9)
         line["seconds_to_band_roofline"] = {"t_roofline_s": t_roof, "t_fast_s": t_fast, "frac": t_roof / t_fast,
