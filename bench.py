@@ -150,6 +150,8 @@ def main():
                     help="fast path: keep the minimiser's directions and gradients in FP64 (default FP32)")
     ap.add_argument("--unscaled", action="store_true",
                     help="fast path: minimise over xi itself (default: over u = sqrt(w / mean w) xi)")
+    ap.add_argument("--unit-first-step", action="store_true",
+                    help="fast path: first step of unit length along -g (default: the bulk Newton step)")
     ap.add_argument("--maxeval", type=int, default=1500)
     ap.add_argument("--mode", choices=["converge", "eval", "copy"], default="converge",
                     help="copy: a plain device copy of 1 GiB (z = x through mdft_vec_lincomb), the "
@@ -285,12 +287,15 @@ def main():
             t0 = time.perf_counter()
             if start == "net":
                 info.update(wsm.warm_start(e32, net, xi))
+            elif start == "net_pure":
+                wsm.warm_start(e32, net, xi, start_mix=1.0)
             elif start == "boltzmann":
                 wsm.boltzmann_start(e32, xi)
             else:
                 wsm.cold_start(e32, xi)
             r = minimize(e32, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6, ftarget=dg_ref,
                          fp32_vectors=not args.fp64_vectors, scaled=not args.unscaled,
+                         step0=0.0 if args.unit_first_step else "newton",
                          fband=band)
             c, _ = eng.energy_and_gradient(xi, None)
             eng.synchronize()
@@ -298,7 +303,9 @@ def main():
 
         t_cold32, res_c32, dg_c32 = fast("cold")
         t_boltz32, res_b32, dg_b32 = fast("boltzmann")
+        t_pure = res_pure = None
         if net is not None:
+            t_pure, res_pure, _ = fast("net_pure")
             t_fast, res_f, dg_fast = fast("net")
         else:
             t_fast, res_f, dg_fast = t_cold32, res_c32, dg_c32
@@ -345,11 +352,14 @@ def main():
             "t_fast_s": t_fast, "t_plain_fp64_s": t_conv, "speedup_vs_plain": t_conv / t_fast,
             "t_plain_fp64_to_band_s": t_plain, "speedup_band_to_band": t_plain / t_fast,
             "t_fp64_scaled_to_band_s": t_scaled64, "fp64_scaled_evaluations": res_s.evaluations,
+            "t_fp32_network_prediction_start_s": t_pure,
+            "fp32_network_prediction_start_evaluations": res_pure.evaluations if res_pure is not None else None,
             "t_fp32_cold_start_s": t_cold32, "t_fp32_boltzmann_start_s": t_boltz32,
             "fp32_cold_start_evaluations": res_c32.evaluations, "fp32_boltzmann_start_evaluations": res_b32.evaluations,
             "dG_fp32_boltzmann_error_kJmol": dg_b32 - dg_ref,
-            "fast_path": ("learned warm start (network inference inside the timed region) + FP32 evaluations + "
-                          f"L-BFGS (FP64 scalars, {vname} vectors, {sname} variables), FP64 confirmation")
+            "fast_path": (f"calibrated warm start ({info.get('start')}; inside the timed region) + FP32 "
+                          f"evaluations + L-BFGS (FP64 scalars, {vname} vectors, {sname} variables, "
+                          f"{'unit' if args.unit_first_step else 'bulk Newton'} first step), FP64 confirmation")
                          if net is not None else
                          f"FP32 evaluations + L-BFGS (FP64 scalars, {vname} vectors, {sname} variables) from the "
                          "reference cold start, FP64 confirmation (no trained warm-start weights found)",

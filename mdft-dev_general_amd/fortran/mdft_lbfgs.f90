@@ -40,7 +40,8 @@ contains
         real(c_double), value :: gtol, ftol, fnoise, ftarget, fband
         real(c_double), intent(out) :: stats(16)
         real(c_double), intent(out) :: trace(3, *)
-        rc = mdft_minimize_ex(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, stats, trace, ntrace, 0)
+        rc = mdft_minimize_ex(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, stats, trace, ntrace, 0, &
+                              0.0_dp)
     end function
 
     ! Minimise F over xi (device vector of length n), starting from the values in xi.
@@ -63,12 +64,17 @@ contains
     ! every term's curvature in xi carries the quadrature weight w(io), which spans a factor
     ! of about 4 over the theta nodes at mmax = 5; in u it does not.  xi is converted in
     ! place on entry and back on return.
+    ! step0 > 0: the first trial step along -g of an iteration without curvature pairs (the
+    ! first, or after a restart), in place of the unit-length move t = 1/|g|; a caller that
+    ! knows the functional's curvature c along the variables passes 1/c, the Newton step.
     ! xi and g (FP64) receive the last iterate and its gradient dF/dxi in every mode.
     integer(c_int) function mdft_minimize_ex(ctx, n, xi, g, m, gtol, ftol, fnoise, ftarget, fband, maxeval, &
-                                             stats, trace, ntrace, opts) bind(C, name="mdft_minimize_ex") result(rc)
+                                             stats, trace, ntrace, opts, step0) bind(C, name="mdft_minimize_ex") &
+                                             result(rc)
         type(c_ptr), value :: ctx, xi, g
         integer(c_size_t), value :: n
         integer(c_int), value :: m, maxeval, ntrace, opts
+        real(c_double), value :: step0
         real(c_double), value :: gtol, ftol, fnoise, ftarget, fband
         real(c_double), intent(out) :: stats(16)
         real(c_double), intent(out) :: trace(3, *)
@@ -174,14 +180,14 @@ contains
                 r = r + (alpha(i) - beta) * sv
             end do
             t = 1.0_dp
-            if (k == 0) t = 1.0_dp / gnorm       ! first step: unit-length move along -g
+            if (k == 0) t = first_step()         ! first step along -g
             gd = -gdot(unit(gcur), r)            ! directional derivative along -d
             if (gd >= 0.0_dp) then               ! not a descent direction: restart from steepest descent
                 k = 0
                 gs(1) = gcur
                 r = unit(gcur)
                 gd = -gnorm * gnorm
-                t = 1.0_dp / gnorm
+                t = first_step()
             end if
             ! ---- the new direction d = sum_j r_j b_j into a free direction slot
             fd = free_direction()
@@ -284,6 +290,12 @@ contains
         call release()
 
     contains
+
+        ! the step of an iteration without curvature information: step0, or a unit-length move
+        real(dp) function first_step()
+            first_step = 1.0_dp / gnorm
+            if (step0 > 0.0_dp) first_step = step0
+        end function
 
         ! dots(1:nl_) = pv(j) . pv(nl_): the last vector of pv is the new gradient itself
         integer(c_int) function multi_dot(nl_) result(r)

@@ -32,7 +32,7 @@ def host_library():
         lib.mdft_minimize.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int]
         lib.mdft_minimize.restype = C.c_int
-        lib.mdft_minimize_ex.argtypes = lib.mdft_minimize.argtypes + [C.c_int]
+        lib.mdft_minimize_ex.argtypes = lib.mdft_minimize.argtypes + [C.c_int, C.c_double]
         lib.mdft_minimize_ex.restype = C.c_int
         _HOST = lib
     return _HOST
@@ -57,7 +57,7 @@ class MinResult:
 def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: int = 5, gtol: float = 1e-6,
              ftol: float = 0.0, fnoise: float = 1e-7, maxeval: int = 2000, ntrace: int = 2000,
              ftarget: float = 0.0, fband: float = 0.0, fp32_vectors: bool = False, scaled: bool = False,
-             lib=None) -> MinResult:
+             step0: float | str = 0.0, compressibility_s0: float = 1.0 / 16.0, lib=None) -> MinResult:
     """Minimise F over xi (in place).  Returns the final F and statistics.
     fnoise is the relative noise level of F used to relax the sufficient-decrease
     test; with fband > 0 the run stops at the first iterate with |F - ftarget| <= fband.
@@ -65,8 +65,18 @@ def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: in
     bytes per vector pass; meant for FP32 evaluations); g still receives FP64.
     scaled minimises over u = s xi with s(io) = sqrt(w(io) / mean w), which takes the
     quadrature weight out of the curvature (fewer iterations at the same minimum);
-    xi and g are returned in the caller's variables."""
+    xi and g are returned in the caller's variables.
+    step0 is the first trial step along -g (0: a unit-length move, the reference's
+    choice); "newton" takes S0 / (4 rho0 kT mean(w)): 1 / (4 rho0 kT mean(w)) is the
+    inverse curvature of the ideal term at the bulk in the scaled variables (in the
+    gradient's units, (1/dv) dF/du), and S0 = 1 / (1 - rho0 c(0)) the structure factor
+    at k = 0 (compressibility_s0, 1/16 by default: water's S(0) is 0.06), which the
+    excess term adds to the curvature of long-wavelength density changes."""
     lib = lib or host_library()
+    if step0 == "newton":
+        rho0 = eng.n0 * eng.mrso / (8.0 * np.pi ** 2)
+        wmean = 8.0 * np.pi ** 2 / eng.mrso / eng.no
+        step0 = compressibility_s0 / (4.0 * rho0 * eng.kT * wmean)
     own = g is None
     if own:
         g = eng.alloc_field()
@@ -76,7 +86,7 @@ def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: in
                               float(fnoise), float(ftarget), float(fband), int(maxeval),
                               stats.ctypes.data_as(C.POINTER(C.c_double)),
                               trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace),
-                              int(bool(fp32_vectors)) | (2 if scaled else 0))
+                              int(bool(fp32_vectors)) | (2 if scaled else 0), float(step0))
     if rc != 0:
         raise MdftError(rc, eng.lib.mdft_last_error(eng._h).decode())
     it = int(stats[6])

@@ -25,10 +25,11 @@ Calibration of the start (run after training, on the target workload):
     python -m mdft.train_warmstart --calibrate-mix --weights PATH
 
 solves 128^3 solutes of seeds 8000-8003 (not the benchmark's solute) from the
-starts (1 - lam) rho_cold + lam rho_pred for lam in 0.5 .. 1.0 with the fast
-path's minimiser (FP32 evaluations and vectors, scaled variables) and stores the
-lam of the fewest mean evaluations to the 0.05 kJ/mol band with the weights
-(`start_mix`), with the table it came from.
+starts (1 - lam) rho_cold + lam rho_pred for lam in {0, 0.5 .. 1.0} (lam = 0: the
+cold start itself) with the fast path's minimiser (FP32 evaluations and vectors,
+scaled variables, Newton first step) and stores the lam of the fewest mean
+evaluations to the 0.05 kJ/mol band with the weights (`start_mix`), with the
+table it came from.
 """
 from __future__ import annotations
 
@@ -44,7 +45,7 @@ from . import warmstart as ws
 from . import workloads
 
 
-def calibrate_mix(path: str, seeds=(8000, 8001, 8002, 8003), lams=(0.5, 0.6, 0.7, 0.8, 0.9, 1.0), log=print):
+def calibrate_mix(path: str, seeds=(8000, 8001, 8002, 8003), lams=(0.0, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0), log=print):
     """Choose the net's start mixture on solutes of the 128^3 workload (see the module text)."""
     from .gpu import Engine
     from .minimize import minimize
@@ -61,7 +62,7 @@ def calibrate_mix(path: str, seeds=(8000, 8001, 8002, 8003), lams=(0.5, 0.6, 0.7
         for j, lam in enumerate(lams):
             ws.warm_start(eng, net, xi, start_mix=lam)
             r = minimize(eng, xi, g=g, m=1, gtol=0.0, maxeval=400, fnoise=1e-6, ftarget=ref.F, fband=0.05,
-                         fp32_vectors=True, scaled=True)
+                         fp32_vectors=True, scaled=True, step0="newton")
             evals[i, j] = r.evaluations
         log(f"  solute {seed}: F*={ref.F:.3f}; evaluations to the band for lam {list(lams)}: {list(evals[i])}")
     best = float(lams[int(np.argmin(evals.mean(axis=0)))])

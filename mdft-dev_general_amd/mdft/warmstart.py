@@ -288,6 +288,10 @@ def warm_start(eng: Engine, net: WarmStartNet, xi: DeviceBuffer, fallback: Devic
     evaluations and never gives a start of higher F than the cold one."""
     assert net.channels[-1] == 2 * eng.np_ and net.channels[0] == 2 * eng.np_
     lam = float(net.start_mix if start_mix is None else start_mix)
+    if not lams and lam <= 0.0 and kind == WS_DENSITY:
+        # the calibration gives the prediction no weight: the cold start, without inference
+        cold_start(eng, xi)
+        return {"start": "cold (calibrated lam = 0, network not run)", "kind": "density", "lam": 0.0}
     f = features(eng, net.input_scale)
     pred = DeviceBuffer(eng, 4 * net.channels[-1] * eng.N)
     net.forward(eng.n, f, pred)

@@ -373,7 +373,7 @@ def test_trained_network_reduces_evaluations():
         ws.cold_start(eng, xi)
         f_cold = ws._F(eng, xi)
         rc = minimize(eng, xi, g=gb, m=5, gtol=0.0, maxeval=600, fnoise=1e-6, ftarget=ref.F, fband=band)
-        ws.warm_start(eng, net, xi)
+        ws.warm_start(eng, net, xi, start_mix=1.0)          # the prediction itself
         f_warm = ws._F(eng, xi)
         rw = minimize(eng, xi, g=gb, m=5, gtol=0.0, maxeval=600, fnoise=1e-6, ftarget=ref.F, fband=band)
         say(f"  solute {seed} ({len(sol.q)} atoms): F*={ref.F:.3f}; cold start F-F*={f_cold - ref.F:.1f}, "

@@ -24,7 +24,7 @@ def host(oracle_mod):
     lib.mdft_minimize.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                                   C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int]
     lib.mdft_minimize.restype = C.c_int
-    lib.mdft_minimize_ex.argtypes = lib.mdft_minimize.argtypes + [C.c_int]
+    lib.mdft_minimize_ex.argtypes = lib.mdft_minimize.argtypes + [C.c_int, C.c_double]
     lib.mdft_minimize_ex.restype = C.c_int
     mock = C.CDLL(os.path.join(MOCK, "_build", "libmdft_abi_mock.so"))
     mock.mdft_ctx_create.argtypes = [C.POINTER(Desc), C.POINTER(C.c_void_p)]
@@ -49,14 +49,14 @@ def _problem(mock, w, sol, t):
     return h, vext
 
 
-def _minimize(lib, h, x, m=5, gtol=1e-6, fnoise=1e-7, maxeval=600, ftarget=0.0, fband=0.0, vec32=None):
+def _minimize(lib, h, x, m=5, gtol=1e-6, fnoise=1e-7, maxeval=600, ftarget=0.0, fband=0.0, vec32=None, step0=0.0):
     g = np.zeros_like(x)
     stats = np.zeros(16)
     trace = np.zeros((maxeval, 3))
     dp = C.POINTER(C.c_double)
     args = (h, x.size, x.ctypes.data, g.ctypes.data, m, gtol, 0.0, fnoise, ftarget, fband, maxeval,
             stats.ctypes.data_as(dp), trace.ctypes.data_as(dp), maxeval)
-    rc = lib.mdft_minimize(*args) if vec32 is None else lib.mdft_minimize_ex(*args, int(vec32))
+    rc = lib.mdft_minimize(*args) if vec32 is None else lib.mdft_minimize_ex(*args, int(vec32), float(step0))
     assert rc == 0
     return stats, trace[:int(stats[6])], g
 
