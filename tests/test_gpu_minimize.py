@@ -330,7 +330,8 @@ def test_scaled_variables_evaluation(precision):
 def test_scaled_variables_minimise_faster():
     """The minimiser over u = s xi reaches the 0.05 kJ/mol band of the same minimum in
     fewer evaluations than over xi (24^3, mmax=5, charged solute, FP64 and FP32 with
-    FP32 vectors); evaluations reported."""
+    FP32 vectors), and the bulk Newton first step in no more than the unit-length
+    one; evaluations reported."""
     from mdft import warmstart as ws
     from mdft.gpu import Engine
     from mdft.minimize import minimize
@@ -351,7 +352,9 @@ def test_scaled_variables_minimise_faster():
     out = {}
     for name, eng, kw in (("fp64", e64, {}), ("fp64 scaled", e64, {"scaled": True}),
                           ("fp32", e32, {"fp32_vectors": True, "fnoise": 1e-6}),
-                          ("fp32 scaled", e32, {"fp32_vectors": True, "fnoise": 1e-6, "scaled": True})):
+                          ("fp32 scaled", e32, {"fp32_vectors": True, "fnoise": 1e-6, "scaled": True}),
+                          ("fp32 scaled newton", e32, {"fp32_vectors": True, "fnoise": 1e-6, "scaled": True,
+                                                       "step0": "newton"})):
         ws.cold_start(eng, x)
         r = minimize(eng, x, m=1, gtol=0.0, maxeval=1500, ftarget=ref.F, fband=0.05, **kw)
         f64 = float(np.sum(e64.energy_and_gradient(x, None)[0]))
@@ -359,6 +362,7 @@ def test_scaled_variables_minimise_faster():
         say(f"  {name}: {r.evaluations} evaluations, FP64 F - F* = {f64 - ref.F:+.4f}")
         assert r.status == 4 and abs(f64 - ref.F) <= 0.1
     assert out["fp64 scaled"] < out["fp64"] and out["fp32 scaled"] < out["fp32"]
+    assert out["fp32 scaled newton"] <= out["fp32 scaled"]      # the bulk Newton first step
 
 
 @pytest.mark.parametrize("precision,df32", [("fp64", False), ("fp32", False), ("fp32", True)])

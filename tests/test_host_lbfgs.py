@@ -182,4 +182,9 @@ def test_fortran_lbfgs_scaled_variables(host):
     xs = x0.copy()
     ss, _, _ = _minimize(lib, h, xs, m=1, gtol=1e-8, maxeval=3000, vec32=2)
     assert abs(ss[0] - sa[0]) < 1e-7 * abs(sa[0])
+    # a given first step (here far too long: the line search must bring it back) reaches
+    # the same minimum
+    xt = x0.copy()
+    st, _, _ = _minimize(lib, h, xt, m=1, gtol=1e-8, maxeval=3000, vec32=2, step0=1e4)
+    assert abs(st[0] - sa[0]) < 1e-7 * abs(sa[0])
     mock.mdft_ctx_destroy(h)
