@@ -171,6 +171,7 @@ __global__ __launch_bounds__(64 * kWaves) void oz32(float2 *__restrict__ proj, O
             const float2 *y = Y + r.x;
             const float *dc = d + r.z;
             float sx = 0.0f, sy = 0.0f;
+#pragma unroll 4
             for (int t = 0; t < W; t++) {
                 const float dv = dc[t * W];
                 const float2 v = y[t * r.y];
@@ -192,6 +193,7 @@ __global__ __launch_bounds__(64 * kWaves) void oz32(float2 *__restrict__ proj, O
             const int *ctk = ctab + kk * kCMax;
             const float2 *lo = c0 + tia0[ip], *hi = c1 + tia0[ip];
             float qx = 0.0f, qy = 0.0f, mx = 0.0f, my = 0.0f;
+#pragma unroll 4
             for (int c = 0; c < nc; c++) {
                 const int ent = ctk[c];
                 const int off = ent >> 1;
@@ -224,6 +226,7 @@ __global__ __launch_bounds__(64 * kWaves) void oz32(float2 *__restrict__ proj, O
             const float2 *g = G + r.x;
             const float *dr = d + r.z;
             float sx = 0.0f, sy = 0.0f;
+#pragma unroll 4
             for (int t = 0; t < W; t++) {
                 const float dv = dr[second ? (W - 1 - t) : t];  // d(mup, -khi) for the -q side
                 const float2 v = g[t * r.y];
