@@ -690,13 +690,15 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     }
     int nb = 0;
     c->df32 = df32 ? reinterpret_cast<float *>(df) : nullptr;
-    c->dot_k = dot_k;
+    // the FP32 final pass forms the products; an FP64 context runs the multi-dot after it
+    const bool fused_dots = dot_k > 0 && c->d.precision == MDFT_PREC_FP32;
+    c->dot_k = fused_dots ? dot_k : 0;
     const hipError_t fe = launch_final(c, xd, dd, mode, &nb);
     c->df32 = nullptr;
     c->vsinv = nullptr;
     c->dot_k = 0;
     HIPCHK(c, fe);
-    if (dot_k > 0) {
+    if (fused_dots) {
         HIPCHK(c, launch_reduce3_buf(c, c->d_dpart, nb, c->d_sums + 5));
         HIPCHK(c, hipMemcpyAsync(c->h_sums + 5, c->d_sums + 5, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
     }
@@ -713,7 +715,16 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     sums[2] *= 0.5 * c->dv;  // ff = ff * 0.5 * dv (cproj_mrso.f90:745)
-    for (int j = 0; j < dot_k; j++) dots[j] = c->h_sums[5 + j];  // read_sums synchronised the stream
+    if (fused_dots) {
+        for (int j = 0; j < dot_k; j++) dots[j] = c->h_sums[5 + j];  // read_sums synchronised the stream
+    } else if (dot_k > 0) {
+        const size_t n = c->N * (size_t)c->no;
+        if (df32)
+            HIPCHK(c, vec_multi_dot32(c, n, dot_k, reinterpret_cast<const float *const *>(dot_vecs),
+                                      reinterpret_cast<const float *>(df), dots));
+        else
+            HIPCHK(c, vec_multi_dot(c, n, dot_k, reinterpret_cast<const double *const *>(dot_vecs), df, dots));
+    }
     return MDFT_OK;
 }
 

@@ -407,8 +407,17 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
 #pragma unroll
         for (int iphi = 0; iphi < NPHI; iphi++) {
             const V v = g[iphi * H1], e = tw[iphi];
-            re += v.x * e.x - v.y * e.y;
-            im += v.x * e.y + v.y * e.x;
+            if constexpr (sizeof(T) == 4) {
+                // four multiply-adds per complex product (the plain expression compiles to six;
+                // in FP64 the plain form keeps the kernel's register count)
+                re = fma(v.x, e.x, re);
+                re = fma(-v.y, e.y, re);
+                im = fma(v.x, e.y, im);
+                im = fma(v.y, e.x, im);
+            } else {
+                re += v.x * e.x - v.y * e.y;
+                im += v.x * e.y + v.y * e.x;
+            }
         }
         F[i] = mk<V>(re, im);
     }
@@ -517,8 +526,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
 #pragma unroll
         for (int j = 0; j < W; j++) {
             const V v = fr[j * H1], e = tw[j];
-            re += v.x * e.x + v.y * e.y;
-            im += v.x * e.y - v.y * e.x;
+            if constexpr (sizeof(T) == 4) {  // four multiply-adds (FP32 only, as in angl2proj_t)
+                re = fma(v.x, e.x, re);
+                re = fma(v.y, e.y, re);
+                im = fma(v.x, e.y, im);
+                im = fma(-v.y, e.x, im);
+            } else {
+                re += v.x * e.x + v.y * e.y;
+                im += v.x * e.y - v.y * e.x;
+            }
         }
         h[i] = mk<V>(re, im);
     }
@@ -536,7 +552,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         const V *tp = twpsi + ipsi * H1;
         T vt = hr[0].x;
 #pragma unroll
-        for (int mu2 = 1; mu2 < H1; mu2++) vt += hr[mu2].x * tp[mu2].x - hr[mu2].y * tp[mu2].y;
+        for (int mu2 = 1; mu2 < H1; mu2++) {
+            if constexpr (sizeof(T) == 4) vt = fma(-hr[mu2].y, tp[mu2].y, fma(hr[mu2].x, tp[mu2].x, vt));
+            else vt += hr[mu2].x * tp[mu2].x - hr[mu2].y * tp[mu2].y;
+        }
         const size_t g = r0 * NO + i;
         if constexpr (sizeof(T) == 4) {
             if (mode == 0) {
@@ -595,13 +614,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
             const double gv = (gid + 2.0 * rho0 * x * vexc) * si;
             if (df32) df32[g] = (float)gv;
             else if (df) df[g] = gv;
-            if (da.k > 0) {
-                // the products see the gradient as stored
-                const double gs = df32 ? (double)(float)gv : gv;
-                d0 += gs * dot_operand(da, 0, g, gs);
-                if (da.k > 1) d1 += gs * dot_operand(da, 1, g, gs);
-                if (da.k > 2) d2 += gs * dot_operand(da, 2, g, gs);
-            }
+            // (the FP64 kernel forms no gradient products: its context runs the separate
+            // multi-dot, which keeps this kernel's registers where they were)
         } else if (mode == 1) {
             if (df) df[g] += 2.0 * rho0 * x * vexc;
         } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
@@ -616,9 +630,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         }
     }
     block_reduce3(fid, fext, fexc, partials);
-    if (da.k > 0) {
-        __syncthreads();
-        block_reduce3(d0, d1, d2, dpart);
+    if constexpr (sizeof(T) == 4) {
+        if (da.k > 0) {
+            __syncthreads();
+            block_reduce3(d0, d1, d2, dpart);
+        }
     }
 }
 
