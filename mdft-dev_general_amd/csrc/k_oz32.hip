@@ -208,10 +208,15 @@ __global__ __launch_bounds__(64 * kWaves) void oz32(float2 *__restrict__ proj, O
                     u = make_float2(bb.x, -bb.y);
                     v = make_float2(aa.x, -aa.y);
                 }
-                qx += cx * u.x - cy * u.y;
-                qy += cx * u.y + cy * u.x;
-                mx += cx * v.x - cy * v.y;
-                my += cx * v.y + cy * v.x;
+                // four multiply-adds per complex product (the plain expression compiles to six)
+                qx = fmaf(cx, u.x, qx);
+                qx = fmaf(-cy, u.y, qx);
+                qy = fmaf(cx, u.y, qy);
+                qy = fmaf(cy, u.x, qy);
+                mx = fmaf(cx, v.x, mx);
+                mx = fmaf(-cy, v.y, mx);
+                my = fmaf(cx, v.y, my);
+                my = fmaf(cy, v.x, my);
             }
             G[ip] = make_float2(qx, qy);
             G[np + ip] = make_float2(mx, my);
