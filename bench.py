@@ -227,7 +227,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 (FP64 reductions, ideal term, L-BFGS)",
+        "dtype": {"fp64": "f64", "fp32": "f32 (f32 transforms and per-orientation terms, FP64 sums)",
                   "bf16": "bf16 MFMA angular transforms, f32 FFT/OZ (FP64 reductions, ideal term, L-BFGS)"}[args.precision],
         "data": "synthetic (seeded 20-atom solute, vext with Poisson electrostatics as the reference by default, seeded c(k) with the convex self term, reference cold start)",
         "config": {"workload": w.name, "grid": list(w.n), "box_A": list(w.length), "mmax": w.mmax,
@@ -363,8 +363,8 @@ def main():
                          if net is not None else
                          f"FP32 evaluations + L-BFGS (FP64 scalars, {vname} vectors, {sname} variables) from the "
                          "reference cold start, FP64 confirmation (no trained warm-start weights found)",
-            "dtype": f"f32 evaluations (FP64 reductions and ideal term), L-BFGS with FP64 scalars and {vname} "
-                     "vectors, f64 confirmation; the plain path all f64",
+            "dtype": f"f32 evaluations (f32 transforms and per-orientation terms, f64 sums), L-BFGS with FP64 "
+                     f"scalars and {vname} vectors, f64 confirmation; the plain path all f64",
             "dG_ref_kJmol": dg_ref, "dG_fast_confirmed_kJmol": dg_fast,
             "dG_fast_error_kJmol": dg_fast - dg_ref, "within_band": bool(abs(dg_fast - dg_ref) <= band),
             "dG_fp32_cold_error_kJmol": dg_c32 - dg_ref,

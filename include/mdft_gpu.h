@@ -105,11 +105,13 @@ int mdft_set_ck(mdft_ctx *ctx, int np_file, const int *m, const int *n, const in
  * the full gradient with the reference convention df = (1/dv) dF/dxi; it is
  * overwritten, not accumulated. */
 int mdft_energy_and_gradient(mdft_ctx *ctx, const double *xi, double *f_parts, double *df, int flags);
-/* The same with the products of the new gradient and k <= 3 device vectors, formed
- * while the gradient is written (the minimiser's multi-dot without its pass over
- * the gradient): dots[j] = df . vecs[j] (host), vecs[j] FP32 with MDFT_DF_FP32,
- * else FP64; vecs[j] == df gives |df|^2.  Needs MDFT_DEVICE_PTRS, molecular
- * symmetry order 2 and an FP64 or FP32 context.  No reference counterpart. */
+/* The same with the products of the new gradient and k <= 3 device vectors:
+ * dots[j] = df . vecs[j] (host), vecs[j] FP32 with MDFT_DF_FP32, else FP64;
+ * vecs[j] == df gives |df|^2.  An FP32 context forms them in its final pass while
+ * the gradient is written (the minimiser's multi-dot without its own pass over
+ * the gradient); an FP64 context runs the multi-dot after the evaluation.  Needs
+ * MDFT_DEVICE_PTRS, molecular symmetry order 2 and an FP64 or FP32 context.  No
+ * reference counterpart. */
 int mdft_energy_and_gradient_dots(mdft_ctx *ctx, const double *xi, double *f_parts, void *df, int flags, int k,
                                   const void *const *vecs, double *dots);
 

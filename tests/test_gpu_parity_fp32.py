@@ -1,6 +1,6 @@
-"""GPU parity of the FP32 path (MDFT_PREC_FP32: angular transforms, FFTs and the
-OZ contraction in FP32; ideal term, reductions and dF assembly in FP64) against
-the FP64 CPU oracle on identical inputs.
+"""GPU parity of the FP32 path (MDFT_PREC_FP32: angular transforms, FFTs, the OZ
+contraction and the per-orientation ideal, external and excess terms in FP32;
+the sums in FP64) against the FP64 CPU oracle on identical inputs.
 
 Tolerances (FP32 path): every F part relative 1e-5, total F relative 1e-5,
 dF max abs difference <= 1e-5 of max |dF|, projection stages <= 1e-5 of the
