@@ -351,11 +351,12 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     // so its HBM latency overlaps the staging
     constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
     const double *src = xi + r0 * NO;
-    double xr[NIT];
+    double xr[NIT], sr[NIT];
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
         const int i = tid + q * kThreads;
         xr[q] = (i < npts * NO) ? src[i] : 0.0;
+        sr[q] = vsinv ? vsinv[i % NO] : 1.0;  // scaled variables: xi = u / s(io), loaded with the tile
     }
     for (int i = tid; i < H1 * NPSI; i += kThreads) {
         const int kk = ((i / NPSI) * (i % NPSI)) % NPSI;
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     for (int q = 0; q < NIT; q++) {
         const int i = tid + q * kThreads;
         if (i >= npts * NO) break;
-        const double x = vsinv ? xr[q] * vsinv[i % NO] : xr[q];  // scaled variables: xi = u / s(io)
+        const double x = vsinv ? xr[q] * sr[q] : xr[q];
         if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
         else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
         else if (feat == 2) o[i] = (T)(x * x - 1.0);
@@ -456,6 +457,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     __shared__ V twpsi[NPSI * H1];  // 2 (cos, sin)(2 pi mu2 ipsi / npsi) at [ipsi][mu2]
     __shared__ T p2a[NP * NTH];
     __shared__ double wq[NO];
+    __shared__ double sth[NTH];  // 1 / s of each theta (the weights, hence s, depend on theta only)
     __shared__ int pidx[(L + 1) * W * H1];
     const int tid = threadIdx.x;
     const size_t r0 = (size_t)blockIdx.x * PT;
@@ -488,6 +490,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     }
     for (int i = tid; i < NP * NTH; i += kThreads) p2a[i] = (T)t.p2a[i];
     for (int i = tid; i < NO; i += kThreads) wq[i] = t.w[i];
+    for (int i = tid; i < NTH; i += kThreads) sth[i] = vsinv ? vsinv[i * NPHI * NPSI] : 1.0;
     const V *src = proj + r0 * NP;
     for (int i = tid; i < npts * NP; i += kThreads) Gm[i] = src[i];
     __syncthreads();
@@ -562,7 +565,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                 // FP32 path: the per-orientation terms in FP32 (the transforms feeding them
                 // are FP32 already), summed in FP64; rho/rho0 - 1 is formed in FP64 first so
                 // that the excess term keeps its small deviations from the bulk
-                const double si = vsinv ? vsinv[io] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
+                const double si = vsinv ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
                 const double x = xr[q] * si;
                 const double xx = x * x;
                 const float wf = (float)wq[io];
@@ -596,7 +599,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         const double v = (double)vt;
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
-        const double si = (mode == 0 && vsinv) ? vsinv[io] : 1.0;  // scaled variables (mode 0)
+        const double si = (mode == 0 && vsinv) ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables (mode 0)
         const double x = xr[q] * si;
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
