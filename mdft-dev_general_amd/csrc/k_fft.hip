@@ -269,17 +269,26 @@ __global__ __launch_bounds__(kThreads) void fft_axis_r(V *__restrict__ data, int
     extern __shared__ __align__(16) unsigned char fft_smem[];
     V *A = reinterpret_cast<V *>(fft_smem), *twl = A + PB * (N + 1);
     const T sgn = (T)sgn_d, scale = (T)scale_d;
-    const int line = blockIdx.x;
+    int line = blockIdx.x, by = blockIdx.y;
+    if (g.es == 1 && np % PB != 0 && (gridDim.x & 7) == 0) {
+        // along x the PB-runs of every other point straddle two cache lines, whose other
+        // halves belong to the neighbouring batches of the same line: workgroups go to
+        // the 8 XCDs round robin in dispatch order, so the batches of one line are given
+        // consecutive slots of one XCD (one L2), each line is read from HBM once
+        const unsigned id = blockIdx.x + gridDim.x * blockIdx.y, k = id >> 3;
+        by = (int)(k % gridDim.y);
+        line = (int)((k / gridDim.y) * 8 + (id & 7));
+    }
     const int a = line % g.na, b = line / g.na;
     const long long base = (long long)a * g.sa + (long long)b * g.sb;
     int p0, pb;
     const int s = g.align > 0 ? (int)((g.align - (base * np) % g.align) % g.align) : 0;
     if (s == 0) {
-        p0 = blockIdx.y * PB;
+        p0 = by * PB;
         pb = (np - p0) < PB ? (np - p0) : PB;
     } else {
-        p0 = blockIdx.y == 0 ? 0 : s + (blockIdx.y - 1) * PB;
-        pb = blockIdx.y == 0 ? s : ((np - p0) < PB ? (np - p0) : PB);
+        p0 = by == 0 ? 0 : s + (by - 1) * PB;
+        pb = by == 0 ? s : ((np - p0) < PB ? (np - p0) : PB);
     }
     if (pb <= 0) return;
     for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = mk<V>((T)tw[i].x, (T)tw[i].y);
