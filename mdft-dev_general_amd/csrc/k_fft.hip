@@ -270,14 +270,17 @@ __global__ __launch_bounds__(kThreads) void fft_axis_r(V *__restrict__ data, int
     V *A = reinterpret_cast<V *>(fft_smem), *twl = A + PB * (N + 1);
     const T sgn = (T)sgn_d, scale = (T)scale_d;
     int line = blockIdx.x, by = blockIdx.y;
-    if (g.es == 1 && np % PB != 0 && (gridDim.x & 7) == 0) {
-        // along x the PB-runs of every other point straddle two cache lines, whose other
-        // halves belong to the neighbouring batches of the same line: workgroups go to
-        // the 8 XCDs round robin in dispatch order, so the batches of one line are given
-        // consecutive slots of one XCD (one L2), each line is read from HBM once
-        const unsigned id = blockIdx.x + gridDim.x * blockIdx.y, k = id >> 3;
-        by = (int)(k % gridDim.y);
-        line = (int)((k / gridDim.y) * 8 + (id & 7));
+    if (((g.es == 1 && np % PB != 0) || g.align > 0) && ((gridDim.x * gridDim.y) & 7) == 0) {
+        // cache lines shared between workgroups: along x the PB-runs of every other point
+        // straddle two lines, whose other halves belong to the neighbouring batches of the
+        // same line; with aligned batches (y, z) the partial lines at the ends of a point's
+        // record belong to the neighbouring lines.  Workgroups go to the 8 XCDs round robin
+        // in dispatch order, so each XCD (one L2) is given a contiguous range of lines with
+        // the batches of each line in consecutive slots: a shared line is read from HBM once
+        const unsigned id = blockIdx.x + gridDim.x * blockIdx.y, per = gridDim.x * gridDim.y / 8;
+        const unsigned G = (id & 7) * per + (id >> 3);
+        line = (int)(G / gridDim.y);
+        by = (int)(G % gridDim.y);
     }
     const int a = line % g.na, b = line / g.na;
     const long long base = (long long)a * g.sa + (long long)b * g.sb;
