@@ -41,6 +41,10 @@ CASES = [
     ("mmax3 24x20x18", (24, 20, 18), (14.0, 13.0, 12.0), 3, 256),
     ("mmax5 16^3", (16, 16, 16), (12.0, 12.0, 12.0), 5, 256),
     ("mmax4 12^3 odd mix", (12, 10, 15), (10.0, 9.0, 11.0), 4, 256),
+    # one 192- or 96-point axis each: the radix-3 FFT kernels of the 192^3 and 96^3 workloads
+    ("fft x192 192x8x6", (192, 8, 6), (48.0, 4.0, 3.0), 1, 1024),
+    ("fft y96 10x96x12", (10, 96, 12), (5.0, 30.0, 6.0), 2, 1024),
+    ("fft z192 8x6x192", (8, 6, 192), (4.0, 3.0, 60.0), 1, 1024),
 ]
 
 
