@@ -646,6 +646,10 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     if (scaled && (mode != 0 || c->d.mrso != 2 || c->d.precision == MDFT_PREC_BF16))
         return fail(c, MDFT_EINVAL, "MDFT_SCALED_VARS needs mdft_energy_and_gradient, molecular symmetry order 2 "
                                     "and an FP64 or FP32 context");
+    // the FP32 gradient, the scaled variables and the fused products exist only in the
+    // transforms specialised for mmax <= 5; the generic kernels of larger orders have none
+    if ((df32 || scaled || dot_k > 0) && c->L > 5)
+        return fail(c, MDFT_EINVAL, "MDFT_DF_FP32, MDFT_SCALED_VARS and gradient products need mmax <= 5");
     if (scaled && (rc = ensure_vscale(c))) return rc;
     if (dot_k > 0) {
         if (dot_k > 3 || !dot_vecs || !dots || !dev || !df || mode != 0 || c->d.mrso != 2 ||

@@ -293,11 +293,11 @@ contains
     ! f = F_id + F_ext + F_exc of the current xi; df (optional) = (1/dv) dF/dxi.
     subroutine energy_and_gradient(f, df)
         real(dp), intent(out) :: f
-        real(dp), intent(out), optional, target, contiguous :: df(:,:,:,:,:)
+        real(dp), intent(out), optional, contiguous :: df(:,:,:,:,:)
         real(dp) :: parts(3)
         if (present(df)) then
             call check(mdft_energy_and_gradient(ctx, xi_dev, parts, df_dev, MDFT_DEVICE_PTRS), "energy_and_gradient")
-            call check(mdft_memcpy_d2h(ctx, c_loc(df), df_dev, 8_c_size_t * nvar), "download df")
+            call from_device(df_dev, df)
         else
             call check(mdft_energy_and_gradient(ctx, xi_dev, parts, c_null_ptr, MDFT_DEVICE_PTRS), "energy_and_gradient")
         end if
@@ -307,6 +307,22 @@ contains
         f = parts(1) + parts(2) + parts(3)
         if (apply_energy_corrections_due_to_charged_solute) f = f + pbc_correction + pscheme_correction
         f_tot = f
+    end subroutine
+
+    ! Copies between a caller's array of nvar values and the device.  The caller's dummy
+    ! arguments keep the reference's attributes (no TARGET: excess_energy_gpu must match
+    ! the abstract interface excess_energy_t exactly); the explicit-size TARGET dummy here
+    ! takes the address of the contiguous actual array.
+    subroutine to_device(a, dst)
+        real(dp), intent(in), target :: a(*)
+        type(c_ptr), intent(in) :: dst
+        call check(mdft_memcpy_h2d(ctx, dst, c_loc(a), 8_c_size_t * nvar), "upload")
+    end subroutine
+
+    subroutine from_device(src, a)
+        type(c_ptr), intent(in) :: src
+        real(dp), intent(inout), target :: a(*)
+        call check(mdft_memcpy_d2h(ctx, c_loc(a), src, 8_c_size_t * nvar), "download")
     end subroutine
 
     ! The type B and type C corrections of a solute (atoms(1:6, natom): x, y, z, q, sigma, eps)
@@ -322,15 +338,17 @@ contains
         pscheme_correction = out(2)
     end subroutine
 
-    ! excess_energy_t-conformant plug-in (module_proc_ptrs.f90:3-21): df accumulated.
+    ! excess_energy_t-conformant plug-in (module_proc_ptrs.f90:3-21): df accumulated; it
+    ! can be bound to a procedure pointer of that abstract interface, as init_vexc binds
+    ! energy_cproj_mrso (src/module_init_simu.f90:60-73).
     subroutine excess_energy_gpu(ff, df, print_timers)
         real(dp), intent(out) :: ff
-        real(dp), contiguous, intent(inout), optional, target :: df(:,:,:,:,:)
+        real(dp), contiguous, intent(inout), optional :: df(:,:,:,:,:)
         logical, intent(in), optional :: print_timers
         if (present(df)) then
-            call check(mdft_memcpy_h2d(ctx, df_dev, c_loc(df), 8_c_size_t * nvar), "upload df")
+            call to_device(df, df_dev)
             call check(mdft_excess_energy(ctx, xi_dev, ff, df_dev, MDFT_DEVICE_PTRS), "excess_energy")
-            call check(mdft_memcpy_d2h(ctx, c_loc(df), df_dev, 8_c_size_t * nvar), "download df")
+            call from_device(df_dev, df)
         else
             call check(mdft_excess_energy(ctx, xi_dev, ff, c_null_ptr, MDFT_DEVICE_PTRS), "excess_energy")
         end if

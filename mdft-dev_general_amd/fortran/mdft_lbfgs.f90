@@ -15,7 +15,9 @@
 ! recursion on vectors.  The line search enforces the strong Wolfe conditions with cubic
 ! interpolation inside a bracket (the role of dcsrch,
 ! src/module_lbfgs_nocedal_mdft.f90:1653).  Stopping rule: |dF|/|dF_0| < gtol,
-! or a relative decrease of F below ftol, or maxeval evaluations.
+! or a decrease of F below ftol in one iteration (relative, or absolute as the
+! reference's (fold - f) <= tol, src/module_lbfgs_nocedal_mdft.f90:425-427), or
+! maxeval evaluations.
 module mdft_lbfgs
     use iso_c_binding
     use mdft_gpu, only: mdft_energy_and_gradient, mdft_energy_and_gradient_dots, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
@@ -64,6 +66,9 @@ contains
     ! every term's curvature in xi carries the quadrature weight w(io), which spans a factor
     ! of about 4 over the theta nodes at mmax = 5; in u it does not.  xi is converted in
     ! place on entry and back on return.
+    ! opts bit 2 (4): ftol is absolute, the reference's stop (fold - f) <= tol
+    ! (src/module_lbfgs_nocedal_mdft.f90:425-427, tol = factr * epsmch = 1e-3 kJ/mol by
+    ! default, :10,188); without it the stop is |f_prev - f| <= ftol max(|f_prev|, |f|, 1).
     ! step0 > 0: the first trial step along -g of an iteration without curvature pairs (the
     ! first, or after a restart), in place of the unit-length move t = 1/|g|; a caller that
     ! knows the functional's curvature c along the variables passes 1/c, the Newton step.
@@ -91,7 +96,7 @@ contains
         integer :: i, j, k, mm, nb, nl, fd, fg, gcur, neval, iter, status
         integer(c_size_t) :: bytes
         integer(c_int) :: eflags, r2
-        logical :: ok, f32, sc, scaled_in
+        logical :: ok, f32, sc, scaled_in, fabs_stop
 
         rc = 0
         stats = 0
@@ -99,6 +104,7 @@ contains
         gcur = mm + 2
         f32 = iand(opts, 1_c_int) /= 0
         sc = iand(opts, 2_c_int) /= 0
+        fabs_stop = iand(opts, 4_c_int) /= 0
         scaled_in = .false.
         bytes = merge(4_c_size_t, 8_c_size_t, f32) * n
         eflags = MDFT_DEVICE_PTRS
@@ -257,7 +263,12 @@ contains
                 status = 4
                 exit
             end if
-            if (abs(f_prev - f) <= ftol * max(abs(f_prev), abs(f), 1.0_dp)) then
+            if (fabs_stop) then
+                if (f_prev - f <= ftol) then
+                    status = 1
+                    exit
+                end if
+            else if (abs(f_prev - f) <= ftol * max(abs(f_prev), abs(f), 1.0_dp)) then
                 status = 1
                 exit
             end if

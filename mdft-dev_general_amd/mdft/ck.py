@@ -333,3 +333,24 @@ def relayout(t: CkTable, mmax: int, mrso: int, nq: int) -> np.ndarray:
     if t.nq_file < nq:
         raise ValueError(f"c-table has {t.nq_file} rows, {nq} needed")
     return np.conj(t.c[:nq, cols]) * np.array(signs, dtype=np.float64)[None, :]
+
+
+def structure_factor_s0(t: CkTable) -> float:
+    """S(0) = 1 / (1 - c_{00}^{00,0}(q = 0)) of a table in the reference's file
+    convention, where the (m, n, mu, nu, khi) = (0, 0, 0, 0, 0) column is n0 times the
+    k = 0 transform of the orientation-averaged direct correlation function (the file
+    carries n0 c; the excess prefactor divides by n0, src/module_energy_cproj_mrso.f90:728)
+    and row 0 is q = 0.  1 / S(0) is the factor by which the excess term raises the
+    curvature of long-wavelength density changes over the ideal term's (SPC/E nmax1:
+    c = -13.67, S(0) = 0.068; water's is about 0.06)."""
+    idx = [i for i in range(t.np_file)
+           if (int(t.m[i]), int(t.n[i]), int(t.mu[i]), int(t.nu[i]), int(t.khi[i])) == (0, 0, 0, 0, 0)]
+    if not idx:
+        raise ValueError("the table has no (0, 0, 0, 0, 0) column")
+    q0 = float(t.q[0]) if t.q is not None else 0.0
+    if q0 != 0.0:
+        raise ValueError("row 0 of the table is not q = 0")
+    c00 = float(np.real(t.c[0, idx[0]]))
+    if not c00 < 1.0:
+        raise ValueError(f"c_000(0) = {c00} >= 1: the homogeneous fluid is unstable (S(0) < 0)")
+    return 1.0 / (1.0 - c00)

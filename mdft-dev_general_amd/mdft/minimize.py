@@ -57,7 +57,8 @@ class MinResult:
 def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: int = 5, gtol: float = 1e-6,
              ftol: float = 0.0, fnoise: float = 1e-7, maxeval: int = 2000, ntrace: int = 2000,
              ftarget: float = 0.0, fband: float = 0.0, fp32_vectors: bool = False, scaled: bool = False,
-             step0: float | str = 0.0, compressibility_s0: float = 1.0 / 16.0, lib=None) -> MinResult:
+             step0: float | str = 0.0, compressibility_s0: float | None = None, ftol_absolute: bool = False,
+             lib=None) -> MinResult:
     """Minimise F over xi (in place).  Returns the final F and statistics.
     fnoise is the relative noise level of F used to relax the sufficient-decrease
     test; with fband > 0 the run stops at the first iterate with |F - ftarget| <= fband.
@@ -67,13 +68,27 @@ def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: in
     quadrature weight out of the curvature (fewer iterations at the same minimum);
     xi and g are returned in the caller's variables.
     step0 is the first trial step along -g (0: a unit-length move, the reference's
-    choice); "newton" takes S0 / (4 rho0 kT mean(w)): 1 / (4 rho0 kT mean(w)) is the
-    inverse curvature of the ideal term at the bulk in the scaled variables (in the
-    gradient's units, (1/dv) dF/du), and S0 = 1 / (1 - rho0 c(0)) the structure factor
-    at k = 0 (compressibility_s0, 1/16 by default: water's S(0) is 0.06), which the
-    excess term adds to the curvature of long-wavelength density changes."""
+    choice); "newton" (scaled variables only) takes S0 / (4 rho0 kT mean(w)):
+    1 / (4 rho0 kT mean(w)) is the inverse curvature of the ideal term at the bulk in the
+    scaled variables (in the gradient's units, (1/dv) dF/du; over xi itself the
+    curvature is 4 rho0 kT w(io), one value per orientation, which a scalar step cannot
+    take), and S0 = 1 / (1 - c_000(q = 0)) the structure factor at k = 0 of the
+    context's own c(k) table (mdft.ck.structure_factor_s0; compressibility_s0 overrides
+    it), by which the excess term divides the curvature of long-wavelength changes.
+    ftol_absolute: stop when one iteration lowers F by at most ftol (kJ/mol), the
+    reference's rule (fold - f) <= tol with tol = 1e-3 by default
+    (src/module_lbfgs_nocedal_mdft.f90:10,188,425-427); otherwise ftol is relative."""
     lib = lib or host_library()
     if step0 == "newton":
+        if not scaled:
+            raise ValueError("step0='newton' needs scaled=True: over xi the ideal curvature 4 rho0 kT w(io) "
+                             "differs per orientation")
+        if compressibility_s0 is None:
+            table = getattr(eng, "ck_table", None)
+            if table is None:
+                raise ValueError("step0='newton' needs the context's c(k) table (Engine.set_ck) or compressibility_s0")
+            from .ck import structure_factor_s0
+            compressibility_s0 = structure_factor_s0(table)
         rho0 = eng.n0 * eng.mrso / (8.0 * np.pi ** 2)
         wmean = 8.0 * np.pi ** 2 / eng.mrso / eng.no
         step0 = compressibility_s0 / (4.0 * rho0 * eng.kT * wmean)
@@ -86,7 +101,8 @@ def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: in
                               float(fnoise), float(ftarget), float(fband), int(maxeval),
                               stats.ctypes.data_as(C.POINTER(C.c_double)),
                               trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace),
-                              int(bool(fp32_vectors)) | (2 if scaled else 0), float(step0))
+                              int(bool(fp32_vectors)) | (2 if scaled else 0) | (4 if ftol_absolute else 0),
+                              float(step0))
     if rc != 0:
         raise MdftError(rc, eng.lib.mdft_last_error(eng._h).decode())
     it = int(stats[6])

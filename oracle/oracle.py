@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# MDFT_ORACLE_LIB selects another build of the same restatement (the address-sanitised one)
+LIB = os.environ.get("MDFT_ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 REFLIB = os.path.join(HERE, "_ref", "libmdft_refpieces.so")
 
 _dp = C.POINTER(C.c_double)

@@ -17,10 +17,13 @@
 !   data/dcf/spce/spce-ck_nonzero_nmax<mmax>_ml   the c(k) file read_c_luc opens
 !   output/                                  (read_c_luc writes a memory report there)
 !   xi.bin, vext.bin                         raw FP64 in the reference layout (no, nx, ny, nz)
-! Arguments: [nrep] [write_df]
+! Arguments: [nrep] [write_df] [inputs]
 !   nrep      evaluations to run (the first one plans the FFTs and reads c(k));
 !             the wall time of the later ones is reported per evaluation
 !   write_df  1 (default): write df.bin (raw FP64, same layout); 0: F only
+!   inputs    "files" (default): read xi.bin and vext.bin; "synthetic": fill
+!             xi = 1 + 0.1 sin(.) and vext = 1 + cos(.) of the indices instead
+!             (timing runs at full size without writing the fields to disk)
 ! Writes ref_out.txt: kT n0 rho0 dv, then F_id F_ext F_exc, then seconds per
 ! evaluation (first, later mean), then the OpenMP thread count.
 program ref_eval
@@ -36,7 +39,8 @@ program ref_eval
     real(dp) :: fid, fext, fexc, t_first, t_rest
     integer :: no, nx, ny, nz, nrep, write_df, irep, u
     integer(8) :: c0, c1, rate
-    character(len=32) :: arg
+    character(len=32) :: arg, inputs
+    integer :: io, ix, iy, iz
 
     nrep = 1
     write_df = 1
@@ -48,6 +52,8 @@ program ref_eval
         call get_command_argument(2, arg)
         read (arg, *) write_df
     end if
+    inputs = "files"
+    if (command_argument_count() >= 3) call get_command_argument(3, inputs)
     nrep = max(nrep, 1)
 
     call grid%init
@@ -60,12 +66,29 @@ program ref_eval
 
     allocate (solvent(1)%xi(no, nx, ny, nz))
     allocate (solvent(1)%vext(no, nx, ny, nz))
-    open (newunit=u, file="xi.bin", access="stream", form="unformatted", status="old", action="read")
-    read (u) solvent(1)%xi
-    close (u)
-    open (newunit=u, file="vext.bin", access="stream", form="unformatted", status="old", action="read")
-    read (u) solvent(1)%vext
-    close (u)
+    if (trim(inputs) == "synthetic") then
+        !$omp parallel do private(ix, iy, io)
+        do iz = 1, nz
+            do iy = 1, ny
+                do ix = 1, nx
+                    do io = 1, no
+                        solvent(1)%xi(io, ix, iy, iz) = 1._dp + 0.1_dp * sin(0.37_dp * io + 0.11_dp * ix &
+                                                                             + 0.07_dp * iy + 0.05_dp * iz)
+                        solvent(1)%vext(io, ix, iy, iz) = 1._dp + cos(0.29_dp * io + 0.13_dp * ix &
+                                                                      + 0.03_dp * iy + 0.09_dp * iz)
+                    end do
+                end do
+            end do
+        end do
+        !$omp end parallel do
+    else
+        open (newunit=u, file="xi.bin", access="stream", form="unformatted", status="old", action="read")
+        read (u) solvent(1)%xi
+        close (u)
+        open (newunit=u, file="vext.bin", access="stream", form="unformatted", status="old", action="read")
+        read (u) solvent(1)%vext
+        close (u)
+    end if
     allocate (df(no, nx, ny, nz, 1))
 
     t_rest = 0._dp

@@ -120,26 +120,31 @@ def write_inputs(d: str, n, length, mmax: int, temperature: float, ck_writer) ->
     ck_writer(os.path.join(d, "data", "dcf", "spce", f"spce-ck_nonzero_nmax{mmax}_ml"))
 
 
-def run(n, length, mmax: int, temperature: float, ck_writer, xi: np.ndarray, vext: np.ndarray,
+def run(n, length, mmax: int, temperature: float, ck_writer, xi: np.ndarray | None, vext: np.ndarray | None,
         nrep: int = 1, want_df: bool = True, threads: int | None = None, workdir: str | None = None,
         timeout: float = 3600.0) -> RefResult:
     """One (or nrep) reference evaluations of F and df at xi with vext.
 
-    xi, vext: FP64 arrays of N*no values in the reference layout xi(no,nx,ny,nz).
-    ck_writer(path) writes the c-file (e.g. mdft.ck.write_ck_file bound to a table)."""
+    xi, vext: FP64 arrays of N*no values in the reference layout xi(no,nx,ny,nz), or
+    both None for the executable's own index-formula fields (timing at full size
+    without writing the fields to disk).
+    ck_writer(path) writes the c-file (write_ck_file_for_flang bound to a table)."""
     if not available():
         raise FileNotFoundError(f"{EXE} is not built (make -C oracle fftw ref)")
     own = workdir is None
     d = tempfile.mkdtemp(prefix="mdft_ref_") if own else workdir
     try:
         write_inputs(d, n, length, mmax, temperature, ck_writer)
-        np.ascontiguousarray(xi, dtype=np.float64).tofile(os.path.join(d, "xi.bin"))
-        np.ascontiguousarray(vext, dtype=np.float64).tofile(os.path.join(d, "vext.bin"))
+        synthetic = xi is None and vext is None
+        if not synthetic:
+            np.ascontiguousarray(xi, dtype=np.float64).tofile(os.path.join(d, "xi.bin"))
+            np.ascontiguousarray(vext, dtype=np.float64).tofile(os.path.join(d, "vext.bin"))
         env = dict(os.environ)
         if threads is not None:
             env["OMP_NUM_THREADS"] = str(int(threads))
         env.setdefault("OMP_STACKSIZE", "64M")
-        subprocess.run([EXE, str(int(nrep)), "1" if want_df else "0"], cwd=d, env=env, check=True,
+        subprocess.run([EXE, str(int(nrep)), "1" if want_df else "0", "synthetic" if synthetic else "files"],
+                       cwd=d, env=env, check=True,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=timeout)
         with open(os.path.join(d, "ref_out.txt")) as fh:
             rows = [ln.split() for ln in fh if ln.strip()]

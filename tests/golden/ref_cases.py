@@ -1,6 +1,7 @@
 """Cases of the reference golden vectors (tests/golden/ref_energy_golden.npz).
 
-Each case fixes a grid, an angular order, a c(k) table and seeded inputs
+Each case fixes a grid (sizes with factors 2, 3 and 5 only, the FFT radices of
+the device kernels), an angular order, a c(k) table and seeded inputs
 (xi, vext).  `make_ref_golden.py` ran the reference's own evaluation on them
 (energy_ideal_and_external + energy_cproj_mrso, compiled from the reference's
 sources, oracle/ref_eval.f90) and stored F_id, F_ext, F_exc and seeded samples
@@ -27,13 +28,13 @@ NSAMPLE = 4096
 
 # name, grid, box (A), mmax, table kind, table seed, rows (synthetic), input seed, size class
 CASES = [
-    ("odd_mmax1_synth", (9, 8, 7), (9.0, 8.5, 7.5), 1, "synthetic", 101, 256, 11, "small"),
+    ("odd_mmax1_synth", (9, 15, 5), (9.0, 13.5, 5.5), 1, "synthetic", 101, 256, 11, "small"),
     ("even_mmax1_spce", (8, 10, 6), (8.0, 9.0, 7.0), 1, "spce_nmax1", 0, 0, 12, "small"),
     ("even_mmax2_spce", (10, 8, 12), (9.0, 8.0, 11.0), 2, "spce_nmax2", 0, 0, 13, "small"),
-    ("odd_mmax2_synth", (7, 9, 11), (7.0, 9.5, 10.0), 2, "synthetic", 102, 256, 14, "small"),
-    ("odd_mmax3_synth", (9, 11, 7), (9.0, 10.0, 7.5), 3, "synthetic", 103, 256, 15, "small"),
+    ("odd_mmax2_synth", (5, 9, 15), (5.5, 9.5, 14.0), 2, "synthetic", 102, 256, 14, "small"),
+    ("odd_mmax3_synth", (9, 15, 5), (9.0, 14.0, 5.0), 3, "synthetic", 103, 256, 15, "small"),
     ("even_mmax3_synth", (12, 10, 8), (11.0, 9.0, 8.0), 3, "synthetic", 104, 256, 16, "small"),
-    ("odd_mmax5_synth", (7, 9, 11), (7.0, 8.0, 10.0), 5, "synthetic", 105, 256, 17, "small"),
+    ("odd_mmax5_synth", (5, 9, 15), (5.0, 8.0, 13.0), 5, "synthetic", 105, 256, 17, "small"),
     ("even_mmax5_synth", (10, 8, 12), (9.0, 8.0, 11.0), 5, "synthetic", 106, 256, 18, "small"),
     # the workloads' own shapes: LJ sphere 48^3 / 24 A / mmax 1 (512 rows), Na+ 96^3 / 32 A / mmax 3 (1024 rows)
     ("lj_sphere_48_mmax1", (48, 48, 48), (24.0, 24.0, 24.0), 1, "synthetic", 1000, 512, 7, "large"),

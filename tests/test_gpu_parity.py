@@ -153,3 +153,28 @@ def test_full_size_properties():
     pc, _ = eng.energy_and_gradient(bx, bd)
     assert np.array_equal(pa, pc) and np.array_equal(bd.download(), dfa)
     del df
+
+
+def test_mmax6_refuses_fast_path_flags():
+    """An FP64 context of mmax 6 runs the generic transforms, which have no FP32
+    gradient, scaled variables or fused products: those flags are refused."""
+    import ctypes as C
+    from mdft.gpu import DEVICE_PTRS, DF_FP32, SCALED_VARS, DeviceBuffer, Engine
+    say("[mmax 6] fast-path flags refused")
+    n, L = (8, 8, 8), (8.0, 8.0, 8.0)
+    eng = Engine(n, L, 6, 2, 2.4777, 0.0333, "fp64")
+    from mdft.ck import CkTable
+    tup = ck.tuples(6)
+    tab = CkTable(*(np.array([u[i] for u in tup], np.int32) for i in range(5)), 0.05,
+                  np.zeros((400, len(tup)), np.complex128))
+    eng.set_ck(tab)
+    xi = DeviceBuffer(eng, eng.N * eng.no * 8)
+    xi.upload(np.ones(eng.N * eng.no))
+    df = DeviceBuffer(eng, eng.N * eng.no * 8)
+    parts = np.zeros(3)
+    pp = parts.ctypes.data_as(C.POINTER(C.c_double))
+    for fl in (DF_FP32, SCALED_VARS):
+        rc = eng.lib.mdft_energy_and_gradient(eng._h, xi.ptr, pp, df.ptr, DEVICE_PTRS | fl)
+        assert rc == -1, fl
+        assert b"mmax <= 5" in eng.lib.mdft_last_error(eng._h)
+    assert eng.lib.mdft_energy_and_gradient(eng._h, xi.ptr, pp, df.ptr, DEVICE_PTRS) == 0
