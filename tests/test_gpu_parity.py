@@ -168,6 +168,7 @@ def test_mmax6_refuses_fast_path_flags():
     tab = CkTable(*(np.array([u[i] for u in tup], np.int32) for i in range(5)), 0.05,
                   np.zeros((400, len(tup)), np.complex128))
     eng.set_ck(tab)
+    eng.set_vext(np.zeros(eng.N * eng.no))
     xi = DeviceBuffer(eng, eng.N * eng.no * 8)
     xi.upload(np.ones(eng.N * eng.no))
     df = DeviceBuffer(eng, eng.N * eng.no * 8)
