@@ -194,6 +194,17 @@ int mdft_vec_axpy_dot(mdft_ctx *ctx, size_t n, double a, const double *x, double
 int mdft_vec_lincomb(mdft_ctx *ctx, size_t n, double a, const double *x, double b, const double *y, double *z);
 /* One pass over x and k <= 16 device vectors: out[j] = x.vecs[j] (out on the host). */
 int mdft_vec_multi_dot(mdft_ctx *ctx, size_t n, int k, const double *const *vecs, const double *x, double *out);
+/* mdft_vec_multi_dot without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  out[j] = x.vecs[j] for
+ * 1 <= k <= 16 device vectors (out on the host), the same kernels and results as
+ * mdft_vec_multi_dot; the per-block partial products live in a fixed 512 KiB device
+ * array of the library (one per GPU, set up by mdft_ctx_create, held until the process
+ * ends) instead of memory mdft_vec_multi_dot allocates on its first call.  k outside
+ * 1..16 is rejected with MDFT_EINVAL.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (several contexts, streams or threads on one device) race. */
+int mdft_vec_multi_dot_noalloc(mdft_ctx *ctx, size_t n, int k, const double *const *vecs, const double *x,
+                               double *out);
 /* One pass: z = sum_j coef[j] vecs[j], k <= 16 (z may be one of vecs). */
 int mdft_vec_combine(mdft_ctx *ctx, size_t n, int k, const double *coef, const double *const *vecs, double *z);
 /* One pass: z = sum_j coef[j] vecs[j] and y = x + a z, k <= 16 -- the new search

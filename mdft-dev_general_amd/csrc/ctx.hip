@@ -410,6 +410,8 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bad(MDFT_EHIP, "hipStreamCreate failed");
+    if (vec_scratch_init(d.device) != hipSuccess)
+        return bad(MDFT_EHIP, "device address of the no-allocation scratch array (hipGetSymbolAddress) failed");
     bool ok = upload(c, &c->d_w, c->w.data(), c->w.size()) == hipSuccess &&
               upload(c, &c->d_a2p, a2p.data(), a2p.size()) == hipSuccess &&
               upload(c, &c->d_p2a, p2a.data(), p2a.size()) == hipSuccess &&
@@ -893,6 +895,20 @@ int mdft_vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs, 
     for (int j = 0; j < k; j++)
         if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "null or unaligned vector");
     HIPCHK(c, vec_multi_dot(c, n, k, vecs, x, out));
+    return MDFT_OK;
+}
+
+int mdft_vec_multi_dot_noalloc(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x,
+                               double *out) {
+    if (!c || !vecs || !x || !out) return MDFT_EINVAL;
+    if (k < 1 || k > MDFT_VEC_MAX)
+        return fail(c, MDFT_EINVAL, "mdft_vec_multi_dot_noalloc: k = " + std::to_string(k) +
+                                        " vectors; its fixed scratch array holds the partial products of 1.." +
+                                        std::to_string(MDFT_VEC_MAX) + " vectors");
+    if (!aligned16(x)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    for (int j = 0; j < k; j++)
+        if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "null or unaligned vector");
+    HIPCHK(c, vec_multi_dot_noalloc(c, n, k, vecs, x, out));
     return MDFT_OK;
 }
 

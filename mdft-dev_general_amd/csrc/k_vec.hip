@@ -5,6 +5,8 @@
 //
 // All kernels stream with 16-byte loads; dot products reduce per block in
 // FP64 and then in one final block, so results are deterministic.
+#include <mutex>
+
 #include "mdft_internal.h"
 
 namespace {
@@ -488,6 +490,63 @@ hipError_t vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, co
     hipLaunchKernelGGL(combine_step, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+// ---- multi-dot without device allocation -------------------------------------------
+// The per-block partial products [kMaxBlocks][MDFT_VEC_MAX] and the k results of
+// vec_multi_dot_noalloc live in this fixed array (512 KiB) instead of memory the
+// library allocates on first use.  Each device holds its own copy until the process
+// ends; host code uses only the device address hipGetSymbolAddress returns for it.
+constexpr size_t kScratchDoubles = (size_t)kMaxBlocks * MDFT_VEC_MAX + MDFT_VEC_MAX;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+hipError_t vec_scratch_init(int device) {
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current first
+    double *p = nullptr;
+    e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+size_t vec_scratch_bytes() { return sizeof(double) * kScratchDoubles; }
+
+hipError_t vec_multi_dot_noalloc(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x,
+                                 double *out) {
+    double *s = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c->d.device >= 0 && c->d.device < kScratchDevices) s = g_scratch_ptr[c->d.device];
+    }
+    if (!s) return hipErrorNotInitialized;
+    VecSet v{};
+    for (int j = 0; j < k; j++) {
+        v.p[j] = vecs[j];
+        if (vecs[j] == x) v.self |= 1 << j;
+    }
+    v.k = k;
+    const unsigned nb = blocks_for(n);  // <= kMaxBlocks: every partial fits the array
+    hipLaunchKernelGGL(multi_dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    double *dout = s + (size_t)kMaxBlocks * MDFT_VEC_MAX;
+    hipLaunchKernelGGL(multi_dot_final, dim3(1), dim3(kThreads), 0, c->stream, s, (int)nb, k, dout);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(out, dout, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
 }
 

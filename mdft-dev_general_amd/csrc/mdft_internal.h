@@ -155,6 +155,10 @@ hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const d
 hipError_t vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, const double *const *vecs, double *z,
                             double a, const double *x, double *y);
 hipError_t vec_multi_dot32(mdft_ctx *c, size_t n, int k, const float *const *vecs, const float *x, double *out);
+hipError_t vec_scratch_init(int device);
+size_t vec_scratch_bytes();
+hipError_t vec_multi_dot_noalloc(mdft_ctx *c, size_t n, int k, const double *const *vecs, const double *x,
+                                 double *out);
 hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs, float *z,
                               double a, const double *x, double *y);
 hipError_t vec_axpy32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y);

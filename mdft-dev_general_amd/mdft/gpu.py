@@ -34,6 +34,7 @@ MDFT_EXPORTS = (
     "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_vec_combine_step",
     "mdft_charged_solute_corrections", "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32",
     "mdft_vec_widen_f32", "mdft_scale_variables", "mdft_energy_and_gradient_dots",
+    "mdft_vec_multi_dot_noalloc",
 )
 
 
@@ -91,6 +92,7 @@ def load_library(path: str = LIBPATH):
     lib.mdft_vec_axpy_dot.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp, _dp]
     lib.mdft_vec_lincomb.argtypes = [vp, C.c_size_t, C.c_double, vp, C.c_double, vp, vp]
     lib.mdft_vec_multi_dot.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(vp), vp, _dp]
+    lib.mdft_vec_multi_dot_noalloc.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(vp), vp, _dp]
     lib.mdft_pressure_correction.argtypes = [vp, vp, C.c_int, _dp]
     lib.mdft_lj_tail_correction.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double, _dp]
     lib.mdft_charged_solute_corrections.argtypes = [_dp, C.c_double, C.c_int, _dp, C.c_int, _dp, C.c_double, _dp]
@@ -113,7 +115,7 @@ def load_library(path: str = LIBPATH):
                  "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj", "mdft_proj2angl",
                  "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_charged_solute_corrections",
                  "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32", "mdft_vec_widen_f32",
-                 "mdft_scale_variables", "mdft_energy_and_gradient_dots"):
+                 "mdft_scale_variables", "mdft_energy_and_gradient_dots", "mdft_vec_multi_dot_noalloc"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib
