@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--memory", type=int, default=1, help="L-BFGS memory m (both paths; 1 is the reference's)")
     ap.add_argument("--fp64-vectors", action="store_true",
                     help="fast path: keep the minimiser's directions and gradients in FP64 (default FP32)")
+    ap.add_argument("--fp64-iterate", action="store_true",
+                    help="fast path: keep the minimiser's iterate in FP64 (default FP32, with FP32 vectors)")
     ap.add_argument("--unscaled", action="store_true",
                     help="fast path: minimise over xi itself (default: over u = sqrt(w / mean w) xi)")
     ap.add_argument("--unit-first-step", action="store_true",
@@ -313,6 +315,7 @@ def main():
             kw = dict(ftol=ref_tol, ftol_absolute=True) if stop == "ref" else dict(ftarget=dg_ref, fband=band)
             r = minimize(e32, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6,
                          fp32_vectors=not args.fp64_vectors, scaled=not args.unscaled,
+                         fp32_iterate=not (args.fp64_vectors or args.fp64_iterate),
                          step0=0.0 if (args.unit_first_step or args.unscaled) else "newton", **kw)
             c, _ = eng.energy_and_gradient(xi, None)
             eng.synchronize()
@@ -356,14 +359,18 @@ def main():
         # than the gradient itself are read there) -- all at the HBM peak
         nvec = eng.N * eng.no
         vb = 4 if not args.fp64_vectors else 8
+        xb = 4 if not (args.fp64_vectors or args.fp64_iterate) else 8   # iterate and trial points
         dot_bytes = 2 * args.memory * vb if (vb == 4 and args.memory == 1) else (2 * args.memory + 1) * vb
-        per_elem = (2 * args.memory + 1) * vb + vb + 16 + dot_bytes
-        t_roof = (res_f.evaluations * algorithmic_bytes(eng.N, eng.no, eng.np_, 8)
-                  + res_f.iterations * per_elem * nvec) / (HBM_PEAK_GBS * 1e9)
+        per_elem = (2 * args.memory + 1) * vb + vb + 2 * xb + dot_bytes
+        # one FP32 evaluation: xi read twice (xb bytes), vext once (8), dF written (vb), projections
+        eval_bytes = eng.N * eng.no * (2 * xb + 8 + vb) + eng.N * 8 * eng.np_ * 8
+        t_roof = (res_f.evaluations * eval_bytes + res_f.iterations * per_elem * nvec) / (HBM_PEAK_GBS * 1e9)
         line["seconds_to_band_roofline"] = {"t_roofline_s": t_roof, "t_fast_s": t_fast, "frac": t_roof / t_fast,
                                             "vector_bytes_per_element_per_iteration": per_elem,
-                                            "minimiser_vectors": "fp64" if args.fp64_vectors else "fp32"}
-        vname = "FP64" if args.fp64_vectors else "FP32"
+                                            "minimiser_vectors": "fp64" if args.fp64_vectors else "fp32",
+                                            "minimiser_iterate": "fp32" if xb == 4 else "fp64",
+                                            "evaluation_algorithmic_GB": eval_bytes / 1e9}
+        vname = "FP64" if args.fp64_vectors else ("FP32" if args.fp64_iterate else "FP32 (iterate too)")
         sname = "xi" if args.unscaled else "u = sqrt(w / mean w) xi"
         line.update({
             "value": t_fast, "unit": "s", "steps": res_f.evaluations,

@@ -52,6 +52,9 @@ enum {
     MDFT_DF_FP32 = 4,     /* mdft_energy_and_gradient only, with MDFT_DEVICE_PTRS and molecular
                              symmetry order 2, FP64 or FP32 contexts: df is a float array that
                              receives dF rounded to FP32 (the minimiser's FP32 vector storage) */
+    MDFT_XI_FP32 = 16,    /* mdft_energy_and_gradient only, with MDFT_DEVICE_PTRS, molecular symmetry order 2,
+                             mmax <= 5, FP64 or FP32 contexts: xi is a float array (the minimiser's iterate
+                             stored in FP32; no reference counterpart) */
     MDFT_SCALED_VARS = 8  /* mdft_energy_and_gradient only, molecular symmetry order 2, FP64 or FP32
                              contexts: xi holds u(io, r) = s(io) xi(io, r) with s = sqrt(w / mean w)
                              (w the quadrature weights) and df receives dF/du = dF/dxi / s -- the
@@ -227,6 +230,14 @@ int mdft_vec_combine_step_f32(mdft_ctx *ctx, size_t n, int k, const double *coef
 int mdft_vec_axpy_f32(mdft_ctx *ctx, size_t n, double a, const float *z, const double *x, double *y);
 /* out = (double) in. */
 int mdft_vec_widen_f32(mdft_ctx *ctx, size_t n, const float *in, double *out);
+/* The minimiser's passes with the iterate itself stored in FP32 as well (MDFT_XI_FP32
+ * evaluations): z = (float) sum_j coef[j] vecs[j] and y = (float)(x + a z), x and y FP32,
+ * y not aliasing x; and y = (float)(x + a z), y may alias x.  No reference counterpart. */
+int mdft_vec_combine_step_f32x(mdft_ctx *ctx, size_t n, int k, const double *coef, const float *const *vecs,
+                               float *z, double a, const float *x, float *y);
+int mdft_vec_axpy_f32x(mdft_ctx *ctx, size_t n, double a, const float *z, const float *x, float *y);
+/* out = (float) in. */
+int mdft_vec_narrow_f64(mdft_ctx *ctx, size_t n, const double *in, float *out);
 /* The change of variables of MDFT_SCALED_VARS over n = (points) x no values: dir = 1,
  * out = s in (xi -> u, or dF/du -> dF/dxi); dir = -1, out = in / s (u -> xi).  in is
  * FP32 when in_fp32 (a gradient stored in FP32), else FP64; out may alias an FP64 in.

@@ -17,6 +17,7 @@
 #include <tuple>
 
 #include "mdft_internal.h"
+#include "oz_tiles.h"
 #include "mdft_wsnet.h"
 
 static std::string g_create_error;
@@ -437,6 +438,14 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
         const int rc = bf16_tables(c, a2p, p2a, fmv);
         if (rc != MDFT_OK) return bad(rc, "BF16 transform tables");
     }
+    if (d.precision == MDFT_PREC_FP64 && d.mrso == 2) {  // the tiled FP64 q-space kernel
+        std::vector<double> ftab;
+        std::vector<int> itab;
+        if (oz_tables(c, ftab, itab, &c->oz_ntin, &c->oz_ntout) != MDFT_OK) return bad(MDFT_EINVAL, "q-space tables");
+        if (upload(c, &c->d_oz_ftab, ftab.data(), ftab.size()) != hipSuccess ||
+            upload(c, &c->d_oz_itab, itab.data(), itab.size()) != hipSuccess)
+            return bad(MDFT_ENOMEM, "device allocation failed");
+    }
     if (d.precision != MDFT_PREC_FP64) {
         std::vector<float> ftab;
         std::vector<int> lanerow;
@@ -477,7 +486,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
     void *bufs[] = {c->d_w,   c->d_a2p,  c->d_p2a,  c->d_fm,   c->d_cph,      c->d_sph,        c->d_cps,
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
-                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
+                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz_ftab, c->d_oz_itab, c->d_oz32_ftab, c->d_oz32_lanerow,
                     c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -648,10 +657,15 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     if (scaled && (mode != 0 || c->d.mrso != 2 || c->d.precision == MDFT_PREC_BF16))
         return fail(c, MDFT_EINVAL, "MDFT_SCALED_VARS needs mdft_energy_and_gradient, molecular symmetry order 2 "
                                     "and an FP64 or FP32 context");
-    // the FP32 gradient, the scaled variables and the fused products exist only in the
-    // transforms specialised for mmax <= 5; the generic kernels of larger orders have none
-    if ((df32 || scaled || dot_k > 0) && c->L > 5)
-        return fail(c, MDFT_EINVAL, "MDFT_DF_FP32, MDFT_SCALED_VARS and gradient products need mmax <= 5");
+    const bool x32 = flags & MDFT_XI_FP32;
+    if (x32 && (!dev || mode != 0 || c->d.mrso != 2 || c->d.precision == MDFT_PREC_BF16))
+        return fail(c, MDFT_EINVAL, "MDFT_XI_FP32 needs device pointers, mdft_energy_and_gradient, molecular "
+                                    "symmetry order 2 and an FP64 or FP32 context");
+    // the FP32 gradient and iterate, the scaled variables and the fused products exist only in
+    // the transforms specialised for mmax <= 5; the generic kernels of larger orders have none
+    if ((df32 || scaled || x32 || dot_k > 0) && c->L > 5)
+        return fail(c, MDFT_EINVAL,
+                    "MDFT_DF_FP32, MDFT_XI_FP32, MDFT_SCALED_VARS and gradient products need mmax <= 5");
     if (scaled && (rc = ensure_vscale(c))) return rc;
     if (dot_k > 0) {
         if (dot_k > 3 || !dot_vecs || !dots || !dev || !df || mode != 0 || c->d.mrso != 2 ||
@@ -689,9 +703,11 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
         }
     }
     c->vsinv = scaled ? c->d_vscale + c->no : nullptr;
+    c->xi32 = x32 ? reinterpret_cast<const float *>(xi) : nullptr;
     rc = run_excess_pipeline(c, xd, 4, timers);
     if (rc) {
         c->vsinv = nullptr;
+        c->xi32 = nullptr;
         return rc;
     }
     int nb = 0;
@@ -702,6 +718,7 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
     const hipError_t fe = launch_final(c, xd, dd, mode, &nb);
     c->df32 = nullptr;
     c->vsinv = nullptr;
+    c->xi32 = nullptr;
     c->dot_k = 0;
     HIPCHK(c, fe);
     if (fused_dots) {
@@ -958,6 +975,30 @@ int mdft_vec_axpy_f32(mdft_ctx *c, size_t n, double a, const float *z, const dou
     if (!c || !z || !x || !y) return MDFT_EINVAL;
     if (!aligned16(z) || !aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
     HIPCHK(c, vec_axpy32(c, n, a, z, x, y));
+    return MDFT_OK;
+}
+
+int mdft_vec_combine_step_f32x(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs,
+                               float *z, double a, const float *x, float *y) {
+    if (!c || !coef || !vecs || !z || !x || !y || k < 1 || k > MDFT_VEC_MAX) return MDFT_EINVAL;
+    if (!aligned16(z) || !aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    for (int j = 0; j < k; j++)
+        if (!vecs[j] || !aligned16(vecs[j])) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    if (y == x) return fail(c, MDFT_EINVAL, "y must not alias x");
+    HIPCHK(c, vec_combine_step32x(c, n, k, coef, vecs, z, a, x, y));
+    return MDFT_OK;
+}
+
+int mdft_vec_axpy_f32x(mdft_ctx *c, size_t n, double a, const float *z, const float *x, float *y) {
+    if (!c || !z || !x || !y) return MDFT_EINVAL;
+    if (!aligned16(z) || !aligned16(x) || !aligned16(y)) return fail(c, MDFT_EINVAL, "vector pointers must be 16-byte aligned");
+    HIPCHK(c, vec_axpy32x(c, n, a, z, x, y));
+    return MDFT_OK;
+}
+
+int mdft_vec_narrow_f64(mdft_ctx *c, size_t n, const double *in, float *out) {
+    if (!c || !in || !out) return MDFT_EINVAL;
+    HIPCHK(c, vec_narrow(c, n, in, out));
     return MDFT_OK;
 }
 

@@ -333,7 +333,8 @@ template <int L, int PT, typename T>
 __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
                                                        size_t N, AngTables t, double k0, int feat, double beta,
                                                        const double *__restrict__ aux,
-                                                       const double *__restrict__ vsinv) {
+                                                       const double *__restrict__ vsinv,
+                                                       const float *__restrict__ xi32) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
         const int i = tid + q * kThreads;
-        xr[q] = (i < npts * NO) ? src[i] : 0.0;
+        xr[q] = (i < npts * NO) ? (xi32 ? (double)xi32[r0 * NO + i] : src[i]) : 0.0;  // FP32 iterate: xi32
         sr[q] = vsinv ? vsinv[i % NO] : 1.0;  // scaled variables: xi = u / s(io), loaded with the tile
     }
     for (int i = tid; i < H1 * NPSI; i += kThreads) {
@@ -446,7 +447,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode, float *__restrict__ df32,
                                                    const double *__restrict__ vsinv, DotArgs da,
-                                                   double *__restrict__ dpart) {
+                                                   double *__restrict__ dpart, const float *__restrict__ xi32) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         vr[q] = 0.0;
         if (i < npts * NO) {
             const size_t g = r0 * NO + i;
-            if (mode < 3) xr[q] = xi[g];
+            if (mode < 3) xr[q] = xi32 ? (double)xi32[g] : xi[g];  // FP32 iterate: xi32
             if (mode == 0 || mode == 3 || mode == 4) vr[q] = vext[g];
         }
     }
@@ -672,7 +673,7 @@ bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int fe
     const double k0 = (feat == 3) ? 0.5 * c->dv : c->rho0;
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, npts, c->ang, k0, feat, beta, aux, feat == 0 ? c->vsinv : nullptr); return true;
+    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, npts, c->ang, k0, feat, beta, aux, feat == 0 ? c->vsinv : nullptr, feat == 0 ? c->xi32 : nullptr); return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;
@@ -693,7 +694,7 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
     }
     switch (c->L) {
 #define FIN_CASE(LL) \
-    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr, da, c->d_dpart); return true;
+    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr, da, c->d_dpart, mode == 0 ? c->xi32 : nullptr); return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
         default: return false;

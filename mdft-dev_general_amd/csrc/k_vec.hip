@@ -336,6 +336,59 @@ __global__ __launch_bounds__(kThreads) void axpy32(size_t n, double a, const flo
         for (size_t t = 4 * n4; t < n; t++) y[t] = x[t] + a * (double)z[t];
 }
 
+// The same two passes with the iterate itself in FP32 (x, y float): y = x + a z
+// formed in FP64 and rounded once.
+__global__ __launch_bounds__(kThreads) void combine_step32x(size_t n, VecSet32 v, float *z, double a,
+                                                           const float *__restrict__ x, float *__restrict__ y) {
+    const size_t n4 = n / 4;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
+        const float4 xv = reinterpret_cast<const float4 *>(x)[i];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int j = 0; j < MDFT_VEC_MAX; j++)
+            if (j < v.k) {
+                const float4 w = reinterpret_cast<const float4 *>(v.p[j])[i];
+                s0 += v.c[j] * w.x;
+                s1 += v.c[j] * w.y;
+                s2 += v.c[j] * w.z;
+                s3 += v.c[j] * w.w;
+            }
+        const float4 zs = make_float4((float)s0, (float)s1, (float)s2, (float)s3);
+        reinterpret_cast<float4 *>(z)[i] = zs;
+        reinterpret_cast<float4 *>(y)[i] =
+            make_float4((float)((double)xv.x + a * zs.x), (float)((double)xv.y + a * zs.y),
+                        (float)((double)xv.z + a * zs.z), (float)((double)xv.w + a * zs.w));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (size_t t = 4 * n4; t < n; t++) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < MDFT_VEC_MAX; j++)
+                if (j < v.k) s += v.c[j] * v.p[j][t];
+            z[t] = (float)s;
+            y[t] = (float)((double)x[t] + a * (double)z[t]);
+        }
+}
+
+__global__ __launch_bounds__(kThreads) void axpy32x(size_t n, double a, const float *__restrict__ z, const float *x,
+                                                   float *y) {
+    const size_t n4 = n / 4;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
+        const float4 w = reinterpret_cast<const float4 *>(z)[i];
+        const float4 xv = reinterpret_cast<const float4 *>(x)[i];
+        reinterpret_cast<float4 *>(y)[i] =
+            make_float4((float)((double)xv.x + a * w.x), (float)((double)xv.y + a * w.y),
+                        (float)((double)xv.z + a * w.z), (float)((double)xv.w + a * w.w));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (size_t t = 4 * n4; t < n; t++) y[t] = (float)((double)x[t] + a * (double)z[t]);
+}
+
+__global__ __launch_bounds__(kThreads) void narrow1(size_t n, const double *__restrict__ in, float *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        out[i] = (float)in[i];
+}
+
 __global__ __launch_bounds__(kThreads) void widen1(size_t n, const float *__restrict__ in, double *__restrict__ out) {
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
         out[i] = (double)in[i];
@@ -588,6 +641,34 @@ hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, 
 
 hipError_t vec_axpy32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y) {
     hipLaunchKernelGGL(axpy32, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, a, z, x, y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_combine_step32x(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs, float *z,
+                               double a, const float *x, float *y) {
+    VecSet32 v{};
+    for (int j = 0; j < k; j++) {
+        v.p[j] = vecs[j];
+        v.c[j] = coef[j];
+    }
+    v.k = k;
+    hipLaunchKernelGGL(combine_step32x, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_axpy32x(mdft_ctx *c, size_t n, double a, const float *z, const float *x, float *y) {
+    hipLaunchKernelGGL(axpy32x, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, a, z, x, y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t vec_narrow(mdft_ctx *c, size_t n, const double *in, float *out) {
+    hipLaunchKernelGGL(narrow1, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);

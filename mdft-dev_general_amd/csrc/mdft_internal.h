@@ -19,8 +19,8 @@
 // stays <= e^3 (next to charged sites beta vext reaches -90, whose Boltzmann factor
 // overflows the FP32 excess term; the converged rho/rho0 rarely exceeds 10 there)
 #define MDFT_WS_BOLTZMANN_MIN_BV (-3.0)
-#define MDFT_OZ32_ROWS_PER_LANE 4  // OZ rows per lane in the FP32 q-space kernel (np <= 256)
 #define MDFT_FFT_MAX_STAGES 12
+#define MDFT_OZ32_ROWS_PER_LANE 4  // OZ rows per lane in the FP32 q-space kernel (np <= 256)
 
 // Angular tables of one context (all device pointers).
 struct AngTables {
@@ -77,6 +77,9 @@ struct mdft_ctx {
     int npc = 0, nq = 0;
     double dq = 0;
     double2 *d_ck = nullptr;      // [nq][npc]
+    double *d_oz_ftab = nullptr;  // Fourier coefficients of d^m(theta), tiled FP64 q-space kernel
+    int *d_oz_itab = nullptr;     // its integer tables (d entries, rotation tasks, OZ rows)
+    int oz_ntin = 0, oz_ntout = 0;  // its rotation task counts
     float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
     int *d_oz32_lanerow = nullptr; // descriptor table of the FP32 q-space kernel (rotation offsets, lane rows, OZ columns)
     void *d_bf16_M = nullptr;     // angl2proj matrix, BF16 MFMA fragment order
@@ -95,6 +98,7 @@ struct mdft_ctx {
     float *df32 = nullptr;        // set during an evaluation with MDFT_DF_FP32: dF goes here, rounded to FP32
     double *d_vscale = nullptr;   // [2 no]: s(io) = sqrt(w(io) / mean w) and 1 / s(io) (scaled variables)
     const double *vsinv = nullptr;  // set during an evaluation with MDFT_SCALED_VARS: 1 / s
+    const float *xi32 = nullptr;    // set during an evaluation with MDFT_XI_FP32: the FP32 iterate
     // set during mdft_energy_and_gradient_dots: products of dF with up to 3 vectors in the final pass
     int dot_k = 0, dot_self = 0;
     const void *dot_vec[3] = {nullptr, nullptr, nullptr};
@@ -163,6 +167,10 @@ hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, 
                               double a, const double *x, double *y);
 hipError_t vec_axpy32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y);
 hipError_t vec_widen(mdft_ctx *c, size_t n, const float *in, double *out);
+hipError_t vec_combine_step32x(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs, float *z,
+                               double a, const float *x, float *y);
+hipError_t vec_axpy32x(mdft_ctx *c, size_t n, double a, const float *z, const float *x, float *y);
+hipError_t vec_narrow(mdft_ctx *c, size_t n, const double *in, float *out);
 hipError_t vec_scale_orient(mdft_ctx *c, size_t n, int no, const double *fac, const void *in, bool in_f32, double *out);
 hipError_t launch_vext(mdft_ctx *c, int natom, const double *d_atoms, int nsite, const double *d_site_p,
                        const double *d_site_off, double rcut, double qfact, double clip, bool coulomb = true);

@@ -16,6 +16,7 @@ module mdft_gpu
     integer, parameter :: dp = c_double
     integer(c_int), parameter, public :: MDFT_PREC_FP64 = 0, MDFT_PREC_FP32 = 1, MDFT_PREC_BF16 = 2
     integer(c_int), parameter, public :: MDFT_DEVICE_PTRS = 1, MDFT_TIMERS = 2, MDFT_DF_FP32 = 4, MDFT_SCALED_VARS = 8
+    integer(c_int), parameter, public :: MDFT_XI_FP32 = 16
 
     type, bind(C), public :: mdft_desc
         integer(c_int) :: nx, ny, nz
@@ -193,6 +194,27 @@ module mdft_gpu
             type(c_ptr), value :: ctx, in, out
             integer(c_size_t), value :: n
         end function
+        integer(c_int) function mdft_vec_narrow_f64(ctx, n, in, out) bind(C, name="mdft_vec_narrow_f64")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: ctx, in, out
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function mdft_vec_axpy_f32x(ctx, n, a, z, x, y) bind(C, name="mdft_vec_axpy_f32x")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, z, x, y
+            integer(c_size_t), value :: n
+            real(c_double), value :: a
+        end function
+        integer(c_int) function mdft_vec_combine_step_f32x(ctx, n, k, coef, vecs, z, a, x, y) &
+                bind(C, name="mdft_vec_combine_step_f32x")
+            import :: c_int, c_ptr, c_size_t, c_double
+            type(c_ptr), value :: ctx, z, x, y
+            integer(c_size_t), value :: n
+            integer(c_int), value :: k
+            real(c_double), intent(in) :: coef(*)
+            type(c_ptr), intent(in) :: vecs(*)
+            real(c_double), value :: a
+        end function
         ! u = s xi (dir = 1) or xi = u / s (dir = -1), s(io) = sqrt(w(io) / mean w)
         integer(c_int) function mdft_scale_variables(ctx, n, dir, in_fp32, in, out) bind(C, name="mdft_scale_variables")
             import :: c_int, c_ptr, c_size_t
@@ -216,6 +238,7 @@ module mdft_gpu
     public :: mdft_memcpy_h2d, mdft_memcpy_d2h, mdft_vec_dot, mdft_vec_axpy_dot, mdft_vec_lincomb
     public :: mdft_vec_multi_dot, mdft_vec_combine, mdft_vec_combine_step
     public :: mdft_vec_multi_dot_f32, mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32
+    public :: mdft_vec_narrow_f64, mdft_vec_axpy_f32x, mdft_vec_combine_step_f32x
     public :: mdft_scale_variables
     public :: mdft_charged_solute_corrections
     public :: check, last_error_text

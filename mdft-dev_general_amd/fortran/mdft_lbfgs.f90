@@ -23,7 +23,8 @@ module mdft_lbfgs
     use mdft_gpu, only: mdft_energy_and_gradient, mdft_energy_and_gradient_dots, mdft_device_alloc, mdft_device_free, mdft_vec_lincomb, &
                         mdft_vec_multi_dot, mdft_vec_combine_step, mdft_vec_multi_dot_f32, &
                         mdft_vec_combine_step_f32, mdft_vec_axpy_f32, mdft_vec_widen_f32, &
-                        mdft_scale_variables, MDFT_DEVICE_PTRS, MDFT_DF_FP32, MDFT_SCALED_VARS
+                        mdft_scale_variables, mdft_vec_narrow_f64, mdft_vec_axpy_f32x, &
+                        mdft_vec_combine_step_f32x, MDFT_DEVICE_PTRS, MDFT_DF_FP32, MDFT_SCALED_VARS, MDFT_XI_FP32
     implicit none
     private
     integer, parameter :: dp = c_double
@@ -66,6 +67,9 @@ contains
     ! every term's curvature in xi carries the quadrature weight w(io), which spans a factor
     ! of about 4 over the theta nodes at mmax = 5; in u it does not.  xi is converted in
     ! place on entry and back on return.
+    ! opts bit 3 (8), with bit 0: the iterate itself (x and the trial points) is stored in FP32
+    ! too (MDFT_XI_FP32 evaluations): 8 fewer bytes per element in every iteration and
+    ! evaluation; xi (FP64) receives the final iterate widened.
     ! opts bit 2 (4): ftol is absolute, the reference's stop (fold - f) <= tol
     ! (src/module_lbfgs_nocedal_mdft.f90:425-427, tol = factr * epsmch = 1e-3 kJ/mol by
     ! default, :10,188); without it the stop is |f_prev - f| <= ftol max(|f_prev|, |f|, 1).
@@ -84,7 +88,7 @@ contains
         real(c_double), intent(out) :: stats(16)
         real(c_double), intent(out) :: trace(3, *)
         ! vector slots: 1..mm+1 hold directions, mm+2..2mm+3 gradients (slot mm+2 starts as the caller's g)
-        type(c_ptr) :: vec(nslot), x, xn, tmp, xn_own, pv(16)
+        type(c_ptr) :: vec(nslot), x, xn, tmp, xn_own, x32_own, pv(16)
         logical :: owned(nslot)
         real(dp) :: gram(nslot, nslot), tstep(nslot)
         integer :: ds(mmem_max + 1), gs(mmem_max + 2)   ! window: direction slots of pairs 1..k, gradient slots 1..k+1
@@ -96,7 +100,7 @@ contains
         integer :: i, j, k, mm, nb, nl, fd, fg, gcur, neval, iter, status
         integer(c_size_t) :: bytes
         integer(c_int) :: eflags, r2
-        logical :: ok, f32, sc, scaled_in, fabs_stop
+        logical :: ok, f32, sc, scaled_in, fabs_stop, x32
 
         rc = 0
         stats = 0
@@ -105,13 +109,16 @@ contains
         f32 = iand(opts, 1_c_int) /= 0
         sc = iand(opts, 2_c_int) /= 0
         fabs_stop = iand(opts, 4_c_int) /= 0
+        x32 = f32 .and. iand(opts, 8_c_int) /= 0
         scaled_in = .false.
         bytes = merge(4_c_size_t, 8_c_size_t, f32) * n
         eflags = MDFT_DEVICE_PTRS
         if (f32) eflags = eflags + MDFT_DF_FP32
         if (sc) eflags = eflags + MDFT_SCALED_VARS
+        if (x32) eflags = eflags + MDFT_XI_FP32
         owned = .false.
         xn_own = c_null_ptr
+        x32_own = c_null_ptr
         t_ready = -1.0_dp
         do i = 1, 2 * mm + 3
             vec(i) = c_null_ptr
@@ -124,13 +131,18 @@ contains
                 owned(i) = .true.
             end if
         end do
-        rc = mdft_device_alloc(ctx, 8_c_size_t * n, xn_own); if (rc /= 0) goto 900
+        rc = mdft_device_alloc(ctx, merge(4_c_size_t, 8_c_size_t, x32) * n, xn_own); if (rc /= 0) goto 900
         if (sc) then
             rc = mdft_scale_variables(ctx, n, 1_c_int, 0_c_int, xi, xi); if (rc /= 0) goto 900
             scaled_in = .true.
         end if
         x = xi
         xn = xn_own
+        if (x32) then                            ! the iterate in FP32: x32_own = (float) xi
+            rc = mdft_device_alloc(ctx, 4_c_size_t * n, x32_own); if (rc /= 0) goto 900
+            rc = mdft_vec_narrow_f64(ctx, n, xi, x32_own); if (rc /= 0) goto 900
+            x = x32_own
+        end if
         gram = 0.0_dp
         tstep = 0.0_dp
 
@@ -206,7 +218,9 @@ contains
                 end if
             end do
             ! the direction and the line search's first trial point in one pass
-            if (f32) then
+            if (x32) then
+                rc = mdft_vec_combine_step_f32x(ctx, n, nb, coef, pv, vec(fd), -t, x, xn)
+            else if (f32) then
                 rc = mdft_vec_combine_step_f32(ctx, n, nb, coef, pv, vec(fd), -t, x, xn)
             else
                 rc = mdft_vec_combine_step(ctx, n, nb, coef, pv, vec(fd), -t, x, xn)
@@ -285,7 +299,11 @@ contains
         stats(9) = g0
 900     continue
         ! the caller's xi/g buffers must hold the result
-        if (rc == 0 .and. .not. c_associated(x, xi)) rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, 0.0_dp, c_null_ptr, xi)
+        if (x32) then
+            if (rc == 0) rc = mdft_vec_widen_f32(ctx, n, x, xi)
+        else if (rc == 0 .and. .not. c_associated(x, xi)) then
+            rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, 0.0_dp, c_null_ptr, xi)
+        end if
         if (rc == 0 .and. sc .and. c_associated(vec(gcur))) then
             ! dF/dxi = s dF/du
             rc = mdft_scale_variables(ctx, n, 1_c_int, merge(1_c_int, 0_c_int, f32), vec(gcur), g)
@@ -368,7 +386,9 @@ contains
         subroutine trial()
             integer :: a, a0
             if (t /= t_ready) then
-                if (f32) then
+                if (x32) then
+                    rc = mdft_vec_axpy_f32x(ctx, n, -t, vec(fd), x, xn)
+                else if (f32) then
                     rc = mdft_vec_axpy_f32(ctx, n, -t, vec(fd), x, xn)
                 else
                     rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, -t, vec(fd), xn)
@@ -489,6 +509,7 @@ contains
                 if (owned(a)) r2 = mdft_device_free(ctx, vec(a))
             end do
             if (c_associated(xn_own)) r2 = mdft_device_free(ctx, xn_own)
+            if (c_associated(x32_own)) r2 = mdft_device_free(ctx, x32_own)
         end subroutine
 
     end function mdft_minimize_ex

@@ -23,6 +23,7 @@ DEVICE_PTRS = 1
 TIMERS = 2
 DF_FP32 = 4
 SCALED_VARS = 8
+XI_FP32 = 16
 
 MDFT_EXPORTS = (
     "mdft_ctx_create", "mdft_ctx_destroy", "mdft_last_error", "mdft_get_info", "mdft_get_quadrature",
@@ -34,7 +35,7 @@ MDFT_EXPORTS = (
     "mdft_proj2angl", "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_vec_combine_step",
     "mdft_charged_solute_corrections", "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32",
     "mdft_vec_widen_f32", "mdft_scale_variables", "mdft_energy_and_gradient_dots",
-    "mdft_vec_multi_dot_noalloc",
+    "mdft_vec_multi_dot_noalloc", "mdft_vec_combine_step_f32x", "mdft_vec_axpy_f32x", "mdft_vec_narrow_f64",
 )
 
 
@@ -100,6 +101,9 @@ def load_library(path: str = LIBPATH):
     lib.mdft_vec_combine_step_f32.argtypes = [vp, C.c_size_t, C.c_int, _dp, C.POINTER(vp), vp, C.c_double, vp, vp]
     lib.mdft_vec_axpy_f32.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp]
     lib.mdft_vec_widen_f32.argtypes = [vp, C.c_size_t, vp, vp]
+    lib.mdft_vec_narrow_f64.argtypes = [vp, C.c_size_t, vp, vp]
+    lib.mdft_vec_axpy_f32x.argtypes = [vp, C.c_size_t, C.c_double, vp, vp, vp]
+    lib.mdft_vec_combine_step_f32x.argtypes = [vp, C.c_size_t, C.c_int, _dp, C.POINTER(vp), vp, C.c_double, vp, vp]
     lib.mdft_scale_variables.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp]
     lib.mdft_energy_and_gradient_dots.argtypes = [vp, vp, _dp, vp, C.c_int, C.c_int, C.POINTER(vp), _dp]
     lib.mdft_angl2proj.argtypes = [vp, vp, C.c_size_t, vp]
@@ -115,7 +119,8 @@ def load_library(path: str = LIBPATH):
                  "mdft_pressure_correction", "mdft_lj_tail_correction", "mdft_angl2proj", "mdft_proj2angl",
                  "mdft_regrid_trilinear", "mdft_compute_vext_ex", "mdft_charged_solute_corrections",
                  "mdft_vec_multi_dot_f32", "mdft_vec_combine_step_f32", "mdft_vec_axpy_f32", "mdft_vec_widen_f32",
-                 "mdft_scale_variables", "mdft_energy_and_gradient_dots", "mdft_vec_multi_dot_noalloc"):
+                 "mdft_scale_variables", "mdft_energy_and_gradient_dots", "mdft_vec_multi_dot_noalloc",
+                 "mdft_vec_combine_step_f32x", "mdft_vec_axpy_f32x", "mdft_vec_narrow_f64"):
         getattr(lib, name).restype = C.c_int
     _LIB = lib
     return lib
@@ -216,6 +221,7 @@ class Engine:
         self._check(self.lib.mdft_set_ck(self._h, len(arr[0]), *[a.ctypes.data_as(_ip) for a in arr],
                                          c.shape[0], float(table.dq), c.view(np.float64).ctypes.data_as(_dp)))
         self._refresh()
+        self.ck_table = table
 
     ELECTROSTATICS = {"poisson": 0, "direct": 1, "none": 2}
 

@@ -58,12 +58,14 @@ def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: in
              ftol: float = 0.0, fnoise: float = 1e-7, maxeval: int = 2000, ntrace: int = 2000,
              ftarget: float = 0.0, fband: float = 0.0, fp32_vectors: bool = False, scaled: bool = False,
              step0: float | str = 0.0, compressibility_s0: float | None = None, ftol_absolute: bool = False,
-             lib=None) -> MinResult:
+             fp32_iterate: bool = False, lib=None) -> MinResult:
     """Minimise F over xi (in place).  Returns the final F and statistics.
     fnoise is the relative noise level of F used to relax the sufficient-decrease
     test; with fband > 0 the run stops at the first iterate with |F - ftarget| <= fband.
     fp32_vectors stores the minimiser's directions and gradients in FP32 (half the
     bytes per vector pass; meant for FP32 evaluations); g still receives FP64.
+    fp32_iterate (with fp32_vectors) stores the iterate and the trial points in FP32 as
+    well, read by the evaluations as such (MDFT_XI_FP32); xi receives the result in FP64.
     scaled minimises over u = s xi with s(io) = sqrt(w(io) / mean w), which takes the
     quadrature weight out of the curvature (fewer iterations at the same minimum);
     xi and g are returned in the caller's variables.
@@ -101,7 +103,8 @@ def minimize(eng: Engine, xi: DeviceBuffer, g: DeviceBuffer | None = None, m: in
                               float(fnoise), float(ftarget), float(fband), int(maxeval),
                               stats.ctypes.data_as(C.POINTER(C.c_double)),
                               trace.ctypes.data_as(C.POINTER(C.c_double)), int(ntrace),
-                              int(bool(fp32_vectors)) | (2 if scaled else 0) | (4 if ftol_absolute else 0),
+                              int(bool(fp32_vectors)) | (2 if scaled else 0) | (4 if ftol_absolute else 0)
+                              | (8 if (fp32_iterate and fp32_vectors) else 0),
                               float(step0))
     if rc != 0:
         raise MdftError(rc, eng.lib.mdft_last_error(eng._h).decode())

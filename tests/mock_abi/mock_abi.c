@@ -103,9 +103,12 @@ int mdft_energy_and_gradient(mdft_ctx *c, const double *xi, double *parts, doubl
     double *sc = (flags & MDFT_SCALED_VARS) ? scale_table(c) : NULL;
     const double *x = xi;
     double *xs = NULL;
-    if (sc) {
+    if (sc || (flags & MDFT_XI_FP32)) {
         xs = malloc(sizeof(double) * c->nvar);
-        for (size_t i = 0; i < c->nvar; i++) xs[i] = xi[i] / sc[i % c->no];
+        for (size_t i = 0; i < c->nvar; i++) {
+            const double v = (flags & MDFT_XI_FP32) ? (double)((const float *)xi)[i] : xi[i];
+            xs[i] = sc ? v / sc[i % c->no] : v;
+        }
         x = xs;
     }
     double *tmp = df ? malloc(sizeof(double) * c->nvar) : NULL;
@@ -278,5 +281,32 @@ int mdft_vec_widen_f32(mdft_ctx *c, size_t n, const float *in, double *out)
 {
     (void)c;
     for (size_t i = 0; i < n; i++) out[i] = (double)in[i];
+    return 0;
+}
+
+int mdft_vec_combine_step_f32x(mdft_ctx *c, size_t n, int k, const double *coef, const float *const *vecs, float *z,
+                               double a, const float *x, float *y)
+{
+    (void)c;
+    for (size_t i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < k; j++) s += coef[j] * vecs[j][i];
+        z[i] = (float)s;
+        y[i] = (float)((double)x[i] + a * (double)z[i]);
+    }
+    return 0;
+}
+
+int mdft_vec_axpy_f32x(mdft_ctx *c, size_t n, double a, const float *z, const float *x, float *y)
+{
+    (void)c;
+    for (size_t i = 0; i < n; i++) y[i] = (float)((double)x[i] + a * (double)z[i]);
+    return 0;
+}
+
+int mdft_vec_narrow_f64(mdft_ctx *c, size_t n, const double *in, float *out)
+{
+    (void)c;
+    for (size_t i = 0; i < n; i++) out[i] = (float)in[i];
     return 0;
 }

@@ -405,3 +405,51 @@ def test_gradient_products_in_the_evaluation(precision, df32):
     ref = np.array([g @ vs[0].astype(np.float64), g @ g, g @ vs[1].astype(np.float64)])
     say(f"  dots {dots}, numpy {ref}")
     assert np.allclose(dots, ref, rtol=1e-12, atol=1e-12 * np.abs(g).max() * np.sqrt(n))
+
+
+def test_fp32_iterate_evaluation_and_minimisation():
+    """MDFT_XI_FP32: an evaluation that reads xi as float gives the evaluation of the FP64
+    array holding the same values, to the last bit; the minimiser with its iterate in FP32
+    as well (FP32 vectors, scaled variables) reaches the 0.05 kJ/mol band of the FP64
+    minimum with the reference's stop rule (fold - f) <= 1e-3."""
+    import ctypes as C
+    from mdft import warmstart as ws
+    from mdft.gpu import DEVICE_PTRS, XI_FP32, DeviceBuffer, Engine
+    from mdft.minimize import minimize
+    w = workloads.Workload("x32", 2, (24, 24, 24), (14.0, 14.0, 14.0), 5, 512)
+    say("[fp32 iterate] evaluation and minimisation, 24^3 mmax=5")
+    table = ck.synthetic_ck(5, 512, 0.05, seed=43, n0=w.n0)
+    sol = workloads.Solute(np.array([[7.0, 7.0, 7.0], [8.3, 7.2, 6.5]]), np.array([0.5, -0.5]),
+                           np.array([3.2, 2.9]), np.array([0.6, 0.5]))
+    e64 = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, "fp64")
+    e64.set_ck(table)
+    e64.compute_vext(sol, w.solvent)
+    e32 = Engine(w.n, w.length, w.mmax, 2, w.kT, w.n0, "fp32")
+    e32.set_ck(table)
+    e32.compute_vext(sol, w.solvent)
+    rng = np.random.default_rng(9)
+    xf = (1.0 + 0.2 * rng.uniform(-1, 1, e32.N * e32.no)).astype(np.float32)
+    x32 = DeviceBuffer(e32, xf.nbytes)
+    x32.upload(xf)
+    x64 = DeviceBuffer(e32, 8 * xf.size)
+    x64.upload(xf.astype(np.float64))
+    for eng in (e32, e64):
+        pa, pb = np.zeros(3), np.zeros(3)
+        dp = C.POINTER(C.c_double)
+        ga, gb = eng.alloc_field(), eng.alloc_field()
+        assert eng.lib.mdft_energy_and_gradient(eng._h, x32.ptr, pa.ctypes.data_as(dp), ga.ptr,
+                                                DEVICE_PTRS | XI_FP32) == 0
+        assert eng.lib.mdft_energy_and_gradient(eng._h, x64.ptr, pb.ctypes.data_as(dp), gb.ptr, DEVICE_PTRS) == 0
+        assert np.array_equal(pa, pb) and np.array_equal(ga.download(), gb.download())
+    x = e64.alloc_field()
+    ws.cold_start(e64, x)
+    ref = minimize(e64, x, m=1, gtol=1e-8, maxeval=3000)
+    out = {}
+    for it32 in (False, True):
+        ws.cold_start(e32, x)
+        r = minimize(e32, x, m=1, gtol=0.0, ftol=1e-3, ftol_absolute=True, maxeval=1500, fnoise=1e-6,
+                     fp32_vectors=True, scaled=True, step0="newton", fp32_iterate=it32)
+        f64 = float(np.sum(e64.energy_and_gradient(x, None)[0]))
+        out[it32] = r.evaluations
+        say(f"  fp32 iterate {it32}: {r.evaluations} evaluations, status {r.status}, FP64 F - F* = {f64 - ref.F:+.4f}")
+        assert r.status == 1 and abs(f64 - ref.F) <= 0.05

@@ -188,3 +188,39 @@ def test_fortran_lbfgs_scaled_variables(host):
     st, _, _ = _minimize(lib, h, xt, m=1, gtol=1e-8, maxeval=3000, vec32=2, step0=1e4)
     assert abs(st[0] - sa[0]) < 1e-7 * abs(sa[0])
     mock.mdft_ctx_destroy(h)
+
+
+def test_fortran_lbfgs_fp32_iterate_and_reference_stop(host):
+    """The iterate stored in FP32 (opts bit 3, with FP32 vectors and scaled variables,
+    evaluations reading xi as float): the same minimum as the FP64 run to the FP32
+    path's resolution, xi handed back in FP64.  The reference's absolute stop
+    (fold - f) <= tol (opts bit 2, src/module_lbfgs_nocedal_mdft.f90:425-427) ends
+    the run with status 1 within 0.05 kJ/mol of the minimum at tol = 1e-3."""
+    lib, mock = host
+    w = workloads.Workload("probe", 0, (9, 9, 9), (10.0, 10.0, 10.0), 3, 256)
+    sol = workloads.Solute(np.array([[5.0, 5.0, 5.0]]), np.array([-0.4]), np.array([3.2]), np.array([0.6]))
+    t = ck.synthetic_ck(3, 256, 0.05, seed=5, n0=w.n0)
+    h, vext = _problem(mock, w, sol, t)
+    x0 = workloads.guess_xi(vext, w.kT)
+    xa = x0.copy()
+    sa, _, _ = _minimize(lib, h, xa, m=1, gtol=1e-8, maxeval=3000)
+    x = x0.copy()
+    s, _, g = _minimize(lib, h, x, m=1, gtol=1e-7, maxeval=3000, vec32=1 | 2 | 8)
+    assert abs(s[0] - sa[0]) <= 1e-6 * abs(sa[0])
+    dp = C.POINTER(C.c_double)
+    mock.mdft_energy_and_gradient.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.c_int]
+    parts, gref = np.zeros(3), np.zeros_like(x)
+    assert mock.mdft_energy_and_gradient(h, x.ctypes.data, parts.ctypes.data_as(dp), gref.ctypes.data, 0) == 0
+    assert abs(parts.sum() - s[0]) <= 1e-9 * abs(s[0])                   # xi handed back is the final iterate
+    # the reference's stop rule
+    gtr = np.zeros_like(x0)
+    stats = np.zeros(16)
+    trace = np.zeros((3000, 3))
+    xb = x0.copy()
+    rc = lib.mdft_minimize_ex(h, xb.size, xb.ctypes.data, gtr.ctypes.data, 1, 0.0, 1e-3, 1e-7, 0.0, 0.0, 3000,
+                              stats.ctypes.data_as(dp), trace.ctypes.data_as(dp), 3000, 4, 0.0)
+    assert rc == 0 and int(stats[7]) == 1
+    assert abs(stats[0] - sa[0]) <= 0.05
+    it = int(stats[6])
+    assert trace[it - 2, 0] - trace[it - 1, 0] <= 1e-3 < trace[it - 3, 0] - trace[it - 2, 0]
+    mock.mdft_ctx_destroy(h)
