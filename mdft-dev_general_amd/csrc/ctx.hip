@@ -132,6 +132,8 @@ void free_pairs(PairSchedule &p) {
     (void)hipFree(p.cstart);
     (void)hipFree(p.ccount);
     (void)hipFree(p.ciq);
+    (void)hipFree(p.tstart);
+    (void)hipFree(p.tinfo);
     p = PairSchedule{};
 }
 
@@ -195,6 +197,16 @@ int build_pairs(mdft_ctx *c) {
     HIPCHK(c, upload(c, &p.cstart, cstart.data(), cstart.size()));
     HIPCHK(c, upload(c, &p.ccount, ccount.data(), ccount.size()));
     HIPCHK(c, upload(c, &p.ciq, ciq.data(), ciq.size()));
+    // tiles of at most 8 pairs of one chunk (one wave each in the matrix-core kernel)
+    std::vector<int> tstart, tinfo;
+    for (size_t k = 0; k < cstart.size(); k++)
+        for (int o = 0; o < ccount[k]; o += 8) {
+            tstart.push_back(cstart[k] + o);
+            tinfo.push_back(std::min(8, ccount[k] - o) | ciq[k] << 4);
+        }
+    p.ntiles = (int)tstart.size();
+    HIPCHK(c, upload(c, &p.tstart, tstart.data(), tstart.size()));
+    HIPCHK(c, upload(c, &p.tinfo, tinfo.data(), tinfo.size()));
     return MDFT_OK;
 }
 
@@ -242,7 +254,10 @@ int run_excess_pipeline(mdft_ctx *c, const double *xi, int stage, bool timers) {
     if (timers) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (stage >= 2) HIPCHK(c, f32 ? launch_fft3d_f32(c, p32, +1) : launch_fft3d_f64(c, p64, +1));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    if (stage >= 3) HIPCHK(c, f32 ? launch_oz32(c, p32) : launch_oz_f64(c, p64));
+    if (stage >= 3) {
+        if (ozm_supported(c)) HIPCHK(c, f32 ? launch_ozm_f32(c, p32) : launch_ozm_f64(c, p64));
+        else HIPCHK(c, f32 ? launch_oz32(c, p32) : launch_oz_f64(c, p64));
+    }
     if (timers) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (stage >= 4) HIPCHK(c, f32 ? launch_fft3d_f32(c, p32, -1) : launch_fft3d_f64(c, p64, -1));
     if (timers) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
@@ -429,6 +444,7 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
               upload(c, &c->d_kx, c->kx.data(), c->kx.size()) == hipSuccess &&
               upload(c, &c->d_ky, c->ky.data(), c->ky.size()) == hipSuccess &&
               upload(c, &c->d_kz, c->kz.data(), c->kz.size()) == hipSuccess;
+    c->ia0_h = ia0;
     ok = ok && hipMalloc(&c->d_proj, proj_elem_bytes(c) * c->N * np) == hipSuccess;
     ok = ok && hipMalloc(&c->d_vext, sizeof(double) * c->N * c->no) == hipSuccess;
     ok = ok && hipMalloc(&c->d_sums, sizeof(double) * 8) == hipSuccess;
@@ -454,6 +470,7 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
             upload(c, &c->d_oz32_lanerow, lanerow.data(), lanerow.size()) != hipSuccess)
             return bad(MDFT_ENOMEM, "device allocation failed");
     }
+    if (ozm_setup(c) != MDFT_OK) return bad(MDFT_ENOMEM, "matrix-core q-space tables");
     AngTables &t = c->ang;
     t.ntheta = c->ntheta;
     t.nphi = c->nphi;
@@ -487,6 +504,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz_ftab, c->d_oz_itab, c->d_oz32_ftab, c->d_oz32_lanerow,
+                    c->d_ozm_frag, c->d_ozm_ftab, c->d_ozm_units, c->d_ozm_rlen,
                     c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -582,6 +600,10 @@ int mdft_set_ck(mdft_ctx *c, int np_file, const int *m, const int *n, const int 
     if (c->d_ck) (void)hipFree(c->d_ck);
     c->d_ck = nullptr;
     HIPCHK(c, upload(c, &c->d_ck, tab.data(), tab.size()));
+    {
+        const int frc = ozm_frags(c, tab, nq, npc, c->ia0_h);
+        if (frc != MDFT_OK) return fail(c, frc, "c(k) fragments of the matrix-core q-space kernel");
+    }
     c->npc = npc;
     c->nq = nq;
     c->dq = dq;

@@ -1,0 +1,696 @@
+// k_ozm.hip -- the q-space step of the excess term (rotation to the q frame,
+// molecular Ornstein-Zernike contraction with c(|q|), rotation back) with the
+// contraction on the matrix cores, FP64 (v_mfma_f64_16x16x4) and FP32
+// (v_mfma_f32_16x16x4), for molecular symmetry order 2 and mmax <= 6.
+//
+// Restates the q-loop of src/module_energy_cproj_mrso.f90:441-674 for each
+// (q, -q) pair: R^m_{mup,khi}(q) chopped at FLT_EPSILON (:491-492), rotation
+// (:505-523), c interpolated linearly in |q| (:534-539), the OZ sums
+// (:561-606), the DBL_EPSILON chop of gamma (:618-623), rotation back with
+// conj(R) (:611-641), the -q side with R(., -khi) and (-1)^m (:517-520,
+// 636-638) and the store rule of self-conjugate Nyquist points (:648-656).
+// The chops apply to the FP64 kernel; the FP32 kernel skips them (their effect,
+// single-precision rounding of one component, is below FP32's own rounding).
+//
+// Work layout.  One wave takes a tile of 8 pairs that share the interpolation
+// interval iq (pairs are sorted by iq on the host); its 16 (side, pair) rows
+// are the M dimension of the matrix products.  Per tile:
+//  1. rotation to the q frame on the vector ALUs, lanes over pairs: lane
+//     (pair p = l & 7, slot j = l >> 3) takes rotation units (m, khi) from a
+//     host-built round table (units of equal m/2 + 1 in a round, so the lanes
+//     run one instruction stream), forms R^m(mup, khi) = exp(-i mup phi_q)
+//     d^m(mup, khi)(theta_q) once per unit entry (d^m from its Fourier series in
+//     theta, coefficients host-built) and uses it for both sides and every mu:
+//       A_q(m, khi, mu) = sum_mup X_q(m, mup, mu) R(mup, khi)
+//       A_-q(m, -khi, mu) = (-1)^m sum_mup X_-q(m, mup, mu) R(mup, khi);
+//     X is read straight from the projection array, A goes to the wave's LDS.
+//  2. OZ contraction: for each khi the rows (m, mu) of both sides are
+//       gamma_s(m, khi, mu) = sum_{(n, nu)} c_{m mu; n nu}^{khi}(|q|) V_s(n, nu)
+//     with V_s(n, nu) = A_s(n, khi, |nu|) for nu < 0 and conj(A_-s(n, khi, nu))
+//     otherwise.  All pairs of the tile share c[iq] and c[iq+1], so with
+//     c = (1 - alpha) c[iq] + alpha c[iq+1] the sums are two matrix products,
+//     [16 rows x 2 ncol] (re/im of V) times a [2 ncol x 4 nrow] real form of
+//     (c[iq], c[iq+1]) whose fragments are precomputed per interval in the
+//     order the matrix instruction reads them (ozm_frags, at c(k) upload);
+//     alpha and 1 - alpha are applied per row after the product.
+//  3. rotation back on the vector ALUs, units (m, mup):
+//       out_q(m, mup, mu) = sum_khi gamma_q(m, khi, mu) conj(R(mup, khi))
+//       out_-q(m, mup, mu) = (-1)^m sum_khi gamma_-q(m, khi, mu) conj(R(mup, -khi))
+//     stored straight to the projection array.
+// The per-entry R is formed exactly as the tiled kernel of k_oz.hip forms it,
+// so the FLT_EPSILON chop decides alike; the OZ sums are summed in the matrix
+// unit's order (a rounding-level difference from the reference's order).
+#include <algorithm>
+#include <cfloat>
+#include <utility>
+#include <vector>
+
+#include "mdft_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Ty;
+template <> struct Ty<double> {
+    using c = double2;
+    using acc = d4;
+};
+template <> struct Ty<float> {
+    using c = float2;
+    using acc = f4;
+};
+
+__device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// Operand layout of both (lane l): A[i][k] with i = l & 15, k = l >> 4; B[k][j] with
+// k = l >> 4, j = l & 15; result register r of lane l holds D[drow(l >> 4, r)][l & 15],
+// rows 4 g + r for f32 and 4 r + g for f64 (measured on gfx950).
+template <typename T> __device__ __forceinline__ constexpr int drow(int g, int r);
+template <> __device__ __forceinline__ constexpr int drow<float>(int g, int r) { return 4 * g + r; }
+template <> __device__ __forceinline__ constexpr int drow<double>(int g, int r) { return 4 * r + g; }
+
+// value of lane (l ^ 1) (CTRL = kSwap1) or (l ^ 2) (kSwap2) of the same quad (DPP quad_perm)
+constexpr int kSwap1 = 0xB1, kSwap2 = 0x4E;
+template <int CTRL> __device__ __forceinline__ float quad_swap(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> __device__ __forceinline__ double quad_swap(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FLT_EPSILON chop of an R entry, keeping the surviving component in single
+// precision (Fortran's default-kind CMPLX of :491-492)
+__device__ inline double2 chop_R(double2 z) {
+    double re = z.x, im = z.y;
+    if (fabs(re) < (double)FLT_EPSILON) {
+        re = 0.0;
+        im = (double)(float)im;
+    }
+    if (fabs(im) < (double)FLT_EPSILON) {
+        im = 0.0;
+        re = (double)(float)re;
+    }
+    return make_double2(re, im);
+}
+__device__ inline float2 chop_R(float2 z) { return z; }
+
+// DBL_EPSILON chop of gamma (:618-623)
+__device__ inline void chop_gamma(double &re, double &im) {
+    if (fabs(re) < DBL_EPSILON && fabs(im) < DBL_EPSILON) {
+        re = 0.0;
+        im = 0.0;
+    }
+    if (fabs(re) < DBL_EPSILON) {
+        re = 0.0;
+        im = (double)(float)im;
+    }
+    if (fabs(im) < DBL_EPSILON) {
+        im = 0.0;
+        re = (double)(float)re;
+    }
+}
+__device__ inline void chop_gamma(float &, float &) {}
+
+// ---- the angular plan of one mmax (molecular symmetry order 2) -------------------
+// Projection index p(m, mup, mu2) = poff(m) + (mup + m) (m/2 + 1) + mu2 (the
+// context's projection order).  OZ block khi: rows (m, mu2), m = |khi|..L, and
+// columns (n, nu2), n = |khi|..L, nu2 = -n/2..n/2, both ascending (the c(k)
+// relayout order of mdft_set_ck).
+template <int L> struct Plan {
+    static constexpr int iabs(int x) { return x < 0 ? -x : x; }
+    static constexpr int nmu(int m) { return m / 2 + 1; }
+    static constexpr int poff(int m) {
+        int o = 0;
+        for (int l = 0; l < m; l++) o += (2 * l + 1) * nmu(l);
+        return o;
+    }
+    static constexpr int np = poff(L + 1);
+    static constexpr int nps = np | 1;  // LDS row stride (odd: rows fall in different banks)
+    static constexpr int kp = ((L + 2) / 2) * 2;  // Fourier coefficients stored per d entry (even)
+    static constexpr int nclass = L / 2 + 1;      // rotation classes (m/2 + 1 = 1 .. nclass)
+    static constexpr int pidx(int m, int mup, int mu2) { return poff(m) + (mup + m) * nmu(m) + mu2; }
+    static constexpr int roff(int m) { return m * (4 * m * m - 1) / 3; }
+    static constexpr int ncols(int khi) {
+        int c = 0;
+        for (int n = iabs(khi); n <= L; n++) c += 2 * (n / 2) + 1;
+        return c;
+    }
+    static constexpr int nrows(int khi) {
+        int r = 0;
+        for (int m = iabs(khi); m <= L; m++) r += nmu(m);
+        return r;
+    }
+    // column c of block khi: projection entry << 1 | (nu2 < 0); padding columns read entry 0
+    static constexpr int col(int khi, int c) {
+        for (int n = iabs(khi); n <= L; n++)
+            for (int nu2 = -(n / 2); nu2 <= n / 2; nu2++) {
+                if (c == 0) return pidx(n, khi, iabs(nu2)) << 1 | (nu2 < 0 ? 1 : 0);
+                c--;
+            }
+        return 1;
+    }
+    // row r of block khi: its projection entry, -1 for padding rows
+    static constexpr int row(int khi, int r) {
+        for (int m = iabs(khi); m <= L; m++)
+            for (int mu2 = 0; mu2 < nmu(m); mu2++) {
+                if (r == 0) return pidx(m, khi, mu2);
+                r--;
+            }
+        return -1;
+    }
+    static constexpr int nkb(int khi) { return (ncols(khi) + 1) / 2; }
+    static constexpr int nnb(int khi) { return (nrows(khi) + 3) / 4; }
+    static constexpr int foff(int khi) {
+        int f = 0;
+        for (int k = -L; k < khi; k++) f += nkb(k) * nnb(k);
+        return f;
+    }
+    static constexpr int nfrag = foff(L + 1);
+};
+
+template <typename T> struct OzmArgs {
+    int nx, ny, np, ntiles, per;
+    double dq;
+    const double *kx, *ky, *kz;
+    const int *pq, *pmq, *pflag;
+    const int *tstart, *tinfo;  // tile: first pair; pair count | iq << 4
+    const T *frag;              // [interval][nfrag][64] B fragments of (c[iq], c[iq+1])
+    const T *ftab;              // [entry][kp] Fourier coefficients of d^m(theta)
+    const int4 *units;          // [round][8] rotation units
+    const int *rlen;            // [round] iterations of each round
+    int cr0[2][4], cnr[2][4];   // first round and round count of each (direction, class)
+};
+
+template <typename T, int L> struct Lds {
+    using C = typename Ty<T>::c;
+    C buf[16][Plan<L>::nps];   // per (side, pair) row: A, then gamma
+    T trig[8][2][Plan<L>::kp]; // cos k theta, sin k theta of each pair
+    C ph[8][Plan<L>::kp];      // exp(+i k phi) of each pair
+    C ab[8];                   // (alpha, 1 - alpha) of each pair
+};
+
+// ---- rotations: one class of units (m/2 + 1 = NMU) ----------------------------------
+template <typename T, int L, int NMU, bool BACK>
+__device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int r0, int nr, int p, int j,
+                                          const typename Ty<T>::c *__restrict__ xq,
+                                          const typename Ty<T>::c *__restrict__ xmq, typename Ty<T>::c *__restrict__ oq,
+                                          typename Ty<T>::c *__restrict__ omq, bool store_q, bool store_mq,
+                                          bool conj_mq) {
+    using C = typename Ty<T>::c;
+    constexpr int KP = Plan<L>::kp;
+    constexpr int KC = (2 * NMU < KP) ? 2 * NMU : KP;  // coefficients that can be nonzero for m <= 2 NMU - 1
+    const T *ct = s.trig[p][0], *st = s.trig[p][1];
+    for (int r = 0; r < nr; r++) {
+        const int4 u = a.units[(r0 + r) * 8 + j];
+        const int len = a.rlen[r0 + r];
+        const int x0 = u.x & 0xffff, e1 = u.x >> 16, e2 = u.y & 0xffff, f0 = u.y >> 16;
+        const int m = u.z & 255, w = (u.z >> 8) & 255, p0 = (u.z >> 16) & 1;
+        const int sub = u.w - 64;  // khi (rotation in) or mup (rotation back)
+        T ta[KC], tb[KC];          // trig of the even / odd iterations: sin when mup - khi is odd
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            ta[k] = p0 ? st[k] : ct[k];
+            tb[k] = p0 ? ct[k] : st[k];
+        }
+        C acq[NMU], acm[NMU];
+#pragma unroll
+        for (int i = 0; i < NMU; i++) {
+            acq[i].x = acq[i].y = (T)0;
+            acm[i].x = acm[i].y = (T)0;
+        }
+        C eph;  // rotation back: exp(-i mup phi) of the unit
+        {
+            const C z = s.ph[p][sub < 0 ? -sub : sub];
+            eph.x = z.x;
+            eph.y = sub < 0 ? z.y : -z.y;
+        }
+        const int fstep = BACK ? KP : w * KP;
+        auto step = [&](int t, const T(&tg)[KC]) {
+            if (t >= w) return;
+            const C *F = reinterpret_cast<const C *>(a.ftab + f0 + t * fstep);
+            T v = (T)0;
+#pragma unroll
+            for (int k = 0; k < KC / 2; k++) {
+                const C f = F[k];
+                v += f.x * tg[2 * k];
+                v += f.y * tg[2 * k + 1];
+            }
+            C e;
+            if (BACK) {
+                e = eph;
+            } else {
+                const int mup = t - m;
+                const C z = s.ph[p][mup < 0 ? -mup : mup];
+                e.x = z.x;
+                e.y = mup < 0 ? z.y : -z.y;
+            }
+            C R;
+            R.x = e.x * v;
+            R.y = e.y * v;
+            R = chop_R(R);
+#pragma unroll
+            for (int i = 0; i < NMU; i++) {
+                if (!BACK) {
+                    const C x1 = xq[x0 + t * NMU + i], x2 = xmq[x0 + t * NMU + i];
+                    acq[i].x += x1.x * R.x - x1.y * R.y;
+                    acq[i].y += x1.x * R.y + x1.y * R.x;
+                    acm[i].x += x2.x * R.x - x2.y * R.y;
+                    acm[i].y += x2.x * R.y + x2.y * R.x;
+                } else {
+                    const C g1 = s.buf[p][x0 + t * NMU + i], g2 = s.buf[8 + p][x0 + (2 * m - t) * NMU + i];
+                    acq[i].x += g1.x * R.x + g1.y * R.y;  // g conj(R)
+                    acq[i].y += g1.y * R.x - g1.x * R.y;
+                    acm[i].x += g2.x * R.x + g2.y * R.y;
+                    acm[i].y += g2.y * R.x - g2.x * R.y;
+                }
+            }
+        };
+        for (int t = 0; t < len; t += 2) {
+            step(t, ta);
+            step(t + 1, tb);
+        }
+        if (w > 0) {
+            const T sg = (m & 1) ? (T)-1 : (T)1;
+#pragma unroll
+            for (int i = 0; i < NMU; i++) {
+                if (!BACK) {
+                    s.buf[p][e1 + i] = acq[i];
+                    C o;
+                    o.x = sg * acm[i].x;
+                    o.y = sg * acm[i].y;
+                    s.buf[8 + p][e2 + i] = o;
+                } else {
+                    if (store_q) oq[e1 + i] = acq[i];
+                    if (store_mq) {
+                        C o;
+                        o.x = sg * acm[i].x;
+                        o.y = sg * acm[i].y;
+                        if (conj_mq) o.y = -o.y;
+                        omq[e1 + i] = o;
+                    }
+                }
+            }
+        }
+        (void)e2;
+        (void)sub;
+    }
+}
+
+template <typename T, int L, int NMU, bool BACK>
+__device__ __forceinline__ void rot_all(const OzmArgs<T> &a, Lds<T, L> &s, int p, int j,
+                                        const typename Ty<T>::c *xq, const typename Ty<T>::c *xmq,
+                                        typename Ty<T>::c *oq, typename Ty<T>::c *omq, bool store_q, bool store_mq,
+                                        bool conj_mq) {
+    rot_class<T, L, NMU, BACK>(a, s, a.cr0[BACK][NMU - 1], a.cnr[BACK][NMU - 1], p, j, xq, xmq, oq, omq, store_q,
+                               store_mq, conj_mq);
+    if constexpr (NMU > 1) rot_all<T, L, NMU - 1, BACK>(a, s, p, j, xq, xmq, oq, omq, store_q, store_mq, conj_mq);
+}
+
+// ---- OZ block khi on the matrix cores -------------------------------------------------
+// A operand (16 x 4 per instruction): lane l holds row l & 15 (side (l >> 3) & 1, pair
+// l & 7), k = l >> 4 of the block: column c = 2 kb + (l >> 5), re (k even) or im.
+template <typename T, int L, int KHI, int KB>
+__device__ __forceinline__ T xfrag(const typename Ty<T>::c *own, const typename Ty<T>::c *oth, bool hi, bool pk) {
+    constexpr int c0 = Plan<L>::col(KHI, 2 * KB), c1 = Plan<L>::col(KHI, 2 * KB + 1);
+    const int e = hi ? (c1 >> 1) : (c0 >> 1);
+    const bool isown = hi ? (c1 & 1) : (c0 & 1);
+    const typename Ty<T>::c v = (isown ? own : oth)[e];
+    return pk ? (isown ? v.y : -v.y) : v.x;
+}
+
+template <typename T, int L, int KHI, int... KB>
+__device__ __forceinline__ void load_x(T *x, const typename Ty<T>::c *own, const typename Ty<T>::c *oth, bool hi,
+                                       bool pk, std::integer_sequence<int, KB...>) {
+    ((x[KB] = xfrag<T, L, KHI, KB>(own, oth, hi, pk)), ...);
+}
+
+// D (16 x 16): lane l holds rows drow(l >> 4, i), i = 0..3, column l & 15 = 4 rr + q of the
+// n-block: OZ row 4 nb + rr, q = re lo, im lo, re hi, im hi.  gamma = (1 - alpha) lo + alpha hi
+// goes to row (side, pair) = drow(l >> 4, q), the other three values of the lane serving its quad.
+template <typename T, int L, int KHI, int NB>
+__device__ __forceinline__ void gamma_store(typename Ty<T>::acc d, Lds<T, L> &s, const T *al, const T *be, int l) {
+    using C = typename Ty<T>::c;
+    constexpr int r0 = Plan<L>::row(KHI, 4 * NB), r1 = Plan<L>::row(KHI, 4 * NB + 1);
+    constexpr int r2 = Plan<L>::row(KHI, 4 * NB + 2), r3 = Plan<L>::row(KHI, 4 * NB + 3);
+    const int q = l & 3, rr = (l >> 2) & 3;
+    const bool lo_here = q < 2;
+    T comp[4], oth[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const T v = d[i], pv = quad_swap<kSwap2>(v);
+        const T lo = lo_here ? v : pv, hi = lo_here ? pv : v;
+        comp[i] = be[i] * lo + al[i] * hi;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) oth[i] = quad_swap<kSwap1>(comp[i]);
+    const T cs = q == 0 ? comp[0] : (q == 1 ? comp[1] : (q == 2 ? comp[2] : comp[3]));
+    const T os = q == 0 ? oth[0] : (q == 1 ? oth[1] : (q == 2 ? oth[2] : oth[3]));
+    T re = (q & 1) ? os : cs, im = (q & 1) ? cs : os;
+    chop_gamma(re, im);
+    const int row = rr == 0 ? r0 : (rr == 1 ? r1 : (rr == 2 ? r2 : r3));
+    if (row >= 0) {
+        C g;
+        g.x = re;
+        g.y = im;
+        s.buf[drow<T>(l >> 4, q)][row] = g;
+    }
+}
+
+template <typename T, int L, int KHI, int... NB>
+__device__ __forceinline__ void store_all(typename Ty<T>::acc *acc, Lds<T, L> &s, const T *al, const T *be, int l,
+                                          std::integer_sequence<int, NB...>) {
+    (gamma_store<T, L, KHI, NB>(acc[NB], s, al, be, l), ...);
+}
+
+template <typename T, int L, int KHI>
+__device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr, const T *al, const T *be, int l) {
+    using P = Plan<L>;
+    using A = typename Ty<T>::acc;
+    constexpr int NKB = P::nkb(KHI), NNB = P::nnb(KHI), F0 = P::foff(KHI);
+    const int row16 = l & 15;
+    const bool hi = (l >> 5) & 1, pk = (l >> 4) & 1;
+    T x[NKB];
+    load_x<T, L, KHI>(x, s.buf[row16], s.buf[row16 ^ 8], hi, pk, std::make_integer_sequence<int, NKB>{});
+    A acc[NNB];
+#pragma unroll
+    for (int nb = 0; nb < NNB; nb++) acc[nb] = A{0, 0, 0, 0};
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+        for (int nb = 0; nb < NNB; nb++) acc[nb] = mfma16(x[kb], fr[(F0 + kb * NNB + nb) * 64 + l], acc[nb]);
+    // every read of block khi precedes its writes in the wave's LDS order
+    store_all<T, L, KHI>(acc, s, al, be, l, std::make_integer_sequence<int, NNB>{});
+    if constexpr (KHI < L) oz_blocks<T, L, KHI + 1>(s, fr, al, be, l);
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(64) void ozm(typename Ty<T>::c *__restrict__ proj, OzmArgs<T> a) {
+    using P = Plan<L>;
+    using C = typename Ty<T>::c;
+    constexpr int KP = P::kp;
+    __shared__ Lds<T, L> s;
+    const int l = threadIdx.x;
+    const int p = l & 7, j = l >> 3;
+    // XCD-aware order: consecutive workgroups go to different XCDs, so each XCD takes a
+    // contiguous range of tiles (tiles of one interval share its c fragments in that L2)
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * a.per + (b >> 3);
+    if (tile >= a.ntiles) return;
+    const int ts = a.tstart[tile], ti = a.tinfo[tile];
+    const int cnt = ti & 15, iq = ti >> 4;
+    const bool valid = p < cnt;
+    const int pr = ts + (valid ? p : 0);
+    const int gq = a.pq[pr], gmq = a.pmq[pr], flag = a.pflag[pr];
+    if (j == 0) {
+        const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
+        const double q0 = a.kx[ix], q1 = a.ky[iy], q2 = a.kz[iz];
+        const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2), rxy = sqrt(q0 * q0 + q1 * q1);
+        const double al = qn / a.dq + 1.0 - (double)iq;  // relative to the tile's row, as on the host
+        C abv;
+        abv.x = (T)al;
+        abv.y = (T)(1.0 - al);
+        s.ab[p] = abv;
+        // frame angles (q along -z: theta = pi, phi = pi; q = 0: the z frame)
+        const double ctt = qn > 0.0 ? q2 / qn : 1.0, stt = qn > 0.0 ? rxy / qn : 0.0;
+        const double cp = rxy > 0.0 ? q0 / rxy : (q2 < 0.0 ? -1.0 : 1.0);
+        const double sp = rxy > 0.0 ? q1 / rxy : 0.0;
+        double ck = 1.0, sk = 0.0, cpk = 1.0, spk = 0.0;
+        for (int k = 0; k < KP; k++) {
+            s.trig[p][0][k] = (T)ck;
+            s.trig[p][1][k] = (T)sk;
+            C e;
+            e.x = (T)cpk;
+            e.y = (T)spk;
+            s.ph[p][k] = e;
+            const double nc = ck * ctt - sk * stt;
+            sk = sk * ctt + ck * stt;
+            ck = nc;
+            const double nq = cpk * cp - spk * sp;
+            spk = spk * cp + cpk * sp;
+            cpk = nq;
+        }
+    }
+    wave_sync();
+    const int np = P::np;
+    const C *xq = proj + (size_t)gq * np, *xmq = proj + (size_t)gmq * np;
+    rot_all<T, L, P::nclass, false>(a, s, p, j, xq, xmq, nullptr, nullptr, false, false, false);
+    wave_sync();
+    T al[4], be[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const C v = s.ab[drow<T>(l >> 4, i) & 7];
+        al[i] = v.x;
+        be[i] = v.y;
+    }
+    const T *fr = a.frag + (size_t)(iq - 1) * P::nfrag * 64;
+    oz_blocks<T, L, -L>(s, fr, al, be, l);
+    wave_sync();
+    const bool eq = flag & 1, nyq = flag & 2;
+    C *oq = proj + (size_t)gq * np, *omq = proj + (size_t)gmq * np;
+    rot_all<T, L, P::nclass, true>(a, s, p, j, nullptr, nullptr, oq, omq, valid && !eq, valid, eq && nyq);
+}
+
+// ---- host side ---------------------------------------------------------------------
+double textbook_d_h(int j, int mp, int m, double beta) {
+    auto f = [](int n) {
+        double r = 1.0;
+        for (int i = 2; i <= n; i++) r *= i;
+        return r;
+    };
+    double s = 0.0;
+    for (int k = 0; k <= 2 * j; k++) {
+        const int a = j + m - k, b = k, c = mp - m + k, e = j - mp - k;
+        if (a < 0 || b < 0 || c < 0 || e < 0) continue;
+        s += (((mp - m + k) & 1) ? -1.0 : 1.0) * pow(cos(beta / 2), 2 * j + m - mp - 2 * k) *
+             pow(sin(beta / 2), mp - m + 2 * k) / (f(a) * f(b) * f(c) * f(e));
+    }
+    return sqrt(f(j + mp) * f(j - mp) * f(j + m) * f(j - m)) * s;
+}
+
+struct HostPlan {
+    int L, np, kp, nclass, nfrag;
+    std::vector<int> col[2 * MDFT_MAX_MMAX + 1], row[2 * MDFT_MAX_MMAX + 1];
+    std::vector<int> nkb, nnb, foff;
+};
+
+template <int L> HostPlan host_plan() {
+    using P = Plan<L>;
+    HostPlan h;
+    h.L = L;
+    h.np = P::np;
+    h.kp = P::kp;
+    h.nclass = P::nclass;
+    h.nfrag = P::nfrag;
+    for (int khi = -L; khi <= L; khi++) {
+        for (int c = 0; c < 2 * P::nkb(khi); c++) h.col[khi + L].push_back(P::col(khi, c));
+        for (int r = 0; r < 4 * P::nnb(khi); r++) h.row[khi + L].push_back(P::row(khi, r));
+        h.nkb.push_back(P::nkb(khi));
+        h.nnb.push_back(P::nnb(khi));
+        h.foff.push_back(P::foff(khi));
+    }
+    return h;
+}
+
+HostPlan host_plan_of(int L) {
+    switch (L) {
+        case 0: return host_plan<0>();
+        case 1: return host_plan<1>();
+        case 2: return host_plan<2>();
+        case 3: return host_plan<3>();
+        case 4: return host_plan<4>();
+        case 5: return host_plan<5>();
+        default: return host_plan<6>();
+    }
+}
+
+template <typename T, int L> hipError_t launch_ozm_t(mdft_ctx *c, typename Ty<T>::c *proj) {
+    OzmArgs<T> a;
+    a.nx = c->d.nx;
+    a.ny = c->d.ny;
+    a.np = c->np;
+    a.ntiles = c->pairs.ntiles;
+    a.per = (c->pairs.ntiles + 7) / 8;
+    a.dq = c->dq;
+    a.kx = c->d_kx;
+    a.ky = c->d_ky;
+    a.kz = c->d_kz;
+    a.pq = c->pairs.pq;
+    a.pmq = c->pairs.pmq;
+    a.pflag = c->pairs.pflag;
+    a.tstart = c->pairs.tstart;
+    a.tinfo = c->pairs.tinfo;
+    a.frag = static_cast<const T *>(c->d_ozm_frag);
+    a.ftab = static_cast<const T *>(c->d_ozm_ftab);
+    a.units = reinterpret_cast<const int4 *>(c->d_ozm_units);
+    a.rlen = c->d_ozm_rlen;
+    for (int d = 0; d < 2; d++)
+        for (int k = 0; k < 4; k++) {
+            a.cr0[d][k] = c->ozm_cr0[d][k];
+            a.cnr[d][k] = c->ozm_cnr[d][k];
+        }
+    if (a.ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL((ozm<T, L>), dim3((unsigned)(8 * a.per)), dim3(64), 0, c->stream, proj, a);
+    return hipGetLastError();
+}
+
+template <typename T> hipError_t launch_ozm_any(mdft_ctx *c, typename Ty<T>::c *proj) {
+    switch (c->L) {
+        case 0: return launch_ozm_t<T, 0>(c, proj);
+        case 1: return launch_ozm_t<T, 1>(c, proj);
+        case 2: return launch_ozm_t<T, 2>(c, proj);
+        case 3: return launch_ozm_t<T, 3>(c, proj);
+        case 4: return launch_ozm_t<T, 4>(c, proj);
+        case 5: return launch_ozm_t<T, 5>(c, proj);
+        case 6: return launch_ozm_t<T, 6>(c, proj);
+        default: return hipErrorInvalidConfiguration;
+    }
+}
+
+template <typename T> int upload_t(mdft_ctx *c, void **dst, const std::vector<double> &src) {
+    std::vector<T> v(src.begin(), src.end());
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    if (hipMalloc(dst, sizeof(T) * (v.size() ? v.size() : 1)) != hipSuccess) return MDFT_ENOMEM;
+    if (!v.empty() && hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return MDFT_EHIP;
+    return MDFT_OK;
+}
+
+}  // namespace
+
+bool ozm_supported(const mdft_ctx *c) {
+    return c->d.mrso == 2 && c->L >= 0 && c->L <= 6;
+}
+
+// Fourier coefficients of d^m(theta) per entry (m, a, b) (kp per entry, zero beyond m) and
+// the rotation round tables of the context's mmax.
+int ozm_setup(mdft_ctx *c) {
+    if (!ozm_supported(c)) return MDFT_OK;
+    const HostPlan h = host_plan_of(c->L);
+    const int L = c->L, kp = h.kp;
+    auto roff = [](int m) { return m * (4 * m * m - 1) / 3; };
+    const double hp = 0.5 * acos(-1.0);
+    std::vector<double> ftab((size_t)roff(L + 1) * kp, 0.0);
+    for (int m = 0; m <= L; m++) {
+        const int W = 2 * m + 1;
+        std::vector<double> D(W * W);
+        for (int k = -m; k <= m; k++)
+            for (int x = -m; x <= m; x++) D[(k + m) * W + (x + m)] = textbook_d_h(m, k, x, hp);
+        // d^m_{a,b}(theta) = i^{a-b} sum_k D_{k,a} D_{k,b} exp(-i k theta) with D = d^m(pi/2):
+        // cos(k theta) coefficients for even a - b, sin(k theta) for odd
+        for (int x = -m; x <= m; x++)
+            for (int y = -m; y <= m; y++) {
+                const int ab = ((x - y) % 4 + 4) % 4;
+                const size_t e = (size_t)roff(m) + (x + m) * W + (y + m);
+                for (int k = 0; k <= m; k++) {
+                    const double sk = D[(k + m) * W + (x + m)] * D[(k + m) * W + (y + m)];
+                    const double smk = D[(-k + m) * W + (x + m)] * D[(-k + m) * W + (y + m)];
+                    double v;
+                    if (k == 0) v = (ab == 0) ? sk : (ab == 2 ? -sk : 0.0);
+                    else if (ab == 0) v = sk + smk;
+                    else if (ab == 2) v = -(sk + smk);
+                    else if (ab == 1) v = sk - smk;
+                    else v = -(sk - smk);
+                    ftab[e * kp + k] = v;
+                }
+            }
+    }
+    auto nmu = [](int m) { return m / 2 + 1; };
+    std::vector<int> poff(L + 2, 0);
+    for (int m = 0; m <= L; m++) poff[m + 1] = poff[m] + (2 * m + 1) * nmu(m);
+    auto pidx = [&](int m, int mup, int mu2) { return poff[m] + (mup + m) * nmu(m) + mu2; };
+    std::vector<int> units, rlen;
+    for (int back = 0; back < 2; back++)
+        for (int k = 1; k <= 4; k++) {
+            c->ozm_cr0[back][k - 1] = (int)rlen.size();
+            c->ozm_cnr[back][k - 1] = 0;
+            if (k > h.nclass) continue;
+            std::vector<std::pair<int, int>> us;  // (m, khi or mup), m descending
+            for (int m = std::min(L, 2 * k - 1); m >= 2 * k - 2; m--)
+                for (int x = -m; x <= m; x++) us.push_back({m, x});
+            for (size_t u0 = 0; u0 < us.size(); u0 += 8) {
+                rlen.push_back(2 * us[u0].first + 1);
+                c->ozm_cnr[back][k - 1]++;
+                for (int jj = 0; jj < 8; jj++) {
+                    int4 e{0, 0, 0, 64};
+                    if (u0 + jj < us.size()) {
+                        const int m = us[u0 + jj].first, x = us[u0 + jj].second, W = 2 * m + 1;
+                        if (!back) {  // rotation in, unit (m, khi = x): iterate mup
+                            e.x = poff[m] | pidx(m, x, 0) << 16;
+                            e.y = pidx(m, -x, 0) | ((roff(m) + (x + m)) * kp) << 16;
+                            e.z = m | W << 8 | ((m + x) & 1) << 16;
+                        } else {  // rotation back, unit (m, mup = x): iterate khi
+                            e.x = poff[m] | pidx(m, x, 0) << 16;
+                            e.y = 0 | ((roff(m) + (x + m) * W) * kp) << 16;
+                            e.z = m | W << 8 | ((m + x) & 1) << 16;
+                        }
+                        e.w = x + 64;
+                    }
+                    units.push_back(e.x);
+                    units.push_back(e.y);
+                    units.push_back(e.z);
+                    units.push_back(e.w);
+                }
+            }
+        }
+    int rc = c->d.precision == MDFT_PREC_FP64 ? upload_t<double>(c, &c->d_ozm_ftab, ftab)
+                                                : upload_t<float>(c, &c->d_ozm_ftab, ftab);
+    if (rc != MDFT_OK) return rc;
+    if (hipMalloc((void **)&c->d_ozm_units, sizeof(int) * units.size()) != hipSuccess ||
+        hipMalloc((void **)&c->d_ozm_rlen, sizeof(int) * rlen.size()) != hipSuccess)
+        return MDFT_ENOMEM;
+    if (hipMemcpy(c->d_ozm_units, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_ozm_rlen, rlen.data(), sizeof(int) * rlen.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return MDFT_EHIP;
+    return MDFT_OK;
+}
+
+// B fragments of every interpolation interval (rows iq - 1 and iq of the relayouted c(k)
+// table tab[nq][npc], ia0 = first coefficient of each projection row).  Fragment f of block
+// khi (kb, nb), lane l: column c = 2 kb + (l >> 5), re/im of V = (l >> 4) & 1, OZ row
+// r = 4 nb + ((l & 15) >> 2), q = l & 3 (re lo, im lo, re hi, im hi); the real form of the
+// complex product V c: re <- (re V re c - im V im c), im <- (re V im c + im V re c).
+int ozm_frags(mdft_ctx *c, const std::vector<double2> &tab, int nq, int npc, const std::vector<int> &ia0) {
+    if (!ozm_supported(c)) return MDFT_OK;
+    const HostPlan h = host_plan_of(c->L);
+    const int L = c->L, nint = nq - 1;
+    std::vector<double> fr((size_t)nint * h.nfrag * 64, 0.0);
+    for (int ii = 0; ii < nint; ii++)
+        for (int khi = -L; khi <= L; khi++) {
+            const int kk = khi + L, nkb = h.nkb[kk], nnb = h.nnb[kk];
+            int ncols = 0;
+            for (int n = std::abs(khi); n <= L; n++) ncols += 2 * (n / 2) + 1;
+            for (int kb = 0; kb < nkb; kb++)
+                for (int nb = 0; nb < nnb; nb++) {
+                    const size_t f = (size_t)ii * h.nfrag + h.foff[kk] + kb * nnb + nb;
+                    for (int l = 0; l < 64; l++) {
+                        const int cc = 2 * kb + (l >> 5), pk = (l >> 4) & 1, r = 4 * nb + ((l & 15) >> 2), q = l & 3;
+                        const int rowent = h.row[kk][r];
+                        if (cc >= ncols || rowent < 0) continue;
+                        const double2 v = tab[(size_t)(ii + (q >> 1)) * npc + ia0[rowent] + cc];
+                        const int qc = q & 1;
+                        fr[f * 64 + l] = pk == 0 ? (qc == 0 ? v.x : v.y) : (qc == 0 ? -v.y : v.x);
+                    }
+                }
+        }
+    return c->d.precision == MDFT_PREC_FP64 ? upload_t<double>(c, &c->d_ozm_frag, fr)
+                                              : upload_t<float>(c, &c->d_ozm_frag, fr);
+}
+
+hipError_t launch_ozm_f64(mdft_ctx *c, double2 *proj) { return launch_ozm_any<double>(c, proj); }
+hipError_t launch_ozm_f32(mdft_ctx *c, float2 *proj) { return launch_ozm_any<float>(c, proj); }

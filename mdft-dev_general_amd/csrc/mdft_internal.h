@@ -54,6 +54,9 @@ struct PairSchedule {
     int *cstart = nullptr;    // first pair of each chunk
     int *ccount = nullptr;    // pairs in each chunk
     int *ciq = nullptr;       // 1-based lower interpolation row iq of each chunk
+    int ntiles = 0;           // tiles of at most 8 pairs of one chunk (the matrix-core q-space kernel)
+    int *tstart = nullptr;    // first pair of each tile
+    int *tinfo = nullptr;     // pair count | iq << 4 of each tile
 };
 
 struct mdft_ctx {
@@ -82,6 +85,12 @@ struct mdft_ctx {
     int oz_ntin = 0, oz_ntout = 0;  // its rotation task counts
     float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
     int *d_oz32_lanerow = nullptr; // descriptor table of the FP32 q-space kernel (rotation offsets, lane rows, OZ columns)
+    void *d_ozm_frag = nullptr;   // matrix-core q-space kernel: c(k) B fragments per interval (FP64 or FP32)
+    void *d_ozm_ftab = nullptr;   // its Fourier coefficients of d^m(theta)
+    int *d_ozm_units = nullptr;   // its rotation unit rounds [round][8] int4
+    int *d_ozm_rlen = nullptr;    // iterations of each round
+    int ozm_cr0[2][4] = {}, ozm_cnr[2][4] = {};  // first round and count per (direction, class)
+    std::vector<int> ia0_h;       // first OZ coefficient of each projection row (host)
     void *d_bf16_M = nullptr;     // angl2proj matrix, BF16 MFMA fragment order
     void *d_bf16_Q = nullptr;     // proj2angl matrix, BF16 MFMA fragment order
     PairSchedule pairs;
@@ -134,6 +143,11 @@ hipError_t launch_reduce3(mdft_ctx *c, int nblocks);
 hipError_t launch_fft3d_f64(mdft_ctx *c, double2 *proj, int sign, int batch = 0);  // batch 0: np
 hipError_t launch_fft3d_f32(mdft_ctx *c, float2 *proj, int sign, int batch = 0);
 hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj);
+bool ozm_supported(const mdft_ctx *c);
+int ozm_setup(mdft_ctx *c);
+int ozm_frags(mdft_ctx *c, const std::vector<double2> &tab, int nq, int npc, const std::vector<int> &ia0);
+hipError_t launch_ozm_f64(mdft_ctx *c, double2 *proj);
+hipError_t launch_ozm_f32(mdft_ctx *c, float2 *proj);
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n);
 hipError_t launch_angl2proj_feat(mdft_ctx *c, const double *src, int feat, const double *aux = nullptr);
 hipError_t launch_transform_f64(mdft_ctx *c, const double *f, double2 *proj, size_t npts, bool to_proj);
