@@ -183,6 +183,8 @@ template <int L> struct Plan {
     static constexpr int nfrag = foff(L + 1);
 };
 
+constexpr int kWaves = 4;  // waves per tile
+
 template <typename T> struct OzmArgs {
     int nx, ny, np, ntiles, per;
     double dq;
@@ -193,28 +195,34 @@ template <typename T> struct OzmArgs {
     const T *ftab;              // [entry][kp] Fourier coefficients of d^m(theta)
     const int4 *units;          // [round][8] rotation units
     const int *rlen;            // [round] iterations of each round
-    int cr0[2][4], cnr[2][4];   // first round and round count of each (direction, class)
+    int cr0[2][kWaves][4], cnr[2][kWaves][4];  // first round and count per (direction, wave, class)
 };
 
+// LDS of one tile: X (then the rotated-back outputs) and A (then gamma) per (side, pair)
+// row, the pairs' angles, interpolation weights and grid indices
 template <typename T, int L> struct Lds {
     using C = typename Ty<T>::c;
-    C buf[16][Plan<L>::nps];   // per (side, pair) row: A, then gamma
-    T trig[8][2][Plan<L>::kp]; // cos k theta, sin k theta of each pair
-    C ph[8][Plan<L>::kp];      // exp(+i k phi) of each pair
-    C ab[8];                   // (alpha, 1 - alpha) of each pair
+    C xs[16][Plan<L>::nps];
+    C buf[16][Plan<L>::nps];
+    T trig[8][2][Plan<L>::kp];  // cos k theta, sin k theta of each pair
+    C ph[8][Plan<L>::kp];       // exp(+i k phi) of each pair
+    C ab[8];                    // (alpha, 1 - alpha) of each pair
+    int g[16];                  // grid index of each row (q side rows 0-7, -q side rows 8-15)
+    int flag[8];                // pair flags; bit 2: padding pair of a partial tile
 };
 
 // ---- rotations: one class of units (m/2 + 1 = NMU) ----------------------------------
+// rotation in: X from s.xs, A to s.buf; rotation back: gamma from s.buf, the outputs
+// (sign, Nyquist conjugate applied) to s.xs
 template <typename T, int L, int NMU, bool BACK>
-__device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int r0, int nr, int p, int j,
-                                          const typename Ty<T>::c *__restrict__ xq,
-                                          const typename Ty<T>::c *__restrict__ xmq, typename Ty<T>::c *__restrict__ oq,
-                                          typename Ty<T>::c *__restrict__ omq, bool store_q, bool store_mq,
-                                          bool conj_mq) {
+__device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int r0, int nr, int p, int j) {
     using C = typename Ty<T>::c;
     constexpr int KP = Plan<L>::kp;
     constexpr int KC = (2 * NMU < KP) ? 2 * NMU : KP;  // coefficients that can be nonzero for m <= 2 NMU - 1
     const T *ct = s.trig[p][0], *st = s.trig[p][1];
+    const C *xq = s.xs[p], *xmq = s.xs[8 + p];
+    const C *gq = s.buf[p], *gmq = s.buf[8 + p];
+    const bool conj_mq = (s.flag[p] & 3) == 3;  // q == -q on a Nyquist plane
     for (int r = 0; r < nr; r++) {
         const int4 u = a.units[(r0 + r) * 8 + j];
         const int len = a.rlen[r0 + r];
@@ -240,15 +248,23 @@ __device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int
             eph.y = sub < 0 ? z.y : -z.y;
         }
         const int fstep = BACK ? KP : w * KP;
-        auto step = [&](int t, const T(&tg)[KC]) {
+        // Fourier coefficients of one iteration, loaded one iteration ahead of their use
+        struct Ops {
+            C f[KC / 2];
+        };
+        auto load = [&](int t, Ops &o) {
             if (t >= w) return;
             const C *F = reinterpret_cast<const C *>(a.ftab + f0 + t * fstep);
+#pragma unroll
+            for (int k = 0; k < KC / 2; k++) o.f[k] = F[k];
+        };
+        auto step = [&](int t, const Ops &o, const T(&tg)[KC]) {
+            if (t >= w) return;
             T v = (T)0;
 #pragma unroll
             for (int k = 0; k < KC / 2; k++) {
-                const C f = F[k];
-                v += f.x * tg[2 * k];
-                v += f.y * tg[2 * k + 1];
+                v += o.f[k].x * tg[2 * k];
+                v += o.f[k].y * tg[2 * k + 1];
             }
             C e;
             if (BACK) {
@@ -265,44 +281,51 @@ __device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int
             R = chop_R(R);
 #pragma unroll
             for (int i = 0; i < NMU; i++) {
-                if (!BACK) {
+                if (!BACK) {  // acc += x R, four multiply-adds
                     const C x1 = xq[x0 + t * NMU + i], x2 = xmq[x0 + t * NMU + i];
-                    acq[i].x += x1.x * R.x - x1.y * R.y;
-                    acq[i].y += x1.x * R.y + x1.y * R.x;
-                    acm[i].x += x2.x * R.x - x2.y * R.y;
-                    acm[i].y += x2.x * R.y + x2.y * R.x;
-                } else {
-                    const C g1 = s.buf[p][x0 + t * NMU + i], g2 = s.buf[8 + p][x0 + (2 * m - t) * NMU + i];
-                    acq[i].x += g1.x * R.x + g1.y * R.y;  // g conj(R)
-                    acq[i].y += g1.y * R.x - g1.x * R.y;
-                    acm[i].x += g2.x * R.x + g2.y * R.y;
-                    acm[i].y += g2.y * R.x - g2.x * R.y;
+                    acq[i].x = fma(x1.x, R.x, acq[i].x);
+                    acq[i].x = fma(-x1.y, R.y, acq[i].x);
+                    acq[i].y = fma(x1.x, R.y, acq[i].y);
+                    acq[i].y = fma(x1.y, R.x, acq[i].y);
+                    acm[i].x = fma(x2.x, R.x, acm[i].x);
+                    acm[i].x = fma(-x2.y, R.y, acm[i].x);
+                    acm[i].y = fma(x2.x, R.y, acm[i].y);
+                    acm[i].y = fma(x2.y, R.x, acm[i].y);
+                } else {  // acc += g conj(R)
+                    const C g1 = gq[x0 + t * NMU + i], g2 = gmq[x0 + (2 * m - t) * NMU + i];
+                    acq[i].x = fma(g1.x, R.x, acq[i].x);
+                    acq[i].x = fma(g1.y, R.y, acq[i].x);
+                    acq[i].y = fma(g1.y, R.x, acq[i].y);
+                    acq[i].y = fma(-g1.x, R.y, acq[i].y);
+                    acm[i].x = fma(g2.x, R.x, acm[i].x);
+                    acm[i].x = fma(g2.y, R.y, acm[i].x);
+                    acm[i].y = fma(g2.y, R.x, acm[i].y);
+                    acm[i].y = fma(-g2.x, R.y, acm[i].y);
                 }
             }
         };
+        Ops o0, o1;
+        load(0, o0);
         for (int t = 0; t < len; t += 2) {
-            step(t, ta);
-            step(t + 1, tb);
+            load(t + 1, o1);
+            step(t, o0, ta);
+            load(t + 2, o0);
+            step(t + 1, o1, tb);
         }
         if (w > 0) {
             const T sg = (m & 1) ? (T)-1 : (T)1;
 #pragma unroll
             for (int i = 0; i < NMU; i++) {
+                C o;
+                o.x = sg * acm[i].x;
+                o.y = sg * acm[i].y;
                 if (!BACK) {
                     s.buf[p][e1 + i] = acq[i];
-                    C o;
-                    o.x = sg * acm[i].x;
-                    o.y = sg * acm[i].y;
                     s.buf[8 + p][e2 + i] = o;
                 } else {
-                    if (store_q) oq[e1 + i] = acq[i];
-                    if (store_mq) {
-                        C o;
-                        o.x = sg * acm[i].x;
-                        o.y = sg * acm[i].y;
-                        if (conj_mq) o.y = -o.y;
-                        omq[e1 + i] = o;
-                    }
+                    if (conj_mq) o.y = -o.y;
+                    s.xs[p][e1 + i] = acq[i];
+                    s.xs[8 + p][e1 + i] = o;
                 }
             }
         }
@@ -312,13 +335,9 @@ __device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int
 }
 
 template <typename T, int L, int NMU, bool BACK>
-__device__ __forceinline__ void rot_all(const OzmArgs<T> &a, Lds<T, L> &s, int p, int j,
-                                        const typename Ty<T>::c *xq, const typename Ty<T>::c *xmq,
-                                        typename Ty<T>::c *oq, typename Ty<T>::c *omq, bool store_q, bool store_mq,
-                                        bool conj_mq) {
-    rot_class<T, L, NMU, BACK>(a, s, a.cr0[BACK][NMU - 1], a.cnr[BACK][NMU - 1], p, j, xq, xmq, oq, omq, store_q,
-                               store_mq, conj_mq);
-    if constexpr (NMU > 1) rot_all<T, L, NMU - 1, BACK>(a, s, p, j, xq, xmq, oq, omq, store_q, store_mq, conj_mq);
+__device__ __forceinline__ void rot_all(const OzmArgs<T> &a, Lds<T, L> &s, int w, int p, int j) {
+    rot_class<T, L, NMU, BACK>(a, s, a.cr0[BACK][w][NMU - 1], a.cnr[BACK][w][NMU - 1], p, j);
+    if constexpr (NMU > 1) rot_all<T, L, NMU - 1, BACK>(a, s, w, p, j);
 }
 
 // ---- OZ block khi on the matrix cores -------------------------------------------------
@@ -377,6 +396,8 @@ __device__ __forceinline__ void store_all(typename Ty<T>::acc *acc, Lds<T, L> &s
     (gamma_store<T, L, KHI, NB>(acc[NB], s, al, be, l), ...);
 }
 
+// blocks khi = KHI, KHI + kWaves, ... (the blocks of one wave; blocks are independent: block
+// khi reads and writes only the entries (., khi, .) of the rows)
 template <typename T, int L, int KHI>
 __device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr, const T *al, const T *be, int l) {
     using P = Plan<L>;
@@ -384,6 +405,9 @@ __device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr
     constexpr int NKB = P::nkb(KHI), NNB = P::nnb(KHI), F0 = P::foff(KHI);
     const int row16 = l & 15;
     const bool hi = (l >> 5) & 1, pk = (l >> 4) & 1;
+    T b[NKB * NNB];
+#pragma unroll
+    for (int f = 0; f < NKB * NNB; f++) b[f] = fr[(F0 + f) * 64 + l];
     T x[NKB];
     load_x<T, L, KHI>(x, s.buf[row16], s.buf[row16 ^ 8], hi, pk, std::make_integer_sequence<int, NKB>{});
     A acc[NNB];
@@ -392,19 +416,28 @@ __device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr
 #pragma unroll
     for (int kb = 0; kb < NKB; kb++)
 #pragma unroll
-        for (int nb = 0; nb < NNB; nb++) acc[nb] = mfma16(x[kb], fr[(F0 + kb * NNB + nb) * 64 + l], acc[nb]);
+        for (int nb = 0; nb < NNB; nb++) acc[nb] = mfma16(x[kb], b[kb * NNB + nb], acc[nb]);
     // every read of block khi precedes its writes in the wave's LDS order
     store_all<T, L, KHI>(acc, s, al, be, l, std::make_integer_sequence<int, NNB>{});
-    if constexpr (KHI < L) oz_blocks<T, L, KHI + 1>(s, fr, al, be, l);
+    if constexpr (KHI + kWaves <= L) oz_blocks<T, L, KHI + kWaves>(s, fr, al, be, l);
+}
+
+template <typename T, int L, int W0>
+__device__ __forceinline__ void oz_wave(Lds<T, L> &s, const T *fr, const T *al, const T *be, int w, int l) {
+    if constexpr (W0 < kWaves && -L + W0 <= L) {
+        if (w == W0) oz_blocks<T, L, -L + W0>(s, fr, al, be, l);
+        else oz_wave<T, L, W0 + 1>(s, fr, al, be, w, l);
+    }
 }
 
 template <typename T, int L>
-__global__ __launch_bounds__(64) void ozm(typename Ty<T>::c *__restrict__ proj, OzmArgs<T> a) {
+__global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict__ proj, OzmArgs<T> a) {
     using P = Plan<L>;
     using C = typename Ty<T>::c;
-    constexpr int KP = P::kp;
-    __shared__ Lds<T, L> s;
-    const int l = threadIdx.x;
+    constexpr int KP = P::kp, NP = P::np;
+    extern __shared__ __align__(16) unsigned char smem[];
+    Lds<T, L> &s = *reinterpret_cast<Lds<T, L> *>(smem);
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
     const int p = l & 7, j = l >> 3;
     // XCD-aware order: consecutive workgroups go to different XCDs, so each XCD takes a
     // contiguous range of tiles (tiles of one interval share its c fragments in that L2)
@@ -413,10 +446,13 @@ __global__ __launch_bounds__(64) void ozm(typename Ty<T>::c *__restrict__ proj, 
     if (tile >= a.ntiles) return;
     const int ts = a.tstart[tile], ti = a.tinfo[tile];
     const int cnt = ti & 15, iq = ti >> 4;
-    const bool valid = p < cnt;
-    const int pr = ts + (valid ? p : 0);
-    const int gq = a.pq[pr], gmq = a.pmq[pr], flag = a.pflag[pr];
-    if (j == 0) {
+    if (tid < 8) {
+        const bool valid = tid < cnt;
+        const int pr = ts + (valid ? tid : 0);
+        const int gq = a.pq[pr], gmq = a.pmq[pr];
+        s.g[tid] = gq;
+        s.g[8 + tid] = gmq;
+        s.flag[tid] = a.pflag[pr] | (valid ? 0 : 4);
         const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
         const double q0 = a.kx[ix], q1 = a.ky[iy], q2 = a.kz[iz];
         const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2), rxy = sqrt(q0 * q0 + q1 * q1);
@@ -424,19 +460,19 @@ __global__ __launch_bounds__(64) void ozm(typename Ty<T>::c *__restrict__ proj, 
         C abv;
         abv.x = (T)al;
         abv.y = (T)(1.0 - al);
-        s.ab[p] = abv;
+        s.ab[tid] = abv;
         // frame angles (q along -z: theta = pi, phi = pi; q = 0: the z frame)
         const double ctt = qn > 0.0 ? q2 / qn : 1.0, stt = qn > 0.0 ? rxy / qn : 0.0;
         const double cp = rxy > 0.0 ? q0 / rxy : (q2 < 0.0 ? -1.0 : 1.0);
         const double sp = rxy > 0.0 ? q1 / rxy : 0.0;
         double ck = 1.0, sk = 0.0, cpk = 1.0, spk = 0.0;
         for (int k = 0; k < KP; k++) {
-            s.trig[p][0][k] = (T)ck;
-            s.trig[p][1][k] = (T)sk;
+            s.trig[tid][0][k] = (T)ck;
+            s.trig[tid][1][k] = (T)sk;
             C e;
             e.x = (T)cpk;
             e.y = (T)spk;
-            s.ph[p][k] = e;
+            s.ph[tid][k] = e;
             const double nc = ck * ctt - sk * stt;
             sk = sk * ctt + ck * stt;
             ck = nc;
@@ -445,11 +481,15 @@ __global__ __launch_bounds__(64) void ozm(typename Ty<T>::c *__restrict__ proj, 
             cpk = nq;
         }
     }
-    wave_sync();
-    const int np = P::np;
-    const C *xq = proj + (size_t)gq * np, *xmq = proj + (size_t)gmq * np;
-    rot_all<T, L, P::nclass, false>(a, s, p, j, xq, xmq, nullptr, nullptr, false, false, false);
-    wave_sync();
+    __syncthreads();
+    // X of the 16 (side, pair) rows, each row one contiguous record
+    for (int i = tid; i < 16 * NP; i += 64 * kWaves) {
+        const int r = i / NP, e = i - r * NP;
+        s.xs[r][e] = proj[(size_t)s.g[r] * NP + e];
+    }
+    __syncthreads();
+    rot_all<T, L, P::nclass, false>(a, s, w, p, j);
+    __syncthreads();
     T al[4], be[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -457,12 +497,17 @@ __global__ __launch_bounds__(64) void ozm(typename Ty<T>::c *__restrict__ proj, 
         al[i] = v.x;
         be[i] = v.y;
     }
-    const T *fr = a.frag + (size_t)(iq - 1) * P::nfrag * 64;
-    oz_blocks<T, L, -L>(s, fr, al, be, l);
-    wave_sync();
-    const bool eq = flag & 1, nyq = flag & 2;
-    C *oq = proj + (size_t)gq * np, *omq = proj + (size_t)gmq * np;
-    rot_all<T, L, P::nclass, true>(a, s, p, j, nullptr, nullptr, oq, omq, valid && !eq, valid, eq && nyq);
+    oz_wave<T, L, 0>(s, a.frag + (size_t)(iq - 1) * P::nfrag * 64, al, be, w, l);
+    __syncthreads();
+    rot_all<T, L, P::nclass, true>(a, s, w, p, j);
+    __syncthreads();
+    // store rule (:648-656): both sides of real pairs; for q == -q only the -q side
+    for (int i = tid; i < 16 * NP; i += 64 * kWaves) {
+        const int r = i / NP, e = i - r * NP;
+        const int f = s.flag[r & 7];
+        if ((f & 4) || (r < 8 && (f & 1))) continue;
+        proj[(size_t)s.g[r] * NP + e] = s.xs[r][e];
+    }
 }
 
 // ---- host side ---------------------------------------------------------------------
@@ -539,12 +584,15 @@ template <typename T, int L> hipError_t launch_ozm_t(mdft_ctx *c, typename Ty<T>
     a.units = reinterpret_cast<const int4 *>(c->d_ozm_units);
     a.rlen = c->d_ozm_rlen;
     for (int d = 0; d < 2; d++)
-        for (int k = 0; k < 4; k++) {
-            a.cr0[d][k] = c->ozm_cr0[d][k];
-            a.cnr[d][k] = c->ozm_cnr[d][k];
-        }
+        for (int wv = 0; wv < kWaves; wv++)
+            for (int k = 0; k < 4; k++) {
+                a.cr0[d][wv][k] = c->ozm_cr0[d][wv][k];
+                a.cnr[d][wv][k] = c->ozm_cnr[d][wv][k];
+            }
     if (a.ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((ozm<T, L>), dim3((unsigned)(8 * a.per)), dim3(64), 0, c->stream, proj, a);
+    const size_t lds = sizeof(Lds<T, L>);
+    if (lds > c->lds_limit) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((ozm<T, L>), dim3((unsigned)(8 * a.per)), dim3(64 * kWaves), lds, c->stream, proj, a);
     return hipGetLastError();
 }
 
@@ -573,8 +621,21 @@ template <typename T> int upload_t(mdft_ctx *c, void **dst, const std::vector<do
 
 }  // namespace
 
+static size_t ozm_lds(int L, bool fp64) {
+    switch (L) {
+        case 0: return fp64 ? sizeof(Lds<double, 0>) : sizeof(Lds<float, 0>);
+        case 1: return fp64 ? sizeof(Lds<double, 1>) : sizeof(Lds<float, 1>);
+        case 2: return fp64 ? sizeof(Lds<double, 2>) : sizeof(Lds<float, 2>);
+        case 3: return fp64 ? sizeof(Lds<double, 3>) : sizeof(Lds<float, 3>);
+        case 4: return fp64 ? sizeof(Lds<double, 4>) : sizeof(Lds<float, 4>);
+        case 5: return fp64 ? sizeof(Lds<double, 5>) : sizeof(Lds<float, 5>);
+        default: return fp64 ? sizeof(Lds<double, 6>) : sizeof(Lds<float, 6>);
+    }
+}
+
 bool ozm_supported(const mdft_ctx *c) {
-    return c->d.mrso == 2 && c->L >= 0 && c->L <= 6;
+    return c->d.mrso == 2 && c->L >= 0 && c->L <= 6 &&
+           ozm_lds(c->L, c->d.precision == MDFT_PREC_FP64) <= c->lds_limit;
 }
 
 // Fourier coefficients of d^m(theta) per entry (m, a, b) (kp per entry, zero beyond m) and
@@ -614,18 +675,20 @@ int ozm_setup(mdft_ctx *c) {
     std::vector<int> poff(L + 2, 0);
     for (int m = 0; m <= L; m++) poff[m + 1] = poff[m] + (2 * m + 1) * nmu(m);
     auto pidx = [&](int m, int mup, int mu2) { return poff[m] + (mup + m) * nmu(m) + mu2; };
-    std::vector<int> units, rlen;
+    // rounds of 8 units (m, khi) (rotation in) or (m, mup) (rotation back) of one class
+    // m/2 + 1 = k, m descending; a round runs as long as its longest unit (2 m + 1)
+    struct Round {
+        int back, k, len;
+        int4 u[8];
+    };
+    std::vector<Round> rounds;
     for (int back = 0; back < 2; back++)
-        for (int k = 1; k <= 4; k++) {
-            c->ozm_cr0[back][k - 1] = (int)rlen.size();
-            c->ozm_cnr[back][k - 1] = 0;
-            if (k > h.nclass) continue;
-            std::vector<std::pair<int, int>> us;  // (m, khi or mup), m descending
+        for (int k = 1; k <= h.nclass; k++) {
+            std::vector<std::pair<int, int>> us;
             for (int m = std::min(L, 2 * k - 1); m >= 2 * k - 2; m--)
                 for (int x = -m; x <= m; x++) us.push_back({m, x});
             for (size_t u0 = 0; u0 < us.size(); u0 += 8) {
-                rlen.push_back(2 * us[u0].first + 1);
-                c->ozm_cnr[back][k - 1]++;
+                Round r{back, k, 2 * us[u0].first + 1, {}};
                 for (int jj = 0; jj < 8; jj++) {
                     int4 e{0, 0, 0, 64};
                     if (u0 + jj < us.size()) {
@@ -633,21 +696,53 @@ int ozm_setup(mdft_ctx *c) {
                         if (!back) {  // rotation in, unit (m, khi = x): iterate mup
                             e.x = poff[m] | pidx(m, x, 0) << 16;
                             e.y = pidx(m, -x, 0) | ((roff(m) + (x + m)) * kp) << 16;
-                            e.z = m | W << 8 | ((m + x) & 1) << 16;
                         } else {  // rotation back, unit (m, mup = x): iterate khi
                             e.x = poff[m] | pidx(m, x, 0) << 16;
-                            e.y = 0 | ((roff(m) + (x + m) * W) * kp) << 16;
-                            e.z = m | W << 8 | ((m + x) & 1) << 16;
+                            e.y = ((roff(m) + (x + m) * W) * kp) << 16;
                         }
+                        e.z = m | W << 8 | ((m + x) & 1) << 16;
                         e.w = x + 64;
                     }
-                    units.push_back(e.x);
-                    units.push_back(e.y);
-                    units.push_back(e.z);
-                    units.push_back(e.w);
+                    r.u[jj] = e;
                 }
+                rounds.push_back(r);
             }
         }
+    // rounds to waves, longest first onto the least loaded wave (cost ~ iterations x (m/2 + 1))
+    std::vector<int> owner(rounds.size());
+    for (int back = 0; back < 2; back++) {
+        std::vector<int> idx;
+        for (int i = 0; i < (int)rounds.size(); i++)
+            if (rounds[i].back == back) idx.push_back(i);
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+            return rounds[x].len * rounds[x].k > rounds[y].len * rounds[y].k;
+        });
+        std::vector<int> load(kWaves, 0);
+        for (int i : idx) {
+            const int wv = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            owner[i] = wv;
+            load[wv] += rounds[i].len * rounds[i].k;
+        }
+    }
+    std::vector<int> units, rlen;
+    for (int back = 0; back < 2; back++)
+        for (int wv = 0; wv < kWaves; wv++)
+            for (int k = 1; k <= 4; k++) {
+                c->ozm_cr0[back][wv][k - 1] = (int)rlen.size();
+                c->ozm_cnr[back][wv][k - 1] = 0;
+                for (size_t i = 0; i < rounds.size(); i++) {
+                    const Round &r = rounds[i];
+                    if (r.back != back || r.k != k || owner[i] != wv) continue;
+                    rlen.push_back(r.len);
+                    c->ozm_cnr[back][wv][k - 1]++;
+                    for (int jj = 0; jj < 8; jj++) {
+                        units.push_back(r.u[jj].x);
+                        units.push_back(r.u[jj].y);
+                        units.push_back(r.u[jj].z);
+                        units.push_back(r.u[jj].w);
+                    }
+                }
+            }
     int rc = c->d.precision == MDFT_PREC_FP64 ? upload_t<double>(c, &c->d_ozm_ftab, ftab)
                                                 : upload_t<float>(c, &c->d_ozm_ftab, ftab);
     if (rc != MDFT_OK) return rc;

@@ -89,7 +89,7 @@ struct mdft_ctx {
     void *d_ozm_ftab = nullptr;   // its Fourier coefficients of d^m(theta)
     int *d_ozm_units = nullptr;   // its rotation unit rounds [round][8] int4
     int *d_ozm_rlen = nullptr;    // iterations of each round
-    int ozm_cr0[2][4] = {}, ozm_cnr[2][4] = {};  // first round and count per (direction, class)
+    int ozm_cr0[2][4][4] = {}, ozm_cnr[2][4][4] = {};  // first round and count per (direction, wave, class)
     std::vector<int> ia0_h;       // first OZ coefficient of each projection row (host)
     void *d_bf16_M = nullptr;     // angl2proj matrix, BF16 MFMA fragment order
     void *d_bf16_Q = nullptr;     // proj2angl matrix, BF16 MFMA fragment order
