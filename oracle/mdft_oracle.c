@@ -788,10 +788,14 @@ void or_excess(const or_ctx *c, const double *xi, double *fexc, double *df)
     or_excess_stage(c, xi, 4, proj);
     const double fourpisq = 4.0 * acos(-1.0) * acos(-1.0);
     const double prefactor = -c->kT * 2.0 * fourpisq / c->mrso / c->n0; /* :728 */
-    double ff = 0.0;
-#pragma omp parallel reduction(+ : ff)
+    /* per-thread running sums (static schedule), added in thread order: the same result on
+       every call with the same thread count */
+    const int nt = omp_get_max_threads();
+    double *part = calloc((size_t)nt, sizeof(double));
+#pragma omp parallel
     {
         double *v = malloc(sizeof(double) * no);
+        double ffp = 0.0;
 #pragma omp for schedule(static)
         for (size_t r = 0; r < N; r++) {
             or_proj2angl(c, proj + r * np, v);
@@ -802,10 +806,14 @@ void or_excess(const or_ctx *c, const double *xi, double *fexc, double *df)
                 s += (x * x * c->rho0 - c->rho0) * vexc;
                 if (df) df[r * no + io] += 2.0 * c->rho0 * x * vexc;
             }
-            ff += s;
+            ffp += s;
         }
+        part[omp_get_thread_num()] = ffp;
         free(v);
     }
+    double ff = 0.0;
+    for (int t = 0; t < nt; t++) ff += part[t];
+    free(part);
     *fexc = ff * 0.5 * c->dv;
     free(proj);
 }
@@ -817,21 +825,36 @@ void or_ideal_ext(const or_ctx *c, const double *xi, const double *vext, double 
     const size_t N = (size_t)c->nx * c->ny * c->nz;
     const int no = c->no;
     const double kT = c->kT, dv = c->dv, rho0 = c->rho0;
-    double a = 0.0, b = 0.0;
-#pragma omp parallel for reduction(+ : a, b) schedule(static)
-    for (size_t r = 0; r < N; r++)
-        for (int io = 0; io < no; io++) {
-            double x = xi[r * no + io];
-            double rho = x * x * rho0;
-            if (rho > DBL_EPSILON) {
-                a += kT * dv * c->w[io] * (rho * log(rho / rho0) - rho + rho0);
-                b += dv * c->w[io] * rho * vext[r * no + io];
-                if (df)
-                    df[r * no + io] += c->w[io] * 2.0 * rho0 * x * (kT * log(rho / rho0) + vext[r * no + io]);
-            } else {
-                a += kT * dv * c->w[io] * rho0;
+    /* per-thread running sums (static schedule), added in thread order: the same result on
+       every call with the same thread count */
+    const int nt = omp_get_max_threads();
+    double *pa = calloc(2 * (size_t)nt, sizeof(double)), *pb = pa + nt;
+#pragma omp parallel
+    {
+        double a = 0.0, b = 0.0;
+#pragma omp for schedule(static)
+        for (size_t r = 0; r < N; r++)
+            for (int io = 0; io < no; io++) {
+                double x = xi[r * no + io];
+                double rho = x * x * rho0;
+                if (rho > DBL_EPSILON) {
+                    a += kT * dv * c->w[io] * (rho * log(rho / rho0) - rho + rho0);
+                    b += dv * c->w[io] * rho * vext[r * no + io];
+                    if (df)
+                        df[r * no + io] += c->w[io] * 2.0 * rho0 * x * (kT * log(rho / rho0) + vext[r * no + io]);
+                } else {
+                    a += kT * dv * c->w[io] * rho0;
+                }
             }
-        }
+        pa[omp_get_thread_num()] = a;
+        pb[omp_get_thread_num()] = b;
+    }
+    double a = 0.0, b = 0.0;
+    for (int t = 0; t < nt; t++) {
+        a += pa[t];
+        b += pb[t];
+    }
+    free(pa);
     *fid = a;
     *fext = b;
 }
