@@ -158,7 +158,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="solute20_128")
     ap.add_argument("--gtol", type=float, default=1e-6)
-    ap.add_argument("--memory", type=int, default=1, help="L-BFGS memory m (both paths; 1 is the reference's)")
+    ap.add_argument("--memory", type=int, default=1, help="L-BFGS memory m of the plain path (1 is the reference's)")
+    ap.add_argument("--fast-memory", type=int, default=None,
+                    help="L-BFGS memory m of the fast path (default: --memory)")
     ap.add_argument("--fp64-vectors", action="store_true",
                     help="fast path: keep the minimiser's directions and gradients in FP64 (default FP32)")
     ap.add_argument("--fp64-iterate", action="store_true",
@@ -300,6 +302,8 @@ def main():
         net = wsm.WarmStartNet.load(args.weights) if os.path.exists(args.weights) else None
         info = {}
 
+        fast_m = args.fast_memory if args.fast_memory is not None else args.memory
+
         def fast(start, stop="ref"):
             e32.synchronize()
             eng.synchronize()
@@ -313,7 +317,7 @@ def main():
             else:
                 wsm.cold_start(e32, xi)
             kw = dict(ftol=ref_tol, ftol_absolute=True) if stop == "ref" else dict(ftarget=dg_ref, fband=band)
-            r = minimize(e32, xi, g=df, m=args.memory, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6,
+            r = minimize(e32, xi, g=df, m=fast_m, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6,
                          fp32_vectors=not args.fp64_vectors, scaled=not args.unscaled,
                          fp32_iterate=not (args.fp64_vectors or args.fp64_iterate),
                          step0=0.0 if (args.unit_first_step or args.unscaled) else "newton", **kw)
@@ -360,8 +364,8 @@ def main():
         nvec = eng.N * eng.no
         vb = 4 if not args.fp64_vectors else 8
         xb = 4 if not (args.fp64_vectors or args.fp64_iterate) else 8   # iterate and trial points
-        dot_bytes = 2 * args.memory * vb if (vb == 4 and args.memory == 1) else (2 * args.memory + 1) * vb
-        per_elem = (2 * args.memory + 1) * vb + vb + 2 * xb + dot_bytes
+        dot_bytes = 2 * fast_m * vb if (vb == 4 and fast_m == 1) else (2 * fast_m + 1) * vb
+        per_elem = (2 * fast_m + 1) * vb + vb + 2 * xb + dot_bytes
         # one FP32 evaluation: xi read twice (xb bytes), vext once (8), dF written (vb), projections
         eval_bytes = eng.N * eng.no * (2 * xb + 8 + vb) + eng.N * 8 * eng.np_ * 8
         t_roof = (res_f.evaluations * eval_bytes + res_f.iterations * per_elem * nvec) / (HBM_PEAK_GBS * 1e9)
@@ -414,7 +418,7 @@ def main():
             "F_parts": [float(v) for v in res.parts],
             "trace_fast": [[float(v) for v in row] for row in res_f.trace],
             "trace_plain": [[float(v) for v in row] for row in res_p.trace],
-            "warm_start": info, "lbfgs_memory": args.memory,
+            "warm_start": info, "lbfgs_memory": args.memory, "lbfgs_memory_fast": fast_m,
         })
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = cpu_baseline(w, res_p.evaluations)

@@ -134,6 +134,7 @@ void free_pairs(PairSchedule &p) {
     (void)hipFree(p.ciq);
     (void)hipFree(p.tstart);
     (void)hipFree(p.tinfo);
+    (void)hipFree(p.tpair);
     p = PairSchedule{};
 }
 
@@ -199,14 +200,37 @@ int build_pairs(mdft_ctx *c) {
     HIPCHK(c, upload(c, &p.ciq, ciq.data(), ciq.size()));
     // tiles of at most 8 pairs of one chunk (one wave each in the matrix-core kernel)
     std::vector<int> tstart, tinfo;
+    std::vector<TilePair> tpair;
     for (size_t k = 0; k < cstart.size(); k++)
         for (int o = 0; o < ccount[k]; o += 8) {
+            const int n8 = std::min(8, ccount[k] - o);
             tstart.push_back(cstart[k] + o);
-            tinfo.push_back(std::min(8, ccount[k] - o) | ciq[k] << 4);
+            tinfo.push_back(n8 | ciq[k] << 4);
+            for (int j = 0; j < 8; j++) {
+                // padding entries repeat the tile's first pair (valid memory, never stored)
+                const int i = cstart[k] + o + (j < n8 ? j : 0);
+                TilePair t{};
+                t.gq = spq[i];
+                t.gmq = spmq[i];
+                t.flag = sflag[i] | (j < n8 ? 0 : 4);
+                t.iq = ciq[k];
+                const int ix = t.gq % nx, iy = (t.gq / nx) % ny, iz = t.gq / (nx * ny);
+                const double q0 = c->kx[ix], q1 = c->ky[iy], q2 = c->kz[iz];
+                const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2), rxy = sqrt(q0 * q0 + q1 * q1);
+                t.al = qn / c->dq + 1.0 - (double)ciq[k];
+                // frame of q: theta from z, phi in the xy plane; q along -z: theta = pi, phi = pi;
+                // q = 0: the z frame
+                t.ct = qn > 0.0 ? q2 / qn : 1.0;
+                t.st = qn > 0.0 ? rxy / qn : 0.0;
+                t.cp = rxy > 0.0 ? q0 / rxy : (q2 < 0.0 ? -1.0 : 1.0);
+                t.sp = rxy > 0.0 ? q1 / rxy : 0.0;
+                tpair.push_back(t);
+            }
         }
     p.ntiles = (int)tstart.size();
     HIPCHK(c, upload(c, &p.tstart, tstart.data(), tstart.size()));
     HIPCHK(c, upload(c, &p.tinfo, tinfo.data(), tinfo.size()));
+    HIPCHK(c, upload(c, &p.tpair, tpair.data(), tpair.size()));
     return MDFT_OK;
 }
 

@@ -75,18 +75,6 @@ template <typename T> __device__ __forceinline__ constexpr int drow(int g, int r
 template <> __device__ __forceinline__ constexpr int drow<float>(int g, int r) { return 4 * g + r; }
 template <> __device__ __forceinline__ constexpr int drow<double>(int g, int r) { return 4 * r + g; }
 
-// value of lane (l ^ 1) (CTRL = kSwap1) or (l ^ 2) (kSwap2) of the same quad (DPP quad_perm)
-constexpr int kSwap1 = 0xB1, kSwap2 = 0x4E;
-template <int CTRL> __device__ __forceinline__ float quad_swap(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
-}
-template <int CTRL> __device__ __forceinline__ double quad_swap(double v) {
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -186,11 +174,9 @@ template <int L> struct Plan {
 constexpr int kWaves = 4;  // waves per tile
 
 template <typename T> struct OzmArgs {
-    int nx, ny, np, ntiles, per;
-    double dq;
-    const double *kx, *ky, *kz;
-    const int *pq, *pmq, *pflag;
-    const int *tstart, *tinfo;  // tile: first pair; pair count | iq << 4
+    int ntiles, per;
+    const TilePair *tpair;      // [tile][8] the tiles' pairs
+    const int *pmup;            // mup of each projection
     const T *frag;              // [interval][nfrag][64] B fragments of (c[iq], c[iq+1])
     const T *ftab;              // [entry][kp] Fourier coefficients of d^m(theta)
     const int4 *units;          // [round][8] rotation units
@@ -198,11 +184,22 @@ template <typename T> struct OzmArgs {
     int cr0[2][kWaves][4], cnr[2][kWaves][4];  // first round and count per (direction, wave, class)
 };
 
-// LDS of one tile: X (then the rotated-back outputs) and A (then gamma) per (side, pair)
-// row, the pairs' angles, interpolation weights and grid indices
+// transpose scratch of the OZ epilogue: per wave a 16 x 16 block of the products, rows
+// padded so that a lane's four values are one aligned 16-byte-multiple read
+template <typename T> constexpr int scr_stride() { return sizeof(T) == 8 ? 18 : 20; }
+
+// LDS of one tile: X (then the rotated-back outputs; during the OZ step the transpose
+// scratch) and A (then gamma) per (side, pair) row, the pairs' angles, interpolation
+// weights and grid indices
 template <typename T, int L> struct Lds {
     using C = typename Ty<T>::c;
-    C xs[16][Plan<L>::nps];
+    static constexpr int kScr = kWaves * 16 * scr_stride<T>();  // scratch values (T)
+    static constexpr int xrows() {
+        const int rowb = Plan<L>::nps * (int)sizeof(C), need = (kScr * (int)sizeof(T) + rowb - 1) / rowb;
+        return need > 16 ? need : 16;
+    }
+    static constexpr int kXRows = xrows();
+    C xs[kXRows][Plan<L>::nps];
     C buf[16][Plan<L>::nps];
     T trig[8][2][Plan<L>::kp];  // cos k theta, sin k theta of each pair
     C ph[8][Plan<L>::kp];       // exp(+i k phi) of each pair
@@ -219,6 +216,9 @@ __device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int
     using C = typename Ty<T>::c;
     constexpr int KP = Plan<L>::kp;
     constexpr int KC = (2 * NMU < KP) ? 2 * NMU : KP;  // coefficients that can be nonzero for m <= 2 NMU - 1
+    // FP32 (no chop to honour): R = exp(-i mup phi) d factored, X carries the phase from its
+    // load into LDS, the rotation back applies it once per output; real d products in between
+    constexpr bool FACT = sizeof(T) == 4;
     const T *ct = s.trig[p][0], *st = s.trig[p][1];
     const C *xq = s.xs[p], *xmq = s.xs[8 + p];
     const C *gq = s.buf[p], *gmq = s.buf[8 + p];
@@ -265,6 +265,18 @@ __device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int
             for (int k = 0; k < KC / 2; k++) {
                 v += o.f[k].x * tg[2 * k];
                 v += o.f[k].y * tg[2 * k + 1];
+            }
+            if constexpr (FACT) {
+#pragma unroll
+                for (int i = 0; i < NMU; i++) {
+                    const C y1 = BACK ? gq[x0 + t * NMU + i] : xq[x0 + t * NMU + i];
+                    const C y2 = BACK ? gmq[x0 + (2 * m - t) * NMU + i] : xmq[x0 + t * NMU + i];
+                    acq[i].x = fma(y1.x, v, acq[i].x);
+                    acq[i].y = fma(y1.y, v, acq[i].y);
+                    acm[i].x = fma(y2.x, v, acm[i].x);
+                    acm[i].y = fma(y2.y, v, acm[i].y);
+                }
+                return;
             }
             C e;
             if (BACK) {
@@ -314,6 +326,16 @@ __device__ __forceinline__ void rot_class(const OzmArgs<T> &a, Lds<T, L> &s, int
         }
         if (w > 0) {
             const T sg = (m & 1) ? (T)-1 : (T)1;
+            if (FACT && BACK) {  // exp(+i mup phi) = conj(eph)
+#pragma unroll
+                for (int i = 0; i < NMU; i++) {
+                    const C u = acq[i], v = acm[i];
+                    acq[i].x = u.x * eph.x + u.y * eph.y;
+                    acq[i].y = u.y * eph.x - u.x * eph.y;
+                    acm[i].x = v.x * eph.x + v.y * eph.y;
+                    acm[i].y = v.y * eph.x - v.x * eph.y;
+                }
+            }
 #pragma unroll
             for (int i = 0; i < NMU; i++) {
                 C o;
@@ -358,48 +380,45 @@ __device__ __forceinline__ void load_x(T *x, const typename Ty<T>::c *own, const
     ((x[KB] = xfrag<T, L, KHI, KB>(own, oth, hi, pk)), ...);
 }
 
-// D (16 x 16): lane l holds rows drow(l >> 4, i), i = 0..3, column l & 15 = 4 rr + q of the
-// n-block: OZ row 4 nb + rr, q = re lo, im lo, re hi, im hi.  gamma = (1 - alpha) lo + alpha hi
-// goes to row (side, pair) = drow(l >> 4, q), the other three values of the lane serving its quad.
+// D (16 x 16) of n-block NB: lane l holds rows drow(l >> 4, i), i = 0..3, column l & 15 =
+// 4 rr + q (OZ row 4 NB + rr; q = re lo, im lo, re hi, im hi).  Through the wave's scratch
+// lane l then takes (side, pair) row ps = l & 15 and OZ row 4 NB + (l >> 4):
+// gamma = (1 - alpha) lo + alpha hi, chopped, into the row's entry of that OZ row.
 template <typename T, int L, int KHI, int NB>
-__device__ __forceinline__ void gamma_store(typename Ty<T>::acc d, Lds<T, L> &s, const T *al, const T *be, int l) {
+__device__ __forceinline__ void gamma_store(typename Ty<T>::acc d, Lds<T, L> &s, T *scr, T al, T be, int l) {
     using C = typename Ty<T>::c;
+    constexpr int SS = scr_stride<T>();
     constexpr int r0 = Plan<L>::row(KHI, 4 * NB), r1 = Plan<L>::row(KHI, 4 * NB + 1);
     constexpr int r2 = Plan<L>::row(KHI, 4 * NB + 2), r3 = Plan<L>::row(KHI, 4 * NB + 3);
-    const int q = l & 3, rr = (l >> 2) & 3;
-    const bool lo_here = q < 2;
-    T comp[4], oth[4];
+    const int g = l >> 4, n = l & 15;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const T v = d[i], pv = quad_swap<kSwap2>(v);
-        const T lo = lo_here ? v : pv, hi = lo_here ? pv : v;
-        comp[i] = be[i] * lo + al[i] * hi;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) oth[i] = quad_swap<kSwap1>(comp[i]);
-    const T cs = q == 0 ? comp[0] : (q == 1 ? comp[1] : (q == 2 ? comp[2] : comp[3]));
-    const T os = q == 0 ? oth[0] : (q == 1 ? oth[1] : (q == 2 ? oth[2] : oth[3]));
-    T re = (q & 1) ? os : cs, im = (q & 1) ? cs : os;
+    for (int i = 0; i < 4; i++) scr[drow<T>(g, i) * SS + n] = d[i];
+    wave_sync();
+    const int ps = l & 15, rr = l >> 4;
+    const C *v = reinterpret_cast<const C *>(scr + ps * SS + 4 * rr);
+    const C lo = v[0], hi = v[1];
+    T re = be * lo.x + al * hi.x, im = be * lo.y + al * hi.y;
     chop_gamma(re, im);
     const int row = rr == 0 ? r0 : (rr == 1 ? r1 : (rr == 2 ? r2 : r3));
     if (row >= 0) {
-        C g;
-        g.x = re;
-        g.y = im;
-        s.buf[drow<T>(l >> 4, q)][row] = g;
+        C o;
+        o.x = re;
+        o.y = im;
+        s.buf[ps][row] = o;
     }
+    wave_sync();  // the scratch is read before the next block writes it
 }
 
 template <typename T, int L, int KHI, int... NB>
-__device__ __forceinline__ void store_all(typename Ty<T>::acc *acc, Lds<T, L> &s, const T *al, const T *be, int l,
+__device__ __forceinline__ void store_all(typename Ty<T>::acc *acc, Lds<T, L> &s, T *scr, T al, T be, int l,
                                           std::integer_sequence<int, NB...>) {
-    (gamma_store<T, L, KHI, NB>(acc[NB], s, al, be, l), ...);
+    (gamma_store<T, L, KHI, NB>(acc[NB], s, scr, al, be, l), ...);
 }
 
 // blocks khi = KHI, KHI + kWaves, ... (the blocks of one wave; blocks are independent: block
 // khi reads and writes only the entries (., khi, .) of the rows)
 template <typename T, int L, int KHI>
-__device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr, const T *al, const T *be, int l) {
+__device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr, T *scr, T al, T be, int l) {
     using P = Plan<L>;
     using A = typename Ty<T>::acc;
     constexpr int NKB = P::nkb(KHI), NNB = P::nnb(KHI), F0 = P::foff(KHI);
@@ -418,15 +437,15 @@ __device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr
 #pragma unroll
         for (int nb = 0; nb < NNB; nb++) acc[nb] = mfma16(x[kb], b[kb * NNB + nb], acc[nb]);
     // every read of block khi precedes its writes in the wave's LDS order
-    store_all<T, L, KHI>(acc, s, al, be, l, std::make_integer_sequence<int, NNB>{});
-    if constexpr (KHI + kWaves <= L) oz_blocks<T, L, KHI + kWaves>(s, fr, al, be, l);
+    store_all<T, L, KHI>(acc, s, scr, al, be, l, std::make_integer_sequence<int, NNB>{});
+    if constexpr (KHI + kWaves <= L) oz_blocks<T, L, KHI + kWaves>(s, fr, scr, al, be, l);
 }
 
 template <typename T, int L, int W0>
-__device__ __forceinline__ void oz_wave(Lds<T, L> &s, const T *fr, const T *al, const T *be, int w, int l) {
+__device__ __forceinline__ void oz_wave(Lds<T, L> &s, const T *fr, T *scr, T al, T be, int w, int l) {
     if constexpr (W0 < kWaves && -L + W0 <= L) {
-        if (w == W0) oz_blocks<T, L, -L + W0>(s, fr, al, be, l);
-        else oz_wave<T, L, W0 + 1>(s, fr, al, be, w, l);
+        if (w == W0) oz_blocks<T, L, -L + W0>(s, fr, scr, al, be, l);
+        else oz_wave<T, L, W0 + 1>(s, fr, scr, al, be, w, l);
     }
 }
 
@@ -444,27 +463,16 @@ __global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict
     const int b = blockIdx.x;
     const int tile = (b & 7) * a.per + (b >> 3);
     if (tile >= a.ntiles) return;
-    const int ts = a.tstart[tile], ti = a.tinfo[tile];
-    const int cnt = ti & 15, iq = ti >> 4;
+    const int iq = a.tpair[(size_t)tile * 8].iq;  // the same for the tile's pairs
     if (tid < 8) {
-        const bool valid = tid < cnt;
-        const int pr = ts + (valid ? tid : 0);
-        const int gq = a.pq[pr], gmq = a.pmq[pr];
-        s.g[tid] = gq;
-        s.g[8 + tid] = gmq;
-        s.flag[tid] = a.pflag[pr] | (valid ? 0 : 4);
-        const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
-        const double q0 = a.kx[ix], q1 = a.ky[iy], q2 = a.kz[iz];
-        const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2), rxy = sqrt(q0 * q0 + q1 * q1);
-        const double al = qn / a.dq + 1.0 - (double)iq;  // relative to the tile's row, as on the host
+        const TilePair tp = a.tpair[(size_t)tile * 8 + tid];
+        s.g[tid] = tp.gq;
+        s.g[8 + tid] = tp.gmq;
+        s.flag[tid] = tp.flag;
         C abv;
-        abv.x = (T)al;
-        abv.y = (T)(1.0 - al);
+        abv.x = (T)tp.al;
+        abv.y = (T)(1.0 - tp.al);
         s.ab[tid] = abv;
-        // frame angles (q along -z: theta = pi, phi = pi; q = 0: the z frame)
-        const double ctt = qn > 0.0 ? q2 / qn : 1.0, stt = qn > 0.0 ? rxy / qn : 0.0;
-        const double cp = rxy > 0.0 ? q0 / rxy : (q2 < 0.0 ? -1.0 : 1.0);
-        const double sp = rxy > 0.0 ? q1 / rxy : 0.0;
         double ck = 1.0, sk = 0.0, cpk = 1.0, spk = 0.0;
         for (int k = 0; k < KP; k++) {
             s.trig[tid][0][k] = (T)ck;
@@ -473,11 +481,11 @@ __global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict
             e.x = (T)cpk;
             e.y = (T)spk;
             s.ph[tid][k] = e;
-            const double nc = ck * ctt - sk * stt;
-            sk = sk * ctt + ck * stt;
+            const double nc = ck * tp.ct - sk * tp.st;
+            sk = sk * tp.ct + ck * tp.st;
             ck = nc;
-            const double nq = cpk * cp - spk * sp;
-            spk = spk * cp + cpk * sp;
+            const double nq = cpk * tp.cp - spk * tp.sp;
+            spk = spk * tp.cp + cpk * tp.sp;
             cpk = nq;
         }
     }
@@ -485,19 +493,25 @@ __global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict
     // X of the 16 (side, pair) rows, each row one contiguous record
     for (int i = tid; i < 16 * NP; i += 64 * kWaves) {
         const int r = i / NP, e = i - r * NP;
-        s.xs[r][e] = proj[(size_t)s.g[r] * NP + e];
+        C x = proj[(size_t)s.g[r] * NP + e];
+        if constexpr (sizeof(T) == 4) {  // X exp(-i mup phi) for the factored rotation
+            const int mup = a.pmup[e];
+            const C z = s.ph[r & 7][mup < 0 ? -mup : mup];
+            const T ex = z.x, ey = mup < 0 ? z.y : -z.y;
+            const C u = x;
+            x.x = u.x * ex - u.y * ey;
+            x.y = u.x * ey + u.y * ex;
+        }
+        s.xs[r][e] = x;
     }
     __syncthreads();
     rot_all<T, L, P::nclass, false>(a, s, w, p, j);
     __syncthreads();
-    T al[4], be[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const C v = s.ab[drow<T>(l >> 4, i) & 7];
-        al[i] = v.x;
-        be[i] = v.y;
+    {
+        const C abv = s.ab[l & 7];  // the pair of row l & 15
+        T *scr = reinterpret_cast<T *>(&s.xs[0][0]) + w * 16 * scr_stride<T>();
+        oz_wave<T, L, 0>(s, a.frag + (size_t)(iq - 1) * P::nfrag * 64, scr, abv.x, abv.y, w, l);
     }
-    oz_wave<T, L, 0>(s, a.frag + (size_t)(iq - 1) * P::nfrag * 64, al, be, w, l);
     __syncthreads();
     rot_all<T, L, P::nclass, true>(a, s, w, p, j);
     __syncthreads();
@@ -565,20 +579,10 @@ HostPlan host_plan_of(int L) {
 
 template <typename T, int L> hipError_t launch_ozm_t(mdft_ctx *c, typename Ty<T>::c *proj) {
     OzmArgs<T> a;
-    a.nx = c->d.nx;
-    a.ny = c->d.ny;
-    a.np = c->np;
     a.ntiles = c->pairs.ntiles;
     a.per = (c->pairs.ntiles + 7) / 8;
-    a.dq = c->dq;
-    a.kx = c->d_kx;
-    a.ky = c->d_ky;
-    a.kz = c->d_kz;
-    a.pq = c->pairs.pq;
-    a.pmq = c->pairs.pmq;
-    a.pflag = c->pairs.pflag;
-    a.tstart = c->pairs.tstart;
-    a.tinfo = c->pairs.tinfo;
+    a.tpair = c->pairs.tpair;
+    a.pmup = c->d_pmup;
     a.frag = static_cast<const T *>(c->d_ozm_frag);
     a.ftab = static_cast<const T *>(c->d_ozm_ftab);
     a.units = reinterpret_cast<const int4 *>(c->d_ozm_units);

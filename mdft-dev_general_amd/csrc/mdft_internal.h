@@ -44,6 +44,14 @@ struct FftAxisPlan {
     double2 *tw;                 // device [n]: exp(+2 pi i k / n)
 };
 
+// One (q, -q) pair of a q-space tile: grid indices of q and -q, flags (bit 0: q == -q,
+// bit 1: Nyquist plane, bit 2: padding entry of a partial tile), the tile's interpolation
+// row iq, the frame of q (cos, sin of theta and phi) and alpha = |q| / dq + 1 - iq.
+struct TilePair {
+    int gq, gmq, flag, iq;
+    double ct, st, cp, sp, al, pad;
+};
+
 // Schedule of the (q, -q) pairs of the OZ step, grouped in chunks of pairs that
 // share the same |q| interpolation interval.
 struct PairSchedule {
@@ -57,6 +65,7 @@ struct PairSchedule {
     int ntiles = 0;           // tiles of at most 8 pairs of one chunk (the matrix-core q-space kernel)
     int *tstart = nullptr;    // first pair of each tile
     int *tinfo = nullptr;     // pair count | iq << 4 of each tile
+    TilePair *tpair = nullptr;  // [ntiles][8] the tiles' pairs, padded (flag bit 2 marks padding)
 };
 
 struct mdft_ctx {
