@@ -13,7 +13,6 @@
 // Layout: points r are processed in tiles of PT consecutive points per
 // workgroup; xi/vext/df tiles [PT][no] and projection tiles [PT][np] are
 // contiguous in HBM, so every global access is a coalesced stream.
-#include <algorithm>
 #include <cfloat>
 
 #include "mdft_internal.h"
@@ -643,165 +642,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     }
 }
 
-// ---- FP32 angl2proj with the phi analysis on the matrix cores (the excess term) -------
-// Same map as angl2proj_t<L, PT, float> with feat = 0: o = rho0 (xi^2 - 1), the psi
-// analysis G(theta, phi, mu2) = sum_psi o exp(+i mu2 psi) on the vector units, then
-//     F(theta, mup, mu2) = sum_phi G(theta, phi, mu2) exp(+i mup phi) / (nphi npsi)
-// as one real matrix product per tile: rows (point, theta, mu2), k = (phi, re/im of G),
-// columns (mup, re/im of F), with v_mfma_f32_16x16x4f32 (lane l: A row l & 15, k = l >> 4;
-// B k = l >> 4, column l & 15; D rows 4 (l >> 4) + i, column l & 15).  The B operand, the
-// fixed (cos, sin)(mup phi) table, is staged in LDS in lane order.  G and then F share one
-// LDS row per (point, theta, mu2): each 16-row block is read and rewritten by the one wave
-// that owns it.  Then the Gauss-Legendre contraction in theta with Wigner d on the vector
-// units.  (The same treatment of the final pass's phi synthesis measured no faster than
-// final_t, 4.7 against 4.6 ms at 128^3, mmax = 5, and is not used.)
-template <int L> struct MfDims {
-    using D = Dims<L>;
-    static constexpr int PT = 4;                               // points per tile
-    static constexpr int ROWS = PT * D::NTH * D::H1;           // (point, theta, mu2) rows
-    static constexpr int RT = (ROWS + 15) / 16;                // 16-row blocks
-    static constexpr int NKB = (2 * D::NPHI + 3) / 4;          // k blocks of 4 (k = 2 iphi + re/im)
-    static constexpr int NNB = (2 * D::W + 15) / 16;           // column blocks of 16 (n = 2 (mup + L) + re/im)
-    static constexpr int FS = (NKB * 4 > NNB * 16 ? NKB * 4 : NNB * 16) + 4;  // row stride (floats)
-};
-
-template <int L>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void a2p_mf(
-    const double *__restrict__ xi, float2 *__restrict__ proj, size_t N, AngTables t, double rho0,
-    const double *__restrict__ vsinv, const float *__restrict__ xi32) {
-    using D = Dims<L>;
-    using M = MfDims<L>;
-    typedef float f4v __attribute__((ext_vector_type(4)));
-    constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
-    constexpr int PT = M::PT, FS = M::FS;
-    constexpr int NKB = (2 * NPHI + 3) / 4, NNB = (2 * W + 15) / 16;  // k = 2 iphi + re/im, n = 2 (mup + L) + re/im
-    static_assert(4 * NKB <= FS && 16 * NNB <= FS, "row stride");
-    __shared__ float o[PT * NO];
-    __shared__ __align__(16) float fh[M::RT * 16 * FS];
-    __shared__ float2 twpsi[H1 * NPSI];  // (cos, sin)(2 pi mu2 ipsi / npsi) at [mu2][ipsi]
-    __shared__ float a2p[NP * NTH], fmv[NP];
-    __shared__ int pmup[NP], pmu2[NP];
-    __shared__ float bfl[NKB * NNB * 64];  // B fragments in lane order
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const size_t r0 = (size_t)blockIdx.x * PT;
-    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
-    // the tile's input goes into registers before the tables are staged, so that its HBM
-    // latency overlaps the staging
-    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
-    const double *src = xi + r0 * NO;
-    double xr[NIT];
-#pragma unroll
-    for (int q = 0; q < NIT; q++) {
-        const int i = tid + q * kThreads;
-        double x = 0.0;
-        if (i < npts * NO) {
-            x = xi32 ? (double)xi32[r0 * NO + i] : src[i];
-            if (vsinv) x *= vsinv[i % NO];
-        }
-        xr[q] = x;
-    }
-    for (int i = tid; i < H1 * NPSI; i += kThreads) {
-        const int kk = ((i / NPSI) * (i % NPSI)) % NPSI;
-        twpsi[i] = make_float2((float)t.cps[kk], (float)t.sps[kk]);
-    }
-    for (int i = tid; i < NP * NTH; i += kThreads) a2p[i] = (float)t.a2p[i];
-    for (int i = tid; i < NP; i += kThreads) {
-        fmv[i] = (float)t.fm[i];
-        pmup[i] = t.pmup[i];
-        pmu2[i] = t.pmu[i] / 2;
-    }
-    // B fragments: k = 2 iphi + c (c = re/im of G), n = 2 (mup + L) + d (d = re/im of F), with
-    // e = (cos, sin)(2 pi mup iphi / nphi) / (nphi npsi): re F = sum re G e.x - im G e.y,
-    // im F = sum re G e.y + im G e.x
-    for (int i = tid; i < NKB * NNB * 64; i += kThreads) {
-        const int ln = i & 63, kb = (i >> 6) / NNB, nb = (i >> 6) % NNB;
-        const int kk = 4 * kb + (ln >> 4), n = 16 * nb + (ln & 15);
-        float v = 0.0f;
-        if (kk < 2 * NPHI && n < 2 * W) {
-            const int iphi = kk >> 1, mup = (n >> 1) - L;
-            int ph = (mup * iphi) % NPHI;
-            ph += (ph < 0) ? NPHI : 0;
-            const double nrm = 1.0 / (double)(NPHI * NPSI);
-            const float ex = (float)(t.cph[ph] * nrm), ey = (float)(t.sph[ph] * nrm);
-            const int c = kk & 1, d = n & 1;
-            v = d == 0 ? (c == 0 ? ex : -ey) : (c == 0 ? ey : ex);
-        }
-        bfl[i] = v;
-    }
-    // k padding columns and padding rows are zero (their B rows are zero, the values must be
-    // finite; the products keep padding rows at zero and write no k padding column)
-    for (int i = tid; i < M::RT * 16 * (4 * NKB - 2 * NPHI); i += kThreads) {
-        const int r = i / (4 * NKB - 2 * NPHI), c = 2 * NPHI + i % (4 * NKB - 2 * NPHI);
-        fh[r * FS + c] = 0.0f;
-    }
-    for (int i = tid; i < (M::RT * 16 - M::ROWS) * 2 * NPHI; i += kThreads)
-        fh[(M::ROWS + i / (2 * NPHI)) * FS + i % (2 * NPHI)] = 0.0f;
-#pragma unroll
-    for (int q = 0; q < NIT; q++) {
-        const int i = tid + q * kThreads;
-        if (i >= PT * NO) break;
-        o[i] = (i < npts * NO) ? (float)(rho0 * (xr[q] * xr[q] - 1.0)) : 0.0f;
-    }
-    __syncthreads();
-    {
-        // psi analysis: G into the rows (point, theta, mu2), columns 2 iphi + re/im
-        for (int i = tid; i < PT * NTH * NPHI * H1; i += kThreads) {
-            const int mu2 = i % H1;
-            const int line = i / H1;  // (pt * NTH + it) * NPHI + iphi
-            const int iphi = line % NPHI, ptit = line / NPHI;
-            const float *ln = o + line * NPSI;
-            const float2 *tw = twpsi + mu2 * NPSI;
-            float re = 0.0f, im = 0.0f;
-#pragma unroll
-            for (int ipsi = 0; ipsi < NPSI; ipsi++) {
-                const float v = ln[ipsi];
-                re += v * tw[ipsi].x;
-                im += v * tw[ipsi].y;
-            }
-            float *row = fh + (ptit * H1 + mu2) * FS;
-            row[2 * iphi] = re;
-            row[2 * iphi + 1] = im;
-        }
-        __syncthreads();
-        for (int rt = wv; rt < M::RT; rt += kThreads / 64) {
-            float *blk = fh + rt * 16 * FS;
-            float a[NKB];
-#pragma unroll
-            for (int kb = 0; kb < NKB; kb++) a[kb] = blk[(lane & 15) * FS + 4 * kb + (lane >> 4)];
-            f4v acc[NNB];
-#pragma unroll
-            for (int nb = 0; nb < NNB; nb++) acc[nb] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kb = 0; kb < NKB; kb++)
-#pragma unroll
-                for (int nb = 0; nb < NNB; nb++)
-                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb], bfl[(kb * NNB + nb) * 64 + lane], acc[nb], 0, 0, 0);
-#pragma unroll
-            for (int nb = 0; nb < NNB; nb++) {
-                const int n = 16 * nb + (lane & 15);
-                if (n < 2 * W) {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) blk[(4 * (lane >> 4) + i) * FS + n] = acc[nb][i];
-                }
-            }
-        }
-        __syncthreads();
-        float2 *dst = proj + r0 * NP;
-        for (int i = tid; i < npts * NP; i += kThreads) {
-            const int pt = i / NP, ip = i - pt * NP;
-            const float *f = fh + (pt * NTH * H1 + pmu2[ip]) * FS + 2 * (pmup[ip] + L);
-            const float *a = a2p + ip * NTH;
-            float re = 0.0f, im = 0.0f;
-#pragma unroll
-            for (int it = 0; it < NTH; it++) {
-                re += f[it * H1 * FS] * a[it];
-                im += f[it * H1 * FS + 1] * a[it];
-            }
-            dst[i] = make_float2(re * fmv[ip], im * fmv[ip]);
-        }
-    }
-}
-
 constexpr int kPT = 4;  // points per workgroup tile (keeps the LDS tile under 64 KB at mmax = 5)
 
 size_t a2p_lds(const AngTables &t) {
@@ -945,16 +785,6 @@ FinalConsts final_consts(mdft_ctx *c) {
 }
 
 hipError_t launch_angl2proj_f32(mdft_ctx *c, const double *xi, float2 *proj) {
-    if (c->d.mrso == 2 && c->L >= 1 && c->L <= 5) {  // phi analysis on the matrix cores
-        const size_t nb = (c->N + MfDims<5>::PT - 1) / MfDims<5>::PT;
-        switch (c->L) {
-#define A2P_MF(LL) \
-    case LL: hipLaunchKernelGGL((a2p_mf<LL>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, c->N, c->ang, c->rho0, c->vsinv, c->xi32); break;
-            A2P_MF(1) A2P_MF(2) A2P_MF(3) A2P_MF(4) A2P_MF(5)
-#undef A2P_MF
-        }
-        return hipGetLastError();
-    }
     if (c->d.mrso != 2 || !launch_a2p_t<kPT32, float>(c, xi, proj)) return hipErrorNotSupported;
     return hipGetLastError();
 }
