@@ -170,9 +170,11 @@ def main():
     ap.add_argument("--unit-first-step", action="store_true",
                     help="fast path: first step of unit length along -g (default: the bulk Newton step)")
     ap.add_argument("--maxeval", type=int, default=1500)
-    ap.add_argument("--mode", choices=["converge", "eval", "copy"], default="converge",
+    ap.add_argument("--mode", choices=["converge", "eval", "copy", "fastpath"], default="converge",
                     help="copy: a plain device copy of 1 GiB (z = x through mdft_vec_lincomb), the "
-                         "reference kernel for calibrating rocprof's FETCH_SIZE / WRITE_SIZE counters")
+                         "reference kernel for calibrating rocprof's FETCH_SIZE / WRITE_SIZE counters; "
+                         "fastpath: the fast path's minimisation alone (cold start, FP32 evaluations, "
+                         "the reference's stop rule), --steps times, for kernel traces of the minimiser")
     ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp64",
                     help="evaluation precision for the rate/phase measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -205,6 +207,26 @@ def main():
                           "GBps": 16 * n / dt / 1e9}), flush=True)
         return
     sol = w.solute()
+    if args.mode == "fastpath":
+        from mdft import warmstart as wsm
+        e32 = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, "fp32")
+        e32.set_ck(w.ck_table())
+        e32.compute_vext(sol, w.solvent)
+        xi, df = e32.alloc_field(), e32.alloc_field()
+        m = args.fast_memory if args.fast_memory is not None else args.memory
+        for rep in range(max(args.steps, 1)):
+            e32.synchronize()
+            t0 = time.perf_counter()
+            wsm.cold_start(e32, xi)
+            r = minimize(e32, xi, g=df, m=m, gtol=0.0, maxeval=args.maxeval, fnoise=1e-6,
+                         fp32_vectors=not args.fp64_vectors, scaled=not args.unscaled,
+                         fp32_iterate=not (args.fp64_vectors or args.fp64_iterate),
+                         step0=0.0 if (args.unit_first_step or args.unscaled) else "newton", ftol=1e-3,
+                         ftol_absolute=True)
+            e32.synchronize()
+            print(json.dumps({"mode": "fastpath", "rep": rep, "s": time.perf_counter() - t0,
+                              "evaluations": r.evaluations, "iterations": r.iterations, "F": r.F}), flush=True)
+        return
     t_setup = time.perf_counter()
     ckt = w.ck_table()
     eng = Engine(w.n, w.length, w.mmax, w.mrso, w.kT, w.n0, args.precision)

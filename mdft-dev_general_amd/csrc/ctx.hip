@@ -17,7 +17,6 @@
 #include <tuple>
 
 #include "mdft_internal.h"
-#include "oz_tiles.h"
 #include "mdft_wsnet.h"
 
 static std::string g_create_error;
@@ -478,14 +477,6 @@ int mdft_ctx_create(const mdft_desc *desc, mdft_ctx **out) {
         const int rc = bf16_tables(c, a2p, p2a, fmv);
         if (rc != MDFT_OK) return bad(rc, "BF16 transform tables");
     }
-    if (d.precision == MDFT_PREC_FP64 && d.mrso == 2) {  // the tiled FP64 q-space kernel
-        std::vector<double> ftab;
-        std::vector<int> itab;
-        if (oz_tables(c, ftab, itab, &c->oz_ntin, &c->oz_ntout) != MDFT_OK) return bad(MDFT_EINVAL, "q-space tables");
-        if (upload(c, &c->d_oz_ftab, ftab.data(), ftab.size()) != hipSuccess ||
-            upload(c, &c->d_oz_itab, itab.data(), itab.size()) != hipSuccess)
-            return bad(MDFT_ENOMEM, "device allocation failed");
-    }
     if (d.precision != MDFT_PREC_FP64) {
         std::vector<float> ftab;
         std::vector<int> lanerow;
@@ -527,7 +518,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
     void *bufs[] = {c->d_w,   c->d_a2p,  c->d_p2a,  c->d_fm,   c->d_cph,      c->d_sph,        c->d_cps,
                     c->d_sps, c->d_pidx, c->d_pm,   c->d_pmup, c->d_pmu,      c->d_ia0,        c->d_kx,
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
-                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz_ftab, c->d_oz_itab, c->d_oz32_ftab, c->d_oz32_lanerow,
+                    c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
                     c->d_ozm_frag, c->d_ozm_ftab, c->d_ozm_units, c->d_ozm_rlen,
                     c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart};
     for (void *b : bufs)

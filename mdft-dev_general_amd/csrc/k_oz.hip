@@ -9,20 +9,18 @@
 // Both chops go through Fortran's default-kind CMPLX, so the component they
 // keep is rounded to single precision; that rounding is reproduced here.
 //
-// Two kernels:
-// * oz64t (molecular symmetry order 2, the workloads' water): tiles of (q, -q)
-//   pairs, R from its factorised form (see the comment above the kernel);
-// * oz_kernel (any symmetry order): one wave per pair, R by the recursion of
-//   src/module_rotation.f90:119-247.  Pairs are pre-sorted by interpolation
-//   interval iq and cut into chunks; one workgroup takes one chunk, stages the
-//   two c rows it needs in LDS once, and its waves each take one pair at a time.
+// One wave per pair, R by the recursion of src/module_rotation.f90:119-247.
+// Pairs are pre-sorted by interpolation interval iq and cut into chunks; one
+// workgroup takes one chunk, stages the two c rows it needs in LDS once, and
+// its waves each take one pair at a time.  This is the kernel of any molecular
+// symmetry order; order 2 (the workloads' water) runs the matrix-core kernel of
+// k_ozm.hip.
 #include <algorithm>
 #include <array>
 #include <cfloat>
 #include <vector>
 
 #include "mdft_internal.h"
-#include "oz_tiles.h"
 
 namespace {
 
@@ -318,258 +316,6 @@ __global__ __launch_bounds__(64 * kWaves) void oz_kernel(typename C2<T>::t *__re
 #undef PIDX
 }
 
-// ---- the tiled FP64 kernel (molecular symmetry order 2) -----------------------
-// Same workgroup structure as the FP32 kernel of k_oz32.hip: one workgroup per chunk
-// of pairs sharing the interpolation interval, kP pairs (both sides) per tile, each
-// phase keeping what it reuses in registers, the pair data in LDS between phases.
-// R(q) is formed from the factorisation R^m_{a,b} = exp(-i a phi_q) d^m_{a,b}(theta_q)
-// in FP64 (d^m from its Fourier series, as in the FP32 kernel), each entry chopped as
-// the reference chops its recursion's result (:491-492); the chop keeps the surviving
-// component in single precision, so both forms give the same chopped entries unless a
-// component lies within ~1e-15 of the FLT_EPSILON threshold or of an FP32 rounding
-// boundary.  Rotation, OZ and rotation back then follow the reference's arithmetic in
-// its order: sums over mup, (n, nu) and khi ascending, ceff = alpha c[iq+1] +
-// (1 - alpha) c[iq] (:534-539), the DBL_EPSILON chop of gamma (:618-623), the -q side
-// with R(., -khi) and (-1)^m (:517-520,636-638), the Nyquist conj (:652-656).
-double textbook_d_fp64(int j, int mp, int m, double beta) {
-    auto f = [](int n) {
-        double r = 1.0;
-        for (int i = 2; i <= n; i++) r *= i;
-        return r;
-    };
-    double s = 0.0;
-    for (int k = 0; k <= 2 * j; k++) {
-        const int a = j + m - k, b = k, c = mp - m + k, e = j - mp - k;
-        if (a < 0 || b < 0 || c < 0 || e < 0) continue;
-        s += (((mp - m + k) & 1) ? -1.0 : 1.0) * pow(cos(beta / 2), 2 * j + m - mp - 2 * k) *
-             pow(sin(beta / 2), mp - m + 2 * k) / (f(a) * f(b) * f(c) * f(e));
-    }
-    return sqrt(f(j + mp) * f(j - mp) * f(j + m) * f(j - m)) * s;
-}
-
-
-struct OztArgs64 {
-    int nx, ny, np, npc, L;
-    oz::Layout lay;
-    double dq;
-    const double *kx, *ky, *kz;
-    const int *pq, *pmq, *pflag, *cstart, *ccount, *ciq;
-    const double2 *ck;
-    const double *ftab;
-    const int *itab;
-};
-
-struct OztLds64 {
-    size_t ys, as, ds, trig, ph, meta, it, total;
-    __host__ __device__ OztLds64(int np, int nR, int nI) {
-        auto up = [](size_t v) { return (v + 15) & ~(size_t)15; };
-        const size_t nps = (size_t)(np | 1);
-        ys = 0;
-        as = up(ys + sizeof(double2) * oz::kP * 2 * nps);
-        ds = up(as + sizeof(double2) * oz::kP * 2 * nps);
-        trig = up(ds + sizeof(double) * oz::kP * nR);
-        ph = up(trig + sizeof(double2) * oz::kP * (oz::kMaxL + 1));
-        meta = up(ph + sizeof(double2) * oz::kP * (oz::kMaxL + 1));
-        it = up(meta + sizeof(double) * 2 * oz::kP);
-        total = up(it + sizeof(int) * nI);
-    }
-};
-
-// R^m(mup, kap) of one pair: exp(-i mup phi) d^m(mup, kap), chopped
-__device__ inline double2 r_entry(const double *dm, int W, int m, int mup, int kap, const double2 *ph) {
-    const double dv = dm[(mup + m) * W + (kap + m)];
-    const double2 e = ph[mup < 0 ? -mup : mup];  // exp(+i |mup| phi)
-    return chop_R(make_double2(e.x * dv, (mup < 0 ? e.y : -e.y) * dv));
-}
-
-__global__ __launch_bounds__(oz::kTh) void oz64t(double2 *__restrict__ proj, OztArgs64 a) {
-    using namespace oz;
-    extern __shared__ __align__(16) unsigned char sm[];
-    const int np = a.np, npc = a.npc, L = a.L;
-    const int nR = roff(L + 1);
-    const int nps = np | 1;
-    const OztLds64 lo(np, nR, a.lay.size());
-    double2 *Ys = reinterpret_cast<double2 *>(sm + lo.ys);  // inputs, then the OZ outputs gamma'
-    double2 *As = reinterpret_cast<double2 *>(sm + lo.as);  // rotated inputs, then the rotated-back outputs
-    double *ds = reinterpret_cast<double *>(sm + lo.ds);
-    double2 *trig = reinterpret_cast<double2 *>(sm + lo.trig);  // (cos k theta, sin k theta) per pair
-    double2 *ph = reinterpret_cast<double2 *>(sm + lo.ph);      // exp(+i k phi) per pair
-    double *meta = reinterpret_cast<double *>(sm + lo.meta);    // alpha, 1 - alpha per pair
-    int *itab = reinterpret_cast<int *>(sm + lo.it);
-    __shared__ int pmeta[3 * kP];                               // gq, gmq, flag per pair
-    const int tid = threadIdx.x;
-    const int chunk = blockIdx.x;
-    const int iq = a.ciq[chunk];
-    for (int i = tid; i < a.lay.size(); i += kTh) itab[i] = a.itab[i];
-    __syncthreads();
-    const int *dd = itab + a.lay.dd();
-    const int4 *tin = reinterpret_cast<const int4 *>(itab + a.lay.tin());
-    const int4 *tout = reinterpret_cast<const int4 *>(itab + a.lay.tout());
-    const int *orow = itab + a.lay.orow(), *ia0 = itab + a.lay.ia0(), *pmup = itab + a.lay.pmup();
-    const int *ncol = itab + a.lay.ncol(), *ctab = itab + a.lay.ctab(), *poff = itab + a.lay.poff();
-    const int ntin = a.lay.ntin, ntout = a.lay.ntout;
-    const double2 *clo = a.ck + (size_t)(iq - 1) * npc, *chi = a.ck + (size_t)iq * npc;
-
-    const int start = a.cstart[chunk], count = a.ccount[chunk];
-    for (int t0 = 0; t0 < count; t0 += kP) {
-        const int npair = (count - t0) < kP ? (count - t0) : kP;
-        if (tid < npair) {
-            const int pr = start + t0 + tid;
-            const int gq = a.pq[pr];
-            pmeta[3 * tid + 0] = gq;
-            pmeta[3 * tid + 1] = a.pmq[pr];
-            pmeta[3 * tid + 2] = a.pflag[pr];
-            const int ix = gq % a.nx, iy = (gq / a.nx) % a.ny, iz = gq / (a.nx * a.ny);
-            const double q0 = a.kx[ix], q1 = a.ky[iy], q2 = a.kz[iz];
-            const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2), rxy = sqrt(q0 * q0 + q1 * q1);
-            // alpha relative to the chunk's row so host and device agree on the interval
-            const double al = qn / a.dq + 1.0 - (double)iq;
-            meta[2 * tid + 0] = al;
-            meta[2 * tid + 1] = 1.0 - al;
-            // frame angles as in the FP32 kernel (q along -z: theta = pi, phi = pi; q = 0: the z frame)
-            const double ct = qn > 0.0 ? q2 / qn : 1.0, st = qn > 0.0 ? rxy / qn : 0.0;
-            const double cp = rxy > 0.0 ? q0 / rxy : (q2 < 0.0 ? -1.0 : 1.0);
-            const double sp = rxy > 0.0 ? q1 / rxy : 0.0;
-            double ck = 1.0, sk = 0.0, cpk = 1.0, spk = 0.0;
-            for (int k = 0; k <= kMaxL; k++) {
-                trig[tid * (kMaxL + 1) + k] = make_double2(ck, sk);
-                ph[tid * (kMaxL + 1) + k] = make_double2(cpk, spk);
-                const double nc = ck * ct - sk * st;
-                sk = sk * ct + ck * st;
-                ck = nc;
-                const double nq = cpk * cp - spk * sp;
-                spk = spk * cp + cpk * sp;
-                cpk = nq;
-            }
-        }
-        __syncthreads();
-        for (int i = tid; i < npair * 2 * np; i += kTh) {
-            const int p = i / (2 * np), r = i - p * 2 * np, s = r >= np, ip = r - s * np;
-            Ys[(2 * p + s) * nps + ip] = proj[(size_t)pmeta[3 * p + s] * np + ip];
-        }
-        for (int p = 0; p < npair; p++)
-        for (int e = tid; e < nR; e += kTh) {
-            const int dsc = dd[e];
-            const double *T = a.ftab + (dsc >> 4);
-            const int m = (dsc >> 1) & 7;
-            const bool odd = dsc & 1;
-            const double2 *tg = trig + p * (kMaxL + 1);
-            double v = 0.0;
-            for (int j = 0; j <= m; j++) v += T[j] * (odd ? tg[j].y : tg[j].x);
-            ds[p * nR + e] = v;
-        }
-        __syncthreads();
-        // ---- rotation to the q frame (:505-523): A_q(m,khi,mu2) = sum_mup X_q R(m,mup,khi),
-        // A_-q(m,khi,mu2) = (-1)^m sum_mup X_-q R(m,mup,-khi)
-        // (pair, side) fastest: the 16 lanes of a task group run the same task
-        for (int i = tid; i < 2 * kP * ntin; i += kTh) {
-            const int ps = i & (2 * kP - 1), tk = i / (2 * kP), p = ps >> 1, s = ps & 1;
-            if (p >= npair) continue;
-            const int4 t = tin[tk];
-            const int m = t.x, W = 2 * m + 1, nmu = m / 2 + 1;
-            const int base = poff[m] + t.y;
-            const double2 *x = Ys + ps * nps + base;
-            double2 xv[kMaxW];
-#pragma unroll
-            for (int j = 0; j < kMaxW; j++)
-                if (j < W) xv[j] = x[j * nmu];
-            const double *dm = ds + p * nR + roff(m);
-            const double2 *php = ph + p * (kMaxL + 1);
-            const double sg = (s && (m & 1)) ? -1.0 : 1.0;
-            double2 *aout = As + ps * nps + base;
-            for (int kk = 0; kk < t.w; kk++) {
-                const int khi = t.z + kk;
-                const int kap = s ? -khi : khi;
-                double sx = 0.0, sy = 0.0;
-#pragma unroll
-                for (int j = 0; j < kMaxW; j++)
-                    if (j < W) {
-                        const double2 rr = r_entry(dm, W, m, j - m, kap, php);
-                        const double2 v = cmul(xv[j], rr);
-                        sx += v.x;
-                        sy += v.y;
-                    }
-                aout[(khi + m) * nmu] = make_double2(sx * sg, sy * sg);
-            }
-        }
-        __syncthreads();
-        // ---- molecular OZ with interpolated c (:534-606), gamma' chopped (:618-623)
-        for (int i = tid; i < 2 * kP * np; i += kTh) {
-            const int ps = i & (2 * kP - 1), p = ps >> 1;
-            if (p >= npair) continue;
-            const int ip = orow[i / (2 * kP)];
-            const int kk = pmup[ip] + kMaxL;
-            const int nc = ncol[kk];
-            const int *ctk = ctab + kk * kCMax;
-            const double2 *lo = clo + ia0[ip], *hi = chi + ia0[ip];
-            const double alpha = meta[2 * p], beta = meta[2 * p + 1];
-            const double2 *own = As + ps * nps, *oth = As + (ps ^ 1) * nps;
-            double sx = 0.0, sy = 0.0;
-            for (int c = 0; c < nc; c++) {
-                const int ent = ctk[c];
-                const int off = ent >> 1;
-                const double2 l = lo[c], h = hi[c];
-                const double2 ce = make_double2(alpha * h.x + beta * l.x, alpha * h.y + beta * l.y);
-                double2 v;
-                if (ent & 1) {
-                    v = cmul(ce, own[off]);
-                } else {
-                    const double2 o = oth[off];
-                    v = cmul(ce, make_double2(o.x, -o.y));
-                }
-                sx += v.x;
-                sy += v.y;
-            }
-            Ys[ps * nps + ip] = chop_gamma(make_double2(sx, sy));
-        }
-        __syncthreads();
-        // ---- rotation back with conj(R) (:611-641); the -q side takes R(., -khi), (-1)^m and,
-        // for a self-conjugate Nyquist pair, the conjugate (:648-656)
-        for (int i = tid; i < 2 * kP * ntout; i += kTh) {
-            const int ps = i & (2 * kP - 1), tk = i / (2 * kP), p = ps >> 1, s = ps & 1;
-            if (p >= npair) continue;
-            const int4 t = tout[tk];
-            const int m = t.x, W = 2 * m + 1, nmu = m / 2 + 1;
-            const int base = poff[m] + t.y;
-            const double2 *gsrc = Ys + ps * nps + base;
-            double2 gv[kMaxW];
-#pragma unroll
-            for (int j = 0; j < kMaxW; j++)
-                if (j < W) gv[j] = gsrc[j * nmu];
-            const double *dm = ds + p * nR + roff(m);
-            const double2 *php = ph + p * (kMaxL + 1);
-            const int flag = pmeta[3 * p + 2];
-            const bool conj_store = s && (flag & 1) && (flag & 2);
-            const double sg = (s && (m & 1)) ? -1.0 : 1.0;
-            double2 *oout = As + ps * nps + base;
-            for (int uu = 0; uu < t.w; uu++) {
-                const int mup = t.z + uu;
-                double sx = 0.0, sy = 0.0;
-#pragma unroll
-                for (int j = 0; j < kMaxW; j++)
-                    if (j < W) {
-                        const int khi = j - m;
-                        const double2 rr = r_entry(dm, W, m, mup, s ? -khi : khi, php);
-                        const double2 v = cmulc(gv[j], rr);
-                        sx += v.x;
-                        sy += v.y;
-                    }
-                sx *= sg;
-                sy *= sg;
-                if (conj_store) sy = -sy;
-                oout[(mup + m) * nmu] = make_double2(sx, sy);
-            }
-        }
-        __syncthreads();
-        for (int i = tid; i < npair * 2 * np; i += kTh) {
-            const int p = i / (2 * np), r = i - p * 2 * np, s = r >= np, ip = r - s * np;
-            if (!s && (pmeta[3 * p + 2] & 1)) continue;  // for q == -q the -q store wins
-            proj[(size_t)pmeta[3 * p + s] * np + ip] = As[(2 * p + s) * nps + ip];
-        }
-        __syncthreads();
-    }
-}
-
 }  // namespace
 
 template <typename T>
@@ -620,121 +366,5 @@ hipError_t launch_oz(mdft_ctx *c, typename C2<T>::t *proj) {
     return hipGetLastError();
 }
 
-// Host tables of the tiled q-space kernel: Fourier coefficients of d^m(theta)
-// (ftab) and the integer table laid out by oz::Layout (itab); *ntin, *ntout receive
-// the task counts.
-int oz_tables(mdft_ctx *c, std::vector<double> &ftab, std::vector<int> &itab, int *ntin, int *ntout) {
-    using namespace oz;
-    const int L = c->L, np = c->np, mrso = c->d.mrso;
-    if (L > kMaxL || mrso != 2) return MDFT_EINVAL;
-    const double hp = 0.5 * acos(-1.0);
-    ftab.clear();
-    for (int m = 0; m <= L; m++) {
-        const int W = 2 * m + 1;
-        std::vector<double> D(W * W);
-        for (int k = -m; k <= m; k++)
-            for (int a = -m; a <= m; a++) D[(k + m) * W + (a + m)] = textbook_d_fp64(m, k, a, hp);
-        for (int a = -m; a <= m; a++)
-            for (int b = -m; b <= m; b++) {
-                const int ab = ((a - b) % 4 + 4) % 4;  // i^{a-b} = 1, i, -1, -i
-                for (int k = 0; k <= m; k++) {
-                    const double sk = D[(k + m) * W + (a + m)] * D[(k + m) * W + (b + m)];
-                    const double smk = D[(-k + m) * W + (a + m)] * D[(-k + m) * W + (b + m)];
-                    double v;
-                    if (k == 0) v = (ab == 0) ? sk : (ab == 2 ? -sk : 0.0);
-                    // even a-b: i^{a-b} (S_k + S_-k) cos; odd: Re(i^{a-b} (-i)) (S_k - S_-k) sin
-                    else if (ab == 0) v = sk + smk;
-                    else if (ab == 2) v = -(sk + smk);
-                    else if (ab == 1) v = sk - smk;
-                    else v = -(sk - smk);
-                    ftab.push_back(v);
-                }
-            }
-    }
-    std::vector<int> poff(kMaxL + 2, 0), foff(L + 2, 0);
-    for (int m = 0; m <= L; m++) {
-        poff[m + 1] = poff[m] + (2 * m + 1) * (m / mrso + 1);
-        foff[m + 1] = foff[m] + (2 * m + 1) * (2 * m + 1) * (m + 1);
-    }
-    for (int m = L + 1; m <= kMaxL; m++) poff[m + 1] = poff[m];
-    auto pidx = [&](int m, int mup, int mu2) { return poff[m] + (mup + m) * (m / mrso + 1) + mu2; };
-    const int nR = roff(L + 1);
-    // rotation tasks: one (m, mu2) column and up to kChunkCols outputs, heaviest first
-    std::vector<std::array<int, 4>> tasks;
-    for (int m = 0; m <= L; m++)
-        for (int mu2 = 0; mu2 <= m / mrso; mu2++)
-            for (int k0 = -m; k0 <= m; k0 += kChunkCols)
-                tasks.push_back({m, mu2, k0, std::min(kChunkCols, m - k0 + 1)});
-    std::stable_sort(tasks.begin(), tasks.end(), [](const std::array<int, 4> &x, const std::array<int, 4> &y) {
-        return (2 * x[0] + 1) * x[3] > (2 * y[0] + 1) * y[3];
-    });
-    *ntin = *ntout = (int)tasks.size();
-    oz::Layout lay{nR, *ntin, *ntout, np};
-    itab.assign(lay.size(), 0);
-    for (int e = 0, m = 0; e < nR; e++) {
-        while (m < L && roff(m + 1) <= e) m++;
-        const int rem = e - roff(m), W = 2 * m + 1, ab = rem / W - rem % W;
-        itab[lay.dd() + e] = (foff[m] + rem * (m + 1)) << 4 | m << 1 | (ab & 1);
-    }
-    for (size_t k = 0; k < tasks.size(); k++)
-        for (int j = 0; j < 4; j++) {
-            itab[lay.tin() + 4 * k + j] = tasks[k][j];
-            itab[lay.tout() + 4 * k + j] = tasks[k][j];
-        }
-    std::vector<int> len(np), order(np);
-    for (int ip = 0; ip < np; ip++) {
-        int n_ = 0;
-        for (int n = std::abs(c->pmup[ip]); n <= L; n++) n_ += 2 * (n / mrso) + 1;
-        len[ip] = n_;
-        order[ip] = ip;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return len[x] > len[y]; });
-    for (int ip = 0, ia = 0; ip < np; ip++) {
-        itab[lay.orow() + ip] = order[ip];
-        itab[lay.ia0() + ip] = ia;
-        ia += len[ip];
-        itab[lay.pmup() + ip] = c->pmup[ip];
-    }
-    for (int khi = -L; khi <= L; khi++) {
-        int n_ = 0;
-        for (int n = std::abs(khi); n <= L; n++) {
-            const int h = n / mrso;
-            for (int nu2 = -h; nu2 <= h; nu2++) {
-                if (n_ >= kCMax) return MDFT_EINVAL;
-                itab[lay.ctab() + (khi + kMaxL) * kCMax + n_++] = (pidx(n, khi, 0) + std::abs(nu2)) << 1 | (nu2 < 0 ? 1 : 0);
-            }
-        }
-        itab[lay.ncol() + khi + kMaxL] = n_;
-    }
-    for (int m = 0; m < kMaxL + 2; m++) itab[lay.poff() + m] = poff[m];
-    return MDFT_OK;
-}
-
-hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) {
-    if (c->d.mrso != 2 || !c->d_oz_itab) return launch_oz<double>(c, proj);  // general symmetry order
-    OztArgs64 a;
-    a.nx = c->d.nx;
-    a.ny = c->d.ny;
-    a.np = c->np;
-    a.npc = c->npc;
-    a.L = c->L;
-    a.lay = oz::Layout{oz::roff(c->L + 1), c->oz_ntin, c->oz_ntout, c->np};
-    a.dq = c->dq;
-    a.kx = c->d_kx;
-    a.ky = c->d_ky;
-    a.kz = c->d_kz;
-    a.pq = c->pairs.pq;
-    a.pmq = c->pairs.pmq;
-    a.pflag = c->pairs.pflag;
-    a.cstart = c->pairs.cstart;
-    a.ccount = c->pairs.ccount;
-    a.ciq = c->pairs.ciq;
-    a.ck = c->d_ck;
-    a.ftab = c->d_oz_ftab;
-    a.itab = c->d_oz_itab;
-    const OztLds64 lds(c->np, oz::roff(c->L + 1), a.lay.size());
-    if (lds.total > c->lds_limit) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(oz64t, dim3((unsigned)c->pairs.nchunks), dim3(oz::kTh), lds.total, c->stream, proj, a);
-    return hipGetLastError();
-}
-
+// any molecular symmetry order (the matrix-core kernel of k_ozm.hip takes order 2)
+hipError_t launch_oz_f64(mdft_ctx *c, double2 *proj) { return launch_oz<double>(c, proj); }

@@ -89,9 +89,6 @@ struct mdft_ctx {
     int npc = 0, nq = 0;
     double dq = 0;
     double2 *d_ck = nullptr;      // [nq][npc]
-    double *d_oz_ftab = nullptr;  // Fourier coefficients of d^m(theta), tiled FP64 q-space kernel
-    int *d_oz_itab = nullptr;     // its integer tables (d entries, rotation tasks, OZ rows)
-    int oz_ntin = 0, oz_ntout = 0;  // its rotation task counts
     float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
     int *d_oz32_lanerow = nullptr; // descriptor table of the FP32 q-space kernel (rotation offsets, lane rows, OZ columns)
     void *d_ozm_frag = nullptr;   // matrix-core q-space kernel: c(k) B fragments per interval (FP64 or FP32)
