@@ -37,9 +37,13 @@
 //       out_q(m, mup, mu) = sum_khi gamma_q(m, khi, mu) conj(R(mup, khi))
 //       out_-q(m, mup, mu) = (-1)^m sum_khi gamma_-q(m, khi, mu) conj(R(mup, -khi))
 //     stored straight to the projection array.
-// The per-entry R is formed exactly as the tiled kernel of k_oz.hip forms it,
-// so the FLT_EPSILON chop decides alike; the OZ sums are summed in the matrix
-// unit's order (a rounding-level difference from the reference's order).
+// R is formed from the factorisation with d^m from its Fourier series (both
+// exact in FP64 to rounding), then chopped as the reference chops its
+// recursion's result; the OZ sums are summed in the matrix unit's order (a
+// rounding-level difference from the reference's order).
+// BF16 contexts run the FP32 kernel with the contraction in BF16 operands and
+// FP32 accumulation (v_mfma_f32_16x16x32_bf16: 16 complex columns per
+// instruction), the c(k) fragments stored in BF16.
 #include <algorithm>
 #include <cfloat>
 #include <utility>
@@ -51,6 +55,7 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 
 template <typename T> struct Ty;
 template <> struct Ty<double> {
@@ -169,6 +174,14 @@ template <int L> struct Plan {
         return f;
     }
     static constexpr int nfrag = foff(L + 1);
+    // BF16 contraction: k blocks of 32 (16 complex columns)
+    static constexpr int nkb32(int khi) { return (ncols(khi) + 15) / 16; }
+    static constexpr int foff32(int khi) {
+        int f = 0;
+        for (int k = -L; k < khi; k++) f += nkb32(k) * nnb(k);
+        return f;
+    }
+    static constexpr int nfrag32 = foff32(L + 1);
 };
 
 constexpr int kWaves = 4;  // waves per tile
@@ -178,6 +191,8 @@ template <typename T> struct OzmArgs {
     const TilePair *tpair;      // [tile][8] the tiles' pairs
     const int *pmup;            // mup of each projection
     const T *frag;              // [interval][nfrag][64] B fragments of (c[iq], c[iq+1])
+    const bf16x8 *frag16;       // BF16 contexts: [interval][nfrag32][64] B fragments, 8 values per lane
+    const int *colent;          // [2 L + 1][32] OZ column entries (entry << 1 | own side) of each block
     const T *ftab;              // [entry][kp] Fourier coefficients of d^m(theta)
     const int4 *units;          // [round][8] rotation units
     const int *rlen;            // [round] iterations of each round
@@ -441,15 +456,53 @@ __device__ __forceinline__ void oz_blocks(Lds<T, L> &s, const T *__restrict__ fr
     if constexpr (KHI + kWaves <= L) oz_blocks<T, L, KHI + kWaves>(s, fr, scr, al, be, l);
 }
 
-template <typename T, int L, int W0>
-__device__ __forceinline__ void oz_wave(Lds<T, L> &s, const T *fr, T *scr, T al, T be, int w, int l) {
+// BF16 contraction of block khi (FP32 rotations and epilogue): A (16 x 32) lane l holds row
+// l & 15, k = 8 (l >> 4) + j: column 16 kb + 4 (l >> 4) + (j >> 1), re (j even) or im; B the
+// same k, column l & 15; D as in the FP32 form.
+template <int L, int KHI>
+__device__ __forceinline__ void oz_blocks_bf(Lds<float, L> &s, const bf16x8 *__restrict__ fr,
+                                             const int *__restrict__ colent, float *scr, float al, float be, int l) {
+    using P = Plan<L>;
+    constexpr int NKB = P::nkb32(KHI), NNB = P::nnb(KHI), F0 = P::foff32(KHI);
+    const int row16 = l & 15, g = l >> 4;
+    const float2 *own = s.buf[row16], *oth = s.buf[row16 ^ 8];
+    const int *ce = colent + (KHI + L) * 32 + 4 * g;
+    f4 acc[NNB];
+#pragma unroll
+    for (int nb = 0; nb < NNB; nb++) acc[nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+        bf16x8 a;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = ce[16 * kb + u];
+            const bool isown = e & 1;
+            const float2 v = (isown ? own : oth)[e >> 1];
+            a[2 * u] = (__bf16)v.x;
+            a[2 * u + 1] = (__bf16)(isown ? v.y : -v.y);
+        }
+#pragma unroll
+        for (int nb = 0; nb < NNB; nb++)
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, fr[(F0 + kb * NNB + nb) * 64 + l], acc[nb], 0, 0, 0);
+    }
+    store_all<float, L, KHI>(acc, s, scr, al, be, l, std::make_integer_sequence<int, NNB>{});
+    if constexpr (KHI + kWaves <= L) oz_blocks_bf<L, KHI + kWaves>(s, fr, colent, scr, al, be, l);
+}
+
+template <typename T, int L, bool BF, int W0>
+__device__ __forceinline__ void oz_wave(Lds<T, L> &s, const OzmArgs<T> &a, size_t iv, T *scr, T al, T be, int w,
+                                        int l) {
     if constexpr (W0 < kWaves && -L + W0 <= L) {
-        if (w == W0) oz_blocks<T, L, -L + W0>(s, fr, scr, al, be, l);
-        else oz_wave<T, L, W0 + 1>(s, fr, scr, al, be, w, l);
+        if (w == W0) {
+            if constexpr (BF) oz_blocks_bf<L, -L + W0>(s, a.frag16 + iv * Plan<L>::nfrag32 * 64, a.colent, scr, al, be, l);
+            else oz_blocks<T, L, -L + W0>(s, a.frag + iv * Plan<L>::nfrag * 64, scr, al, be, l);
+        } else {
+            oz_wave<T, L, BF, W0 + 1>(s, a, iv, scr, al, be, w, l);
+        }
     }
 }
 
-template <typename T, int L>
+template <typename T, int L, bool BF>
 __global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict__ proj, OzmArgs<T> a) {
     using P = Plan<L>;
     using C = typename Ty<T>::c;
@@ -510,7 +563,7 @@ __global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict
     {
         const C abv = s.ab[l & 7];  // the pair of row l & 15
         T *scr = reinterpret_cast<T *>(&s.xs[0][0]) + w * 16 * scr_stride<T>();
-        oz_wave<T, L, 0>(s, a.frag + (size_t)(iq - 1) * P::nfrag * 64, scr, abv.x, abv.y, w, l);
+        oz_wave<T, L, BF, 0>(s, a, (size_t)(iq - 1), scr, abv.x, abv.y, w, l);
     }
     __syncthreads();
     rot_all<T, L, P::nclass, true>(a, s, w, p, j);
@@ -542,9 +595,9 @@ double textbook_d_h(int j, int mp, int m, double beta) {
 }
 
 struct HostPlan {
-    int L, np, kp, nclass, nfrag;
+    int L, np, kp, nclass, nfrag, nfrag32;
     std::vector<int> col[2 * MDFT_MAX_MMAX + 1], row[2 * MDFT_MAX_MMAX + 1];
-    std::vector<int> nkb, nnb, foff;
+    std::vector<int> nkb, nnb, foff, ncol, nkb32, foff32;
 };
 
 template <int L> HostPlan host_plan() {
@@ -555,12 +608,16 @@ template <int L> HostPlan host_plan() {
     h.kp = P::kp;
     h.nclass = P::nclass;
     h.nfrag = P::nfrag;
+    h.nfrag32 = P::nfrag32;
     for (int khi = -L; khi <= L; khi++) {
-        for (int c = 0; c < 2 * P::nkb(khi); c++) h.col[khi + L].push_back(P::col(khi, c));
+        for (int c = 0; c < 32; c++) h.col[khi + L].push_back(P::col(khi, c));
         for (int r = 0; r < 4 * P::nnb(khi); r++) h.row[khi + L].push_back(P::row(khi, r));
         h.nkb.push_back(P::nkb(khi));
         h.nnb.push_back(P::nnb(khi));
         h.foff.push_back(P::foff(khi));
+        h.ncol.push_back(P::ncols(khi));
+        h.nkb32.push_back(P::nkb32(khi));
+        h.foff32.push_back(P::foff32(khi));
     }
     return h;
 }
@@ -577,13 +634,15 @@ HostPlan host_plan_of(int L) {
     }
 }
 
-template <typename T, int L> hipError_t launch_ozm_t(mdft_ctx *c, typename Ty<T>::c *proj) {
+template <typename T, int L, bool BF> hipError_t launch_ozm_t(mdft_ctx *c, typename Ty<T>::c *proj) {
     OzmArgs<T> a;
     a.ntiles = c->pairs.ntiles;
     a.per = (c->pairs.ntiles + 7) / 8;
     a.tpair = c->pairs.tpair;
     a.pmup = c->d_pmup;
     a.frag = static_cast<const T *>(c->d_ozm_frag);
+    a.frag16 = static_cast<const bf16x8 *>(c->d_ozm_frag16);
+    a.colent = c->d_ozm_col;
     a.ftab = static_cast<const T *>(c->d_ozm_ftab);
     a.units = reinterpret_cast<const int4 *>(c->d_ozm_units);
     a.rlen = c->d_ozm_rlen;
@@ -596,19 +655,19 @@ template <typename T, int L> hipError_t launch_ozm_t(mdft_ctx *c, typename Ty<T>
     if (a.ntiles == 0) return hipSuccess;
     const size_t lds = sizeof(Lds<T, L>);
     if (lds > c->lds_limit) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((ozm<T, L>), dim3((unsigned)(8 * a.per)), dim3(64 * kWaves), lds, c->stream, proj, a);
+    hipLaunchKernelGGL((ozm<T, L, BF>), dim3((unsigned)(8 * a.per)), dim3(64 * kWaves), lds, c->stream, proj, a);
     return hipGetLastError();
 }
 
-template <typename T> hipError_t launch_ozm_any(mdft_ctx *c, typename Ty<T>::c *proj) {
+template <typename T, bool BF> hipError_t launch_ozm_any(mdft_ctx *c, typename Ty<T>::c *proj) {
     switch (c->L) {
-        case 0: return launch_ozm_t<T, 0>(c, proj);
-        case 1: return launch_ozm_t<T, 1>(c, proj);
-        case 2: return launch_ozm_t<T, 2>(c, proj);
-        case 3: return launch_ozm_t<T, 3>(c, proj);
-        case 4: return launch_ozm_t<T, 4>(c, proj);
-        case 5: return launch_ozm_t<T, 5>(c, proj);
-        case 6: return launch_ozm_t<T, 6>(c, proj);
+        case 0: return launch_ozm_t<T, 0, BF>(c, proj);
+        case 1: return launch_ozm_t<T, 1, BF>(c, proj);
+        case 2: return launch_ozm_t<T, 2, BF>(c, proj);
+        case 3: return launch_ozm_t<T, 3, BF>(c, proj);
+        case 4: return launch_ozm_t<T, 4, BF>(c, proj);
+        case 5: return launch_ozm_t<T, 5, BF>(c, proj);
+        case 6: return launch_ozm_t<T, 6, BF>(c, proj);
         default: return hipErrorInvalidConfiguration;
     }
 }
@@ -756,6 +815,13 @@ int ozm_setup(mdft_ctx *c) {
     if (hipMemcpy(c->d_ozm_units, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_ozm_rlen, rlen.data(), sizeof(int) * rlen.size(), hipMemcpyHostToDevice) != hipSuccess)
         return MDFT_EHIP;
+    // OZ column entries of each block (entry << 1 | own side; padding columns read entry 0), 32 per block
+    std::vector<int> colent;
+    for (int khi = -L; khi <= L; khi++)
+        for (int cc = 0; cc < 32; cc++) colent.push_back(cc < h.ncol[khi + L] ? h.col[khi + L][cc] : 1);
+    if (hipMalloc((void **)&c->d_ozm_col, sizeof(int) * colent.size()) != hipSuccess) return MDFT_ENOMEM;
+    if (hipMemcpy(c->d_ozm_col, colent.data(), sizeof(int) * colent.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return MDFT_EHIP;
     return MDFT_OK;
 }
 
@@ -787,9 +853,40 @@ int ozm_frags(mdft_ctx *c, const std::vector<double2> &tab, int nq, int npc, con
                     }
                 }
         }
-    return c->d.precision == MDFT_PREC_FP64 ? upload_t<double>(c, &c->d_ozm_frag, fr)
-                                              : upload_t<float>(c, &c->d_ozm_frag, fr);
+    int rc = c->d.precision == MDFT_PREC_FP64 ? upload_t<double>(c, &c->d_ozm_frag, fr)
+                                                : upload_t<float>(c, &c->d_ozm_frag, fr);
+    if (rc != MDFT_OK || c->d.precision != MDFT_PREC_BF16) return rc;
+    // BF16 fragments of v_mfma_f32_16x16x32_bf16: lane l, value j: column 16 kb + 4 (l >> 4) + (j >> 1),
+    // re/im of V = j & 1, OZ row 4 nb + ((l & 15) >> 2), q = l & 3
+    std::vector<__bf16> f16((size_t)nint * h.nfrag32 * 64 * 8, (__bf16)0.0f);
+    for (int ii = 0; ii < nint; ii++)
+        for (int khi = -L; khi <= L; khi++) {
+            const int kk = khi + L, nkb = h.nkb32[kk], nnb = h.nnb[kk], ncols = h.ncol[kk];
+            for (int kb = 0; kb < nkb; kb++)
+                for (int nb = 0; nb < nnb; nb++) {
+                    const size_t f = (size_t)ii * h.nfrag32 + h.foff32[kk] + kb * nnb + nb;
+                    for (int l = 0; l < 64; l++)
+                        for (int j = 0; j < 8; j++) {
+                            const int cc = 16 * kb + 4 * (l >> 4) + (j >> 1), pk = j & 1;
+                            const int r = 4 * nb + ((l & 15) >> 2), q = l & 3;
+                            const int rowent = h.row[kk][r];
+                            if (cc >= ncols || rowent < 0) continue;
+                            const double2 v = tab[(size_t)(ii + (q >> 1)) * npc + ia0[rowent] + cc];
+                            const int qc = q & 1;
+                            const double x = pk == 0 ? (qc == 0 ? v.x : v.y) : (qc == 0 ? -v.y : v.x);
+                            f16[(f * 64 + l) * 8 + j] = (__bf16)(float)x;
+                        }
+                }
+        }
+    if (c->d_ozm_frag16) (void)hipFree(c->d_ozm_frag16);
+    c->d_ozm_frag16 = nullptr;
+    if (hipMalloc(&c->d_ozm_frag16, sizeof(__bf16) * f16.size()) != hipSuccess) return MDFT_ENOMEM;
+    if (hipMemcpy(c->d_ozm_frag16, f16.data(), sizeof(__bf16) * f16.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return MDFT_EHIP;
+    return MDFT_OK;
 }
 
-hipError_t launch_ozm_f64(mdft_ctx *c, double2 *proj) { return launch_ozm_any<double>(c, proj); }
-hipError_t launch_ozm_f32(mdft_ctx *c, float2 *proj) { return launch_ozm_any<float>(c, proj); }
+hipError_t launch_ozm_f64(mdft_ctx *c, double2 *proj) { return launch_ozm_any<double, false>(c, proj); }
+hipError_t launch_ozm_f32(mdft_ctx *c, float2 *proj) {
+    return c->d.precision == MDFT_PREC_BF16 ? launch_ozm_any<float, true>(c, proj) : launch_ozm_any<float, false>(c, proj);
+}

@@ -92,6 +92,8 @@ struct mdft_ctx {
     float *d_oz32_ftab = nullptr; // Fourier coefficients of d^m(theta), FP32 q-space kernel
     int *d_oz32_lanerow = nullptr; // descriptor table of the FP32 q-space kernel (rotation offsets, lane rows, OZ columns)
     void *d_ozm_frag = nullptr;   // matrix-core q-space kernel: c(k) B fragments per interval (FP64 or FP32)
+    void *d_ozm_frag16 = nullptr; // BF16 contexts: the same fragments in BF16, 32 k values per instruction
+    int *d_ozm_col = nullptr;     // OZ column entries of each khi block
     void *d_ozm_ftab = nullptr;   // its Fourier coefficients of d^m(theta)
     int *d_ozm_units = nullptr;   // its rotation unit rounds [round][8] int4
     int *d_ozm_rlen = nullptr;    // iterations of each round
