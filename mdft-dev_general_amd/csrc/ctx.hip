@@ -520,7 +520,7 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
                     c->d_ozm_frag, c->d_ozm_frag16, c->d_ozm_col, c->d_ozm_ftab, c->d_ozm_units, c->d_ozm_rlen,
-                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart};
+                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart, c->d_vext32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);
@@ -565,6 +565,7 @@ int mdft_set_vext(mdft_ctx *c, const double *vext, int flags) {
                              (flags & MDFT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_vext = true;
+    c->vext32_ok = false;
     return MDFT_OK;
 }
 
@@ -739,6 +740,13 @@ static int evaluate(mdft_ctx *c, const double *xi, double *df, int flags, int mo
             dd = c->d_df_stage;
         }
     }
+    if (x32 && df32 && dot_k > 0 && c->d.precision == MDFT_PREC_FP32 && !c->vext32_ok) {
+        // the fast minimiser's final pass reads vext in FP32 (its FP32 terms round vext to FP32
+        // first, so the copy changes no result): made once per vext
+        if (!c->d_vext32) HIPCHK(c, hipMalloc(&c->d_vext32, sizeof(float) * c->N * c->no));
+        HIPCHK(c, vec_narrow(c, c->N * c->no, c->d_vext, c->d_vext32));
+        c->vext32_ok = true;
+    }
     c->vsinv = scaled ? c->d_vscale + c->no : nullptr;
     c->xi32 = x32 ? reinterpret_cast<const float *>(xi) : nullptr;
     rc = run_excess_pipeline(c, xd, 4, timers);
@@ -907,6 +915,7 @@ int mdft_compute_vext_ex(mdft_ctx *c, int natom, const double *atoms, int nsite,
     (void)hipFree(d_at);
     HIPCHK(c, e);
     c->have_vext = true;
+    c->vext32_ok = false;
     return MDFT_OK;
 }
 

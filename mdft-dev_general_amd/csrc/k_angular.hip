@@ -30,6 +30,12 @@ template <> struct C2<double> { using t = double2; };
 template <> struct C2<float> { using t = float2; };
 template <typename V> __device__ inline V mk(decltype(V::x) a, decltype(V::x) b) { V v; v.x = a; v.y = b; return v; }
 
+// An FP32 value held in the low word of an FP64 register pair.  The tile prefetches keep the
+// loaded word as it is and convert it where it is used: a conversion next to each load would
+// make the wave wait for every load before issuing the next, one HBM latency per element.
+__device__ inline double hold_f32(float v) { return __longlong_as_double((long long)__float_as_uint(v)); }
+__device__ inline double held_f32(double h) { return (double)__uint_as_float((unsigned)__double_as_longlong(h)); }
+
 template <int PT>
 __global__ __launch_bounds__(kThreads) void angl2proj_f64(const double *__restrict__ xi, double2 *__restrict__ proj,
                                                          size_t N, AngTables t, double rho0) {
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
         const int i = tid + q * kThreads;
-        xr[q] = (i < npts * NO) ? (xi32 ? (double)xi32[r0 * NO + i] : src[i]) : 0.0;  // FP32 iterate: xi32
+        xr[q] = (i < npts * NO) ? (xi32 ? hold_f32(xi32[r0 * NO + i]) : src[i]) : 0.0;  // FP32 iterate: xi32
         sr[q] = vsinv ? vsinv[i % NO] : 1.0;  // scaled variables: xi = u / s(io), loaded with the tile
     }
     for (int i = tid; i < H1 * NPSI; i += kThreads) {
@@ -376,7 +382,8 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     for (int q = 0; q < NIT; q++) {
         const int i = tid + q * kThreads;
         if (i >= npts * NO) break;
-        const double x = vsinv ? xr[q] * sr[q] : xr[q];
+        const double xv = xi32 ? held_f32(xr[q]) : xr[q];
+        const double x = vsinv ? xv * sr[q] : xv;
         if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
         else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
         else if (feat == 2) o[i] = (T)(x * x - 1.0);
@@ -440,17 +447,25 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     }
 }
 
-template <int L, int PT, typename T>
+// DOT32: mode 0 with an FP32 iterate, forming the gradient products over FP32 vectors (the
+// evaluations of the fast minimiser), on full tiles only (N a multiple of PT).  Every load of the tile is then issued on
+// every path (the last partial round of a thread's elements with its index clamped into the
+// tile), and dF is stored on every path: the wait before each use counts exactly the memory
+// operations issued after it, so the product operands loaded ahead stay in flight.
+template <int L, int PT, typename T, bool DOT32 = false>
 // 7 waves per SIMD (the LDS limit at mmax = 5, FP32): caps the registers the prefetch below adds
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
+// at 72, and keeps the compiler from spilling to reach 8
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode, float *__restrict__ df32,
                                                    const double *__restrict__ vsinv, DotArgs da,
-                                                   double *__restrict__ dpart, const float *__restrict__ xi32) {
+                                                   double *__restrict__ dpart, const float *__restrict__ xi32,
+                                                   const float *__restrict__ vext32) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
+    if constexpr (DOT32) mode = 0;
     __shared__ V Gm[PT * NP];
     __shared__ V f[PT * NTH * W * H1];
     __shared__ V h[PT * NTH * NPHI * H1];
@@ -462,22 +477,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     __shared__ int pidx[(L + 1) * W * H1];
     const int tid = threadIdx.x;
     const size_t r0 = (size_t)blockIdx.x * PT;
-    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
-    // the tile's FP64 xi and vext go into registers first: their HBM latency then
-    // overlaps the angular synthesis below instead of following it
-    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
-    double xr[NIT], vr[NIT];
-#pragma unroll
-    for (int q = 0; q < NIT; q++) {
-        const int i = tid + q * kThreads;
-        xr[q] = 0.0;
-        vr[q] = 0.0;
-        if (i < npts * NO) {
-            const size_t g = r0 * NO + i;
-            if (mode < 3) xr[q] = xi32 ? (double)xi32[g] : xi[g];  // FP32 iterate: xi32
-            if (mode == 0 || mode == 3 || mode == 4) vr[q] = vext[g];
-        }
-    }
+    const int npts = DOT32 ? PT : (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
     for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
     for (int i = tid; i < NPHI * W; i += kThreads) {
         const int iphi = i / W, mup = i % W - L;
@@ -494,6 +494,29 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     for (int i = tid; i < NTH; i += kThreads) sth[i] = vsinv ? vsinv[i * NPHI * NPSI] : 1.0;
     const V *src = proj + r0 * NP;
     for (int i = tid; i < npts * NP; i += kThreads) Gm[i] = src[i];
+    // the tile's xi and vext go into registers after the staging above (each staged element
+    // waits for its own load, and such a wait drains every load issued before it): their HBM
+    // latency then overlaps the angular synthesis below instead of following it
+    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
+    double xr[NIT], vr[NIT];
+    float xr32[NIT], vr32[NIT];  // DOT32: the FP32 iterate and the FP32 copy of vext
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+        const int i = tid + q * kThreads;
+        if constexpr (DOT32) {
+            const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
+            xr32[q] = xi32[g];
+            vr32[q] = vext32[g];
+            continue;
+        }
+        xr[q] = 0.0;
+        vr[q] = 0.0;
+        if (i < npts * NO) {
+            const size_t g = r0 * NO + i;
+            if (mode < 3) xr[q] = xi32 ? hold_f32(xi32[g]) : xi[g];  // FP32 iterate: xi32
+            if (mode == 0 || mode == 3 || mode == 4) vr[q] = vext[g];
+        }
+    }
     __syncthreads();
     // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m): one thread per
     // (point, mup, mu2), all theta at once (each projection read once)
@@ -546,10 +569,40 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
     double fid = 0.0, fext = 0.0, fexc = 0.0;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;  // products of dF with the DotArgs vectors
     const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
+    // DOT32: FP32 operands of the first two gradient products, loaded two elements ahead of
+    // their use (a load next to its use would make every element wait for HBM); an operand that
+    // is not needed (the gradient itself, or beyond da.k) is read from dF, which exists here
+    constexpr int kAhead = 2;
+    float dq[2][NIT];
+    const float *dp0 = (da.k > 0 && !(da.self & 1)) ? static_cast<const float *>(da.p[0]) : df32;
+    const float *dp1 = (da.k > 1 && !((da.self >> 1) & 1)) ? static_cast<const float *>(da.p[1]) : df32;
+    auto load_dq = [&](int q) {
+        const int i = tid + q * kThreads;
+        const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
+        dq[0][q] = dp0[g];
+        dq[1][q] = dp1[g];
+    };
+    if constexpr (DOT32) {
+#pragma unroll
+        for (int q = 0; q < kAhead && q < NIT; q++) load_dq(q);
+    }
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
+        if constexpr (DOT32) {
+            if (q + kAhead < NIT) load_dq(q + kAhead);
+        }
         const int i = tid + q * kThreads;
         if (i >= npts * NO) break;
+        double xq;
+        float ve32 = 0.0f;
+        if constexpr (DOT32) {
+            float xv = xr32[q];
+            ve32 = vr32[q];
+            asm volatile("" : "+v"(xv), "+v"(ve32));  // the conversion stays here, not next to the load
+            xq = (double)xv;
+        } else {
+            xq = xi32 ? held_f32(xr[q]) : xr[q];
+        }
         const int pt = i / NO, io = i - pt * NO;
         const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
         const V *hr = h + (pt * NTH * NPHI + line) * H1;
@@ -567,7 +620,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                 // are FP32 already), summed in FP64; rho/rho0 - 1 is formed in FP64 first so
                 // that the excess term keeps its small deviations from the bulk
                 const double si = vsinv ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
-                const double x = xr[q] * si;
+                const double x = xq * si;
                 const double xx = x * x;
                 const float wf = (float)wq[io];
                 const float vexcf = (float)k.prefactor * wf * vt;
@@ -576,8 +629,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                 fexc += (double)(rho0f * drf * vexcf);
                 float gid = 0.0f;
                 if (xx * rho0 > DBL_EPSILON) {
-                    const float lg = logf(x2);  // ln(rho / rho0)
-                    const float ve = (float)vr[q];
+                    // ln(rho / rho0) from the hardware base-2 logarithm: x2 > 6e-15 here (rho > DBL_EPSILON),
+                    // a normal FP32 number, so logf's denormal scaling and its extra-precise
+                    // product with ln 2 change nothing at this path's FP32 accuracy
+                    const float lg = __builtin_amdgcn_logf(x2) * 0.693147181f;
+                    const float ve = DOT32 ? ve32 : (float)vr[q];
                     const float kTf = (float)kT, dvf = (float)dv;
                     fid += (double)(kTf * dvf * wf * rho0f * (x2 * lg - drf));
                     fext += (double)(dvf * wf * rho0f * x2 * ve);
@@ -586,6 +642,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
                     fid += kT * dv * wq[io] * rho0;
                 }
                 const float gv = (gid + 2.0f * rho0f * xf * vexcf) * (float)si;
+                if constexpr (DOT32) {
+                    df32[g] = gv;
+                    const double gd = (double)gv;
+                    // the empty asm keeps the conversions here: hoisted next to the loads above,
+                    // they would make each load wait
+                    float o0 = dq[0][q], o1 = dq[1][q];
+                    asm volatile("" : "+v"(o0), "+v"(o1));
+                    d0 += gd * ((da.self & 1) ? gd : (double)o0);
+                    if (da.k > 1) d1 += gd * (((da.self >> 1) & 1) ? gd : (double)o1);
+                    if (da.k > 2) d2 += gd * (((da.self >> 2) & 1) ? gd : (double)static_cast<const float *>(da.p[2])[g]);
+                    continue;
+                }
                 if (df32) df32[g] = gv;
                 else if (df) df[g] = (double)gv;
                 if (da.k > 0) {
@@ -601,7 +669,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8))
         const double wio = wq[io];
         const double vexc = k.prefactor * wio * v;
         const double si = (mode == 0 && vsinv) ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables (mode 0)
-        const double x = xr[q] * si;
+        const double x = xq * si;
         const double rho = x * x * rho0;
         fexc += (x * x * rho0 - rho0) * vexc;
         if (mode == 0) {
@@ -692,9 +760,14 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
         da.f32 = c->df32 != nullptr;
         for (int j = 0; j < 3; j++) da.p[j] = c->dot_vec[j];
     }
+    // the FP32 kernel forming products over FP32 vectors (dF then FP32 too) has its own variant
+    const bool dot32 = sizeof(T) == 4 && mode == 0 && da.k > 0 && da.f32 && c->xi32 && c->vext32_ok && npts % PT == 0;
     switch (c->L) {
 #define FIN_CASE(LL) \
-    case LL: hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr, da, c->d_dpart, mode == 0 ? c->xi32 : nullptr); return true;
+    case LL: \
+        if (dot32) hipLaunchKernelGGL((final_t<LL, PT, T, sizeof(T) == 4>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, c->df32, c->vsinv, da, c->d_dpart, c->xi32, c->d_vext32); \
+        else hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr, da, c->d_dpart, mode == 0 ? c->xi32 : nullptr, nullptr); \
+        return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
         default: return false;

@@ -120,6 +120,8 @@ struct mdft_ctx {
     int dot_k = 0, dot_self = 0;
     const void *dot_vec[3] = {nullptr, nullptr, nullptr};
     double *d_dpart = nullptr;    // [blocks][3] their block partials
+    float *d_vext32 = nullptr;    // FP32 copy of vext read by the fast minimiser's final pass
+    bool vext32_ok = false;       // d_vext32 holds the current vext
     size_t dpart_cap = 0;
     float *d_chscale = nullptr;   // per-channel input scale of the warm-start features [2 np]
     float *d_adjscale = nullptr;  // per-channel factor of the adjoint of proj2angl [2 np]

@@ -407,6 +407,57 @@ def test_gradient_products_in_the_evaluation(precision, df32):
     assert np.allclose(dots, ref, rtol=1e-12, atol=1e-12 * np.abs(g).max() * np.sqrt(n))
 
 
+@pytest.mark.parametrize("n", [(16, 16, 16), (15, 15, 15)])
+def test_gradient_products_with_fp32_iterate(n):
+    """The fast minimiser's evaluation (FP32 iterate, FP32 gradient, gradient products over
+    FP32 vectors, scaled variables) has its own final-pass variant on grids of whole 4-point
+    tiles, reading an FP32 copy of vext: F and the gradient equal, to the last bit, those of
+    the evaluation without products (the general variant), the products match numpy, and a
+    new vext reaches the copy.  15^3 (not whole tiles) takes the general variant."""
+    import ctypes as C
+    from mdft.gpu import DEVICE_PTRS, DF_FP32, SCALED_VARS, XI_FP32, DeviceBuffer, Engine
+    w = workloads.small("solute20_128", 16)
+    say(f"[dots fp32 iterate] {n} fp32")
+    eng = Engine(n, w.length, w.mmax, 2, w.kT, w.n0, "fp32")
+    eng.set_ck(w.ck_table())
+    eng.compute_vext(w.solute(), w.solvent)
+    rng = np.random.default_rng(5)
+    nv = eng.N * eng.no
+    base = workloads.guess_xi(eng.get_vext(), w.kT) * (1 + 0.05 * rng.standard_normal(nv))
+    xi = DeviceBuffer(eng, 4 * nv)
+    xi.upload(base.astype(np.float32))
+    flags = DEVICE_PTRS | DF_FP32 | XI_FP32 | SCALED_VARS
+    dp = C.POINTER(C.c_double)
+    vs = [rng.standard_normal(nv).astype(np.float32) for _ in range(2)]
+    bv = []
+    for v in vs:
+        b = DeviceBuffer(eng, v.nbytes)
+        b.upload(v)
+        bv.append(b)
+    vext0 = eng.get_vext()
+    for step, vext in enumerate((None, 1.1 * vext0)):
+        if vext is not None:
+            eng.set_vext(vext)   # after the FP32 copy was made: the next evaluation must see it
+        g0 = DeviceBuffer(eng, 4 * nv)
+        p0 = np.zeros(3)
+        eng._check(eng.lib.mdft_energy_and_gradient(eng._h, xi.ptr, p0.ctypes.data_as(dp), g0.ptr, flags))
+        g1 = DeviceBuffer(eng, 4 * nv)
+        ptrs = (C.c_void_p * 3)(bv[0].ptr.value, g1.ptr.value, bv[1].ptr.value)
+        p1, dots = np.zeros(3), np.zeros(3)
+        eng._check(eng.lib.mdft_energy_and_gradient_dots(eng._h, xi.ptr, p1.ctypes.data_as(dp), g1.ptr, flags, 3,
+                                                         ptrs, dots.ctypes.data_as(dp)))
+        ga, gb = g0.download(np.float32), g1.download(np.float32)
+        say(f"  vext {step}: F parts {p1}, max |dg| {np.abs(ga - gb).max():.3g}")
+        assert np.array_equal(p1, p0)
+        assert np.array_equal(ga, gb)
+        g = gb.astype(np.float64)
+        ref = np.array([g @ vs[0].astype(np.float64), g @ g, g @ vs[1].astype(np.float64)])
+        assert np.allclose(dots, ref, rtol=1e-12, atol=1e-12 * np.abs(g).max() * np.sqrt(nv))
+        if step == 0:
+            f_first = p1.copy()
+    assert not np.array_equal(p1, f_first)   # the external term changed with vext
+
+
 def test_fp32_iterate_evaluation_and_minimisation():
     """MDFT_XI_FP32: an evaluation that reads xi as float gives the evaluation of the FP64
     array holding the same values, to the last bit; the minimiser with its iterate in FP32
