@@ -13,12 +13,14 @@ knows nothing of the answer; each is then checked against Delta G_ref:
   solute and 1 elsewhere; FP64 evaluations; device-resident L-BFGS with memory
   1 over xi, unit first step) run to |dF|/|dF0| < 1e-6; its final F.
 * plain FP64 path (the denominator): the same method stopped by the rule, T_plain.
-* fast path (`value`): the learned warm start (features of vext, network
-  inference and the guess mixed with the cold start, inside the timed region),
-  FP32 evaluations (angular transforms, FFTs, OZ and per-orientation terms in
-  FP32; sums in FP64), L-BFGS with FP64 scalars, FP32 vectors, the weight-scaled
-  variables and a bulk Newton first step, stopped by the rule, plus one FP64
-  evaluation of the final iterate (the confirmed Delta G).
+* fast path (`value`): the calibrated warm start inside the timed region (the mix
+  (1 - lam) cold start + lam network prediction; lam is calibrated per network on
+  other solutes, and at this workload lam = 0: the reference's cold start, the
+  network not run), FP32 evaluations (angular transforms, FFTs, OZ and
+  per-orientation terms in FP32; sums in FP64), L-BFGS with FP64 scalars, FP32
+  vectors and iterate, the weight-scaled variables and a bulk Newton first step,
+  stopped by the rule, plus one FP64 evaluation of the final iterate (the
+  confirmed Delta G).
   speedup_band_to_band = T_plain / value.
 
 Also reported: both paths stopped by an oracle at the first iterate within the
@@ -26,8 +28,9 @@ band (what the stop rule costs), the plain run to full convergence, the FP64
 method over the scaled variables, the FP32 path from the cold, Boltzmann and
 pure-network starts, F+dF evaluations per second (one step = one evaluation over
 the whole grid, inputs resident in HBM), per-phase HIP-event timings on the
-library's stream, the roofline of the dominant kernel, and the reference's own
-CPU evaluation (compiled from its sources) timed on the host cores.
+library's stream, the roofline of the dominant kernel of the fast path's own
+evaluation, and the reference's own CPU evaluation (compiled from its sources)
+timed on the host cores.
 """
 import argparse
 import json
@@ -48,15 +51,19 @@ METRIC = "s to ΔG_solv within 0.05 kJ/mol, 128³ mmax=5: learned+mixed vs plain
 PHASES = ["angl2proj", "fft_forward", "oz", "fft_inverse", "proj2angl_ideal_grad", "reductions", "total"]
 
 
-def phase_bytes(N, no, np_, b=16):
-    """Algorithmic HBM bytes of each phase of one evaluation (complex of b bytes)."""
+def phase_bytes(N, no, np_, b=16, fast=False):
+    """Algorithmic HBM bytes of each phase of one evaluation (complex of b bytes).
+    fast: the fast minimiser's evaluation -- FP32 iterate (4 B), the final pass reading the
+    FP32 copy of vext (4 B) and two FP32 vectors for the gradient products (8 B) and
+    writing the FP32 gradient (4 B); otherwise FP64 xi and vext read, FP64 dF written."""
     pb = N * np_ * b
+    x = 4 if fast else 8
     return {
-        "angl2proj": N * no * 8 + pb,
+        "angl2proj": N * no * x + pb,
         "fft_forward": 3 * 2 * pb,            # three axis passes, read + write
         "oz": 2 * pb,
         "fft_inverse": 3 * 2 * pb,
-        "proj2angl_ideal_grad": pb + N * no * 8 * 3,
+        "proj2angl_ideal_grad": pb + N * no * (20 if fast else 24),
     }
 
 
@@ -68,14 +75,45 @@ KERNELS = {
     "proj2angl_ideal_grad": ("final_t (proj2angl + ideal + gradient)", 1, "final_t"),
 }
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_eval_fp32_solute20_128_mmax5.md")
+TRAFFIC_PROFILE_FAST = os.path.join(ROOT, "profiles", "traffic_fastpath_solute20_128_mmax5.md")
 
 
-def measured_traffic(prefix):
+def fast_eval_phases(e32, xi, reps=3):
+    """Median per-phase HIP-event times (ms) of the fast minimiser's own evaluation --
+    FP32 iterate u = s xi, FP32 gradient, scaled variables, the products of the new
+    gradient with two FP32 vectors and itself formed in the final pass -- at xi."""
+    import ctypes as C
+    from mdft.gpu import DEVICE_PTRS, DF_FP32, SCALED_VARS, TIMERS, XI_FP32, DeviceBuffer
+    n = e32.N * e32.no
+    u = DeviceBuffer(e32, 8 * n)
+    e32._check(e32.lib.mdft_scale_variables(e32._h, n, 1, 0, xi.ptr, u.ptr))
+    u32, g32, v0, v1 = (DeviceBuffer(e32, 4 * n) for _ in range(4))
+    e32._check(e32.lib.mdft_vec_narrow_f64(e32._h, n, u.ptr, u32.ptr))
+    u.free()
+    flags = DEVICE_PTRS | DF_FP32 | XI_FP32 | SCALED_VARS
+    dp = C.POINTER(C.c_double)
+    parts, dots = np.zeros(3), np.zeros(3)
+    e32._check(e32.lib.mdft_energy_and_gradient(e32._h, u32.ptr, parts.ctypes.data_as(dp), g32.ptr, flags))
+    for v in (v0, v1):   # two stored vectors of the gradient's size and scale
+        e32._check(e32.lib.mdft_vec_axpy_f32x(e32._h, n, 0.0, g32.ptr, g32.ptr, v.ptr))
+    vecs = (C.c_void_p * 3)(v0.ptr.value, v1.ptr.value, g32.ptr.value)
+    ph = []
+    for _ in range(reps):
+        e32._check(e32.lib.mdft_energy_and_gradient_dots(e32._h, u32.ptr, parts.ctypes.data_as(dp), g32.ptr,
+                                                         flags | TIMERS, 3, vecs, dots.ctypes.data_as(dp)))
+        ph.append(e32.phase_times())
+    for b in (u32, g32, v0, v1):
+        b.free()
+    return np.median(np.array(ph), axis=0)
+
+
+def measured_traffic(prefix, fast=False):
     """HBM bytes per launch (fetched + written) of the kernel whose name starts with
-    `prefix` in the committed rocprofv3 PMC summary of the FP32 evaluation, or None."""
+    `prefix` in the committed rocprofv3 PMC summary of the FP32 evaluation (fast: of the
+    fast minimiser's evaluation), or None."""
     import re
     try:
-        txt = open(TRAFFIC_PROFILE).read()
+        txt = open(TRAFFIC_PROFILE_FAST if fast else TRAFFIC_PROFILE).read()
     except OSError:
         return None
     for line in txt.splitlines():
@@ -85,11 +123,11 @@ def measured_traffic(prefix):
     return None
 
 
-def roofline_of(ph, N, no, np_, cbytes, traffic=False):
+def roofline_of(ph, N, no, np_, cbytes, traffic=False, fast=False):
     """Roofline of the dominant kernel from median phase times: per launch, the
     algorithmic bytes over the average launch duration (HIP events on the
     library's stream)."""
-    pbytes = phase_bytes(N, no, np_, cbytes)
+    pbytes = phase_bytes(N, no, np_, cbytes, fast)
     per = {"angl2proj": (ph[0], pbytes["angl2proj"]),
            "fft": (ph[1] + ph[3], pbytes["fft_forward"] + pbytes["fft_inverse"]),
            "oz": (ph[2], pbytes["oz"]),
@@ -98,7 +136,7 @@ def roofline_of(ph, N, no, np_, cbytes, traffic=False):
     name, launches, prof = KERNELS[dom]
     ms, nbytes = per[dom][0] / launches, per[dom][1] / launches
     achieved = nbytes / (ms * 1e-3) / 1e9
-    t = measured_traffic(prof) if traffic else None
+    t = measured_traffic(prof, fast) if traffic else None
     return {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": t, "algorithmic_bytes_per_launch": nbytes,
             "launch_ms": ms, "launches_per_evaluation": launches}
@@ -357,15 +395,12 @@ def main():
         else:
             t_fast, res_f, dg_fast = t_cold32, res_c32, dg_c32
             t_fast_band, res_fb, dg_fast_band = fast("cold", stop="band")
-        # the fast path's evaluations: phase times of the FP32 engine (HIP events on its
-        # stream) give the roofline of the dominant kernel of the timed region
-        ph32 = []
-        for _ in range(3):
-            e32.energy_and_gradient(xi, df, timers=True)
-            ph32.append(e32.phase_times())
-        ph32 = np.median(np.array(ph32), axis=0)
+        # the fast path's own evaluation (the variant its minimiser calls), timed by phase
+        # (HIP events on the FP32 engine's stream): the roofline of the dominant kernel of
+        # the timed region
+        ph32 = fast_eval_phases(e32, xi)
         line["roofline_fp64"] = line["roofline"]
-        line["roofline"] = roofline_of(ph32, e32.N, e32.no, e32.np_, 8, traffic=True)
+        line["roofline"] = roofline_of(ph32, e32.N, e32.no, e32.np_, 8, traffic=True, fast=True)
         line["phases_ms_fp32"] = {k: float(v) for k, v in zip(PHASES, ph32)}
         # reference-style solvation free energy of the fast path's final density
         # (src/module_pressure_correction.f90: F + PMV correction + LJ tail), outside the timed region

@@ -447,15 +447,21 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     }
 }
 
+// Persistent: each workgroup stages the tables once and then takes tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ...  A tile needs no barrier beyond its three: every LDS array a
+// tile writes was last read before a barrier that all threads have passed by then.  The
+// free-energy sums and gradient products accumulate over a workgroup's tiles and are
+// written once, one partial per workgroup.
 // DOT32: mode 0 with an FP32 iterate, forming the gradient products over FP32 vectors (the
-// evaluations of the fast minimiser), on full tiles only (N a multiple of PT).  Every load of the tile is then issued on
-// every path (the last partial round of a thread's elements with its index clamped into the
-// tile), and dF is stored on every path: the wait before each use counts exactly the memory
-// operations issued after it, so the product operands loaded ahead stay in flight.
+// evaluations of the fast minimiser), on full tiles only (N a multiple of PT).  Every load
+// of the tile is then issued on every path (the last partial round of a thread's elements
+// with its index clamped into the tile), and dF is stored on every path: the wait before
+// each use counts exactly the memory operations issued after it, so the product operands
+// loaded ahead stay in flight.
 template <int L, int PT, typename T, bool DOT32 = false>
-// 7 waves per SIMD (the LDS limit at mmax = 5, FP32): caps the registers the prefetch below adds
-// at 72, and keeps the compiler from spilling to reach 8
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
+// DOT32: 7 waves per SIMD (the LDS limit at mmax = 5, FP32), at most 72 registers; the general
+// variant, with every mode's branches, may take more registers rather than spill
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 ? 7 : 4, 7))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode, float *__restrict__ df32,
@@ -476,8 +482,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
     __shared__ double sth[NTH];  // 1 / s of each theta (the weights, hence s, depend on theta only)
     __shared__ int pidx[(L + 1) * W * H1];
     const int tid = threadIdx.x;
-    const size_t r0 = (size_t)blockIdx.x * PT;
-    const int npts = DOT32 ? PT : (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
     for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
     for (int i = tid; i < NPHI * W; i += kThreads) {
         const int iphi = i / W, mup = i % W - L;
@@ -492,213 +496,223 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
     for (int i = tid; i < NP * NTH; i += kThreads) p2a[i] = (T)t.p2a[i];
     for (int i = tid; i < NO; i += kThreads) wq[i] = t.w[i];
     for (int i = tid; i < NTH; i += kThreads) sth[i] = vsinv ? vsinv[i * NPHI * NPSI] : 1.0;
-    const V *src = proj + r0 * NP;
-    for (int i = tid; i < npts * NP; i += kThreads) Gm[i] = src[i];
-    // the tile's xi and vext go into registers after the staging above (each staged element
-    // waits for its own load, and such a wait drains every load issued before it): their HBM
-    // latency then overlaps the angular synthesis below instead of following it
-    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
-    double xr[NIT], vr[NIT];
-    float xr32[NIT], vr32[NIT];  // DOT32: the FP32 iterate and the FP32 copy of vext
-#pragma unroll
-    for (int q = 0; q < NIT; q++) {
-        const int i = tid + q * kThreads;
-        if constexpr (DOT32) {
-            const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
-            xr32[q] = xi32[g];
-            vr32[q] = vext32[g];
-            continue;
-        }
-        xr[q] = 0.0;
-        vr[q] = 0.0;
-        if (i < npts * NO) {
-            const size_t g = r0 * NO + i;
-            if (mode < 3) xr[q] = xi32 ? hold_f32(xi32[g]) : xi[g];  // FP32 iterate: xi32
-            if (mode == 0 || mode == 3 || mode == 4) vr[q] = vext[g];
-        }
-    }
-    __syncthreads();
-    // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m): one thread per
-    // (point, mup, mu2), all theta at once (each projection read once)
-    for (int i = tid; i < npts * W * H1; i += kThreads) {
-        const int mu2 = i % H1;
-        const int mup = (i / H1) % W - L, pt = i / (H1 * W);
-        const int a = mup < 0 ? -mup : mup, b = 2 * mu2;
-        const int mlo = a > b ? a : b;
-        T re[NTH], im[NTH];
-#pragma unroll
-        for (int it = 0; it < NTH; it++) re[it] = im[it] = (T)0;
-        for (int m = mlo; m <= L; m++) {
-            const int p = pidx[(m * W + mup + L) * H1 + mu2];
-            const V v = Gm[pt * NP + p];
-            const T *c = p2a + p * NTH;
-#pragma unroll
-            for (int it = 0; it < NTH; it++) {
-                re[it] += v.x * c[it];
-                im[it] += v.y * c[it];
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < NTH; it++) f[((pt * NTH + it) * W + (mup + L)) * H1 + mu2] = mk<V>(re[it], im[it]);
-    }
-    __syncthreads();
-    for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
-        const int mu2 = i % H1;
-        int rest = i / H1;
-        const int iphi = rest % NPHI;
-        rest /= NPHI;
-        const V *fr = f + rest * W * H1 + mu2;
-        const V *tw = twphi + iphi * W;
-        T re = 0, im = 0;
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-            const V v = fr[j * H1], e = tw[j];
-            if constexpr (sizeof(T) == 4) {  // four multiply-adds (FP32 only, as in angl2proj_t)
-                re = fma(v.x, e.x, re);
-                re = fma(v.y, e.y, re);
-                im = fma(v.x, e.y, im);
-                im = fma(-v.y, e.x, im);
-            } else {
-                re += v.x * e.x + v.y * e.y;
-                im += v.x * e.y - v.y * e.x;
-            }
-        }
-        h[i] = mk<V>(re, im);
-    }
-    __syncthreads();
     double fid = 0.0, fext = 0.0, fexc = 0.0;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;  // products of dF with the DotArgs vectors
     const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
-    // DOT32: FP32 operands of the first two gradient products, loaded two elements ahead of
-    // their use (a load next to its use would make every element wait for HBM); an operand that
-    // is not needed (the gradient itself, or beyond da.k) is read from dF, which exists here
-    constexpr int kAhead = 2;
-    float dq[2][NIT];
-    const float *dp0 = (da.k > 0 && !(da.self & 1)) ? static_cast<const float *>(da.p[0]) : df32;
-    const float *dp1 = (da.k > 1 && !((da.self >> 1) & 1)) ? static_cast<const float *>(da.p[1]) : df32;
-    auto load_dq = [&](int q) {
-        const int i = tid + q * kThreads;
-        const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
-        dq[0][q] = dp0[g];
-        dq[1][q] = dp1[g];
-    };
-    if constexpr (DOT32) {
-#pragma unroll
-        for (int q = 0; q < kAhead && q < NIT; q++) load_dq(q);
-    }
-#pragma unroll
-    for (int q = 0; q < NIT; q++) {
-        if constexpr (DOT32) {
-            if (q + kAhead < NIT) load_dq(q + kAhead);
+    const size_t ntiles = (N + PT - 1) / PT;
+    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        // the thread index, opaque to the compiler in each tile: the tile's indices and LDS
+        // addresses depend on the thread only, and hoisted out of the tile loop they would
+        // all stay in registers (and spill)
+        int tl = tid;
+        asm volatile("" : "+v"(tl));
+        const size_t r0 = tile * PT;
+        const int npts = DOT32 ? PT : (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+        const V *src = proj + r0 * NP;
+        for (int i = tl; i < npts * NP; i += kThreads) Gm[i] = src[i];
+        __syncthreads();
+        // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m): one thread per
+        // (point, mup, mu2), all theta at once (each projection read once)
+        for (int i = tl; i < npts * W * H1; i += kThreads) {
+            const int mu2 = i % H1;
+            const int mup = (i / H1) % W - L, pt = i / (H1 * W);
+            const int a = mup < 0 ? -mup : mup, b = 2 * mu2;
+            const int mlo = a > b ? a : b;
+            T re[NTH], im[NTH];
+    #pragma unroll
+            for (int it = 0; it < NTH; it++) re[it] = im[it] = (T)0;
+            for (int m = mlo; m <= L; m++) {
+                const int p = pidx[(m * W + mup + L) * H1 + mu2];
+                const V v = Gm[pt * NP + p];
+                const T *c = p2a + p * NTH;
+    #pragma unroll
+                for (int it = 0; it < NTH; it++) {
+                    re[it] += v.x * c[it];
+                    im[it] += v.y * c[it];
+                }
+            }
+    #pragma unroll
+            for (int it = 0; it < NTH; it++) f[((pt * NTH + it) * W + (mup + L)) * H1 + mu2] = mk<V>(re[it], im[it]);
         }
-        const int i = tid + q * kThreads;
-        if (i >= npts * NO) break;
-        double xq;
-        float ve32 = 0.0f;
-        if constexpr (DOT32) {
-            float xv = xr32[q];
-            ve32 = vr32[q];
-            asm volatile("" : "+v"(xv), "+v"(ve32));  // the conversion stays here, not next to the load
-            xq = (double)xv;
-        } else {
-            xq = xi32 ? held_f32(xr[q]) : xr[q];
-        }
-        const int pt = i / NO, io = i - pt * NO;
-        const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
-        const V *hr = h + (pt * NTH * NPHI + line) * H1;
-        const V *tp = twpsi + ipsi * H1;
-        T vt = hr[0].x;
-#pragma unroll
-        for (int mu2 = 1; mu2 < H1; mu2++) {
-            if constexpr (sizeof(T) == 4) vt = fma(-hr[mu2].y, tp[mu2].y, fma(hr[mu2].x, tp[mu2].x, vt));
-            else vt += hr[mu2].x * tp[mu2].x - hr[mu2].y * tp[mu2].y;
-        }
-        const size_t g = r0 * NO + i;
-        if constexpr (sizeof(T) == 4) {
-            if (mode == 0) {
-                // FP32 path: the per-orientation terms in FP32 (the transforms feeding them
-                // are FP32 already), summed in FP64; rho/rho0 - 1 is formed in FP64 first so
-                // that the excess term keeps its small deviations from the bulk
-                const double si = vsinv ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
-                const double x = xq * si;
-                const double xx = x * x;
-                const float wf = (float)wq[io];
-                const float vexcf = (float)k.prefactor * wf * vt;
-                const float drf = (float)(xx - 1.0);
-                const float rho0f = (float)rho0, x2 = (float)xx, xf = (float)x;
-                fexc += (double)(rho0f * drf * vexcf);
-                float gid = 0.0f;
-                if (xx * rho0 > DBL_EPSILON) {
-                    // ln(rho / rho0) from the hardware base-2 logarithm: x2 > 6e-15 here (rho > DBL_EPSILON),
-                    // a normal FP32 number, so logf's denormal scaling and its extra-precise
-                    // product with ln 2 change nothing at this path's FP32 accuracy
-                    const float lg = __builtin_amdgcn_logf(x2) * 0.693147181f;
-                    const float ve = DOT32 ? ve32 : (float)vr[q];
-                    const float kTf = (float)kT, dvf = (float)dv;
-                    fid += (double)(kTf * dvf * wf * rho0f * (x2 * lg - drf));
-                    fext += (double)(dvf * wf * rho0f * x2 * ve);
-                    gid = wf * 2.0f * rho0f * xf * (kTf * lg + ve);
+        __syncthreads();
+        for (int i = tl; i < npts * NTH * NPHI * H1; i += kThreads) {
+            const int mu2 = i % H1;
+            int rest = i / H1;
+            const int iphi = rest % NPHI;
+            rest /= NPHI;
+            const V *fr = f + rest * W * H1 + mu2;
+            const V *tw = twphi + iphi * W;
+            T re = 0, im = 0;
+    #pragma unroll
+            for (int j = 0; j < W; j++) {
+                const V v = fr[j * H1], e = tw[j];
+                if constexpr (sizeof(T) == 4) {  // four multiply-adds (FP32 only, as in angl2proj_t)
+                    re = fma(v.x, e.x, re);
+                    re = fma(v.y, e.y, re);
+                    im = fma(v.x, e.y, im);
+                    im = fma(-v.y, e.x, im);
                 } else {
-                    fid += kT * dv * wq[io] * rho0;
+                    re += v.x * e.x + v.y * e.y;
+                    im += v.x * e.y - v.y * e.x;
                 }
-                const float gv = (gid + 2.0f * rho0f * xf * vexcf) * (float)si;
-                if constexpr (DOT32) {
-                    df32[g] = gv;
-                    const double gd = (double)gv;
-                    // the empty asm keeps the conversions here: hoisted next to the loads above,
-                    // they would make each load wait
-                    float o0 = dq[0][q], o1 = dq[1][q];
-                    asm volatile("" : "+v"(o0), "+v"(o1));
-                    d0 += gd * ((da.self & 1) ? gd : (double)o0);
-                    if (da.k > 1) d1 += gd * (((da.self >> 1) & 1) ? gd : (double)o1);
-                    if (da.k > 2) d2 += gd * (((da.self >> 2) & 1) ? gd : (double)static_cast<const float *>(da.p[2])[g]);
-                    continue;
-                }
-                if (df32) df32[g] = gv;
-                else if (df) df[g] = (double)gv;
-                if (da.k > 0) {
-                    d0 += (double)gv * dot_operand(da, 0, g, (double)gv);
-                    if (da.k > 1) d1 += (double)gv * dot_operand(da, 1, g, (double)gv);
-                    if (da.k > 2) d2 += (double)gv * dot_operand(da, 2, g, (double)gv);
-                }
+            }
+            h[i] = mk<V>(re, im);
+        }
+        __syncthreads();
+        // the tile's xi and vext, all of a thread's loads issued together for the loop below;
+        // the other workgroups on the compute unit cover their latency (loaded before the
+        // synthesis phases, they would hold registers through them)
+        constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
+        double xr[NIT], vr[NIT];
+        float xr32[NIT], vr32[NIT];  // DOT32: the FP32 iterate and the FP32 copy of vext
+    #pragma unroll
+        for (int q = 0; q < NIT; q++) {
+            const int i = tl + q * kThreads;
+            if constexpr (DOT32) {
+                const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
+                xr32[q] = xi32[g];
+                vr32[q] = vext32[g];
                 continue;
             }
-        }
-        // from here on FP64: free-energy reductions, rho ln rho and the gradient
-        const double v = (double)vt;
-        const double wio = wq[io];
-        const double vexc = k.prefactor * wio * v;
-        const double si = (mode == 0 && vsinv) ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables (mode 0)
-        const double x = xq * si;
-        const double rho = x * x * rho0;
-        fexc += (x * x * rho0 - rho0) * vexc;
-        if (mode == 0) {
-            const double ve = vr[q];
-            double gid = 0.0;
-            if (rho > DBL_EPSILON) {
-                const double lg = log(x * x);  // ln(rho / rho0)
-                fid += kT * dv * wio * (rho * lg - rho + rho0);
-                fext += dv * wio * rho * ve;
-                gid = wio * 2.0 * rho0 * x * (kT * lg + ve);
-            } else {
-                fid += kT * dv * wio * rho0;
+            xr[q] = 0.0;
+            vr[q] = 0.0;
+            if (i < npts * NO) {
+                const size_t g = r0 * NO + i;
+                if (mode < 3) xr[q] = xi32 ? hold_f32(xi32[g]) : xi[g];  // FP32 iterate: xi32
+                if (mode == 0 || mode == 3 || mode == 4) vr[q] = vext[g];
             }
-            const double gv = (gid + 2.0 * rho0 * x * vexc) * si;
-            if (df32) df32[g] = (float)gv;
-            else if (df) df[g] = gv;
-            // (the FP64 kernel forms no gradient products: its context runs the separate
-            // multi-dot, which keeps this kernel's registers where they were)
-        } else if (mode == 1) {
-            if (df) df[g] += 2.0 * rho0 * x * vexc;
-        } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
-            if (df) df[g] = -(k.prefactor / kT) * v;
-        } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
-            df[g] = exp(0.5 * fmin(fmax(-vr[q] / kT + v, -200.0), 7.0));
-        } else if (mode == 4) {  // proj holds the projections of rho/rho0 - 1;
-                                 // df = sqrt(max(1 + v, floor)), 0 where the cold start excludes the solvent
-            df[g] = (vr[q] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
-        } else {  // mode 5: df = proj2angl(proj), the plain transform
-            df[g] = v;
+        }
+        // DOT32: FP32 operands of the first two gradient products, loaded two elements ahead of
+        // their use (a load next to its use would make every element wait for HBM); an operand that
+        // is not needed (the gradient itself, or beyond da.k) is read from dF, which exists here
+        constexpr int kAhead = 2;
+        float dq[2][NIT];
+        const float *dp0 = (da.k > 0 && !(da.self & 1)) ? static_cast<const float *>(da.p[0]) : df32;
+        const float *dp1 = (da.k > 1 && !((da.self >> 1) & 1)) ? static_cast<const float *>(da.p[1]) : df32;
+        auto load_dq = [&](int q) {
+            const int i = tl + q * kThreads;
+            const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
+            dq[0][q] = dp0[g];
+            dq[1][q] = dp1[g];
+        };
+        if constexpr (DOT32) {
+    #pragma unroll
+            for (int q = 0; q < kAhead && q < NIT; q++) load_dq(q);
+        }
+    #pragma unroll
+        for (int q = 0; q < NIT; q++) {
+            if constexpr (DOT32) {
+                if (q + kAhead < NIT) load_dq(q + kAhead);
+            }
+            const int i = tl + q * kThreads;
+            if (i >= npts * NO) break;
+            double xq;
+            float ve32 = 0.0f;
+            if constexpr (DOT32) {
+                float xv = xr32[q];
+                ve32 = vr32[q];
+                asm volatile("" : "+v"(xv), "+v"(ve32));  // the conversion stays here, not next to the load
+                xq = (double)xv;
+            } else {
+                xq = xi32 ? held_f32(xr[q]) : xr[q];
+            }
+            const int pt = i / NO, io = i - pt * NO;
+            const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
+            const V *hr = h + (pt * NTH * NPHI + line) * H1;
+            const V *tp = twpsi + ipsi * H1;
+            T vt = hr[0].x;
+    #pragma unroll
+            for (int mu2 = 1; mu2 < H1; mu2++) {
+                if constexpr (sizeof(T) == 4) vt = fma(-hr[mu2].y, tp[mu2].y, fma(hr[mu2].x, tp[mu2].x, vt));
+                else vt += hr[mu2].x * tp[mu2].x - hr[mu2].y * tp[mu2].y;
+            }
+            const size_t g = r0 * NO + i;
+            if constexpr (sizeof(T) == 4) {
+                if (mode == 0) {
+                    // FP32 path: the per-orientation terms in FP32 (the transforms feeding them
+                    // are FP32 already), summed in FP64; rho/rho0 - 1 is formed in FP64 first so
+                    // that the excess term keeps its small deviations from the bulk
+                    const double si = vsinv ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
+                    const double x = xq * si;
+                    const double xx = x * x;
+                    const float wf = (float)wq[io];
+                    const float vexcf = (float)k.prefactor * wf * vt;
+                    const float drf = (float)(xx - 1.0);
+                    const float rho0f = (float)rho0, x2 = (float)xx, xf = (float)x;
+                    fexc += (double)(rho0f * drf * vexcf);
+                    float gid = 0.0f;
+                    if (xx * rho0 > DBL_EPSILON) {
+                        // ln(rho / rho0) from the hardware base-2 logarithm: x2 > 6e-15 here (rho > DBL_EPSILON),
+                        // a normal FP32 number, so logf's denormal scaling and its extra-precise
+                        // product with ln 2 change nothing at this path's FP32 accuracy
+                        const float lg = __builtin_amdgcn_logf(x2) * 0.693147181f;
+                        const float ve = DOT32 ? ve32 : (float)vr[q];
+                        const float kTf = (float)kT, dvf = (float)dv;
+                        fid += (double)(kTf * dvf * wf * rho0f * (x2 * lg - drf));
+                        fext += (double)(dvf * wf * rho0f * x2 * ve);
+                        gid = wf * 2.0f * rho0f * xf * (kTf * lg + ve);
+                    } else {
+                        fid += kT * dv * wq[io] * rho0;
+                    }
+                    const float gv = (gid + 2.0f * rho0f * xf * vexcf) * (float)si;
+                    if constexpr (DOT32) {
+                        df32[g] = gv;
+                        const double gd = (double)gv;
+                        // the empty asm keeps the conversions here: hoisted next to the loads above,
+                        // they would make each load wait
+                        float o0 = dq[0][q], o1 = dq[1][q];
+                        asm volatile("" : "+v"(o0), "+v"(o1));
+                        d0 += gd * ((da.self & 1) ? gd : (double)o0);
+                        if (da.k > 1) d1 += gd * (((da.self >> 1) & 1) ? gd : (double)o1);
+                        if (da.k > 2) d2 += gd * (((da.self >> 2) & 1) ? gd : (double)static_cast<const float *>(da.p[2])[g]);
+                        continue;
+                    }
+                    if (df32) df32[g] = gv;
+                    else if (df) df[g] = (double)gv;
+                    if (da.k > 0) {
+                        d0 += (double)gv * dot_operand(da, 0, g, (double)gv);
+                        if (da.k > 1) d1 += (double)gv * dot_operand(da, 1, g, (double)gv);
+                        if (da.k > 2) d2 += (double)gv * dot_operand(da, 2, g, (double)gv);
+                    }
+                    continue;
+                }
+            }
+            // from here on FP64: free-energy reductions, rho ln rho and the gradient
+            const double v = (double)vt;
+            const double wio = wq[io];
+            const double vexc = k.prefactor * wio * v;
+            const double si = (mode == 0 && vsinv) ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables (mode 0)
+            const double x = xq * si;
+            const double rho = x * x * rho0;
+            fexc += (x * x * rho0 - rho0) * vexc;
+            if (mode == 0) {
+                const double ve = vr[q];
+                double gid = 0.0;
+                if (rho > DBL_EPSILON) {
+                    const double lg = log(x * x);  // ln(rho / rho0)
+                    fid += kT * dv * wio * (rho * lg - rho + rho0);
+                    fext += dv * wio * rho * ve;
+                    gid = wio * 2.0 * rho0 * x * (kT * lg + ve);
+                } else {
+                    fid += kT * dv * wio * rho0;
+                }
+                const double gv = (gid + 2.0 * rho0 * x * vexc) * si;
+                if (df32) df32[g] = (float)gv;
+                else if (df) df[g] = gv;
+                // (the FP64 kernel forms no gradient products: its context runs the separate
+                // multi-dot, which keeps this kernel's registers where they were)
+            } else if (mode == 1) {
+                if (df) df[g] += 2.0 * rho0 * x * vexc;
+            } else if (mode == 2) {  // gamma(r, Omega) in units of ln(rho/rho0)
+                if (df) df[g] = -(k.prefactor / kT) * v;
+            } else if (mode == 3) {  // proj holds gamma's projections; df = exp((-beta vext + gamma) / 2)
+                df[g] = exp(0.5 * fmin(fmax(-vr[q] / kT + v, -200.0), 7.0));
+            } else if (mode == 4) {  // proj holds the projections of rho/rho0 - 1;
+                                     // df = sqrt(max(1 + v, floor)), 0 where the cold start excludes the solvent
+                df[g] = (vr[q] / kT >= -log(DBL_EPSILON) - 0.05) ? 0.0 : sqrt(fmax(1.0 + v, MDFT_WS_DENSITY_FLOOR));
+            } else {  // mode 5: df = proj2angl(proj), the plain transform
+                df[g] = v;
+            }
         }
     }
     block_reduce3(fid, fext, fexc, partials);
@@ -711,6 +725,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
 }
 
 constexpr int kPT = 4;  // points per workgroup tile (keeps the LDS tile under 64 KB at mmax = 5)
+
+// Workgroups of a persistent transform: 8 per compute unit (32 waves, the occupancy the
+// LDS of the mmax = 5 kernels allows), never more than the tiles.
+size_t persistent_grid(const mdft_ctx *c, size_t ntiles) {
+    const size_t g = 8 * (size_t)(c->num_cus > 0 ? c->num_cus : 256);
+    return ntiles < g ? ntiles : g;
+}
 
 size_t a2p_lds(const AngTables &t) {
     return sizeof(double) * kPT * t.no + sizeof(double2) * kPT * t.ntheta * t.nphi * (t.H + 1) +
@@ -750,9 +771,10 @@ bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int fe
 
 template <int PT, typename T>
 bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi, double *df, int mode, FinalConsts k,
-                    size_t npts = 0) {
+                    size_t npts = 0, size_t *grid_out = nullptr) {
     if (!npts) npts = c->N;
-    const size_t nb = (npts + PT - 1) / PT;
+    const size_t nb = persistent_grid(c, (npts + PT - 1) / PT);
+    if (grid_out) *grid_out = nb;
     DotArgs da{};
     if (mode == 0 && c->dot_k > 0) {
         da.k = c->dot_k;
@@ -802,8 +824,9 @@ hipError_t launch_final_f64(mdft_ctx *c, const double2 *proj, const double *xi, 
     if (c->d.mrso == 2) {
         const size_t nbt = (c->N + kPTt - 1) / kPTt;
         if ((e = ensure_partials(c, nbt)) != hipSuccess) return e;
-        if (launch_final_t<kPTt, double>(c, proj, xi, df, mode, k)) {
-            *nblocks_out = (int)nbt;
+        size_t grid = 0;
+        if (launch_final_t<kPTt, double>(c, proj, xi, df, mode, k, 0, &grid)) {
+            *nblocks_out = (int)grid;
             return hipGetLastError();
         }
     }
@@ -866,8 +889,10 @@ hipError_t launch_final_f32(mdft_ctx *c, const float2 *proj, const double *xi, d
     const size_t nbt = (c->N + kPT32 - 1) / kPT32;
     hipError_t e = ensure_partials(c, nbt);
     if (e != hipSuccess) return e;
-    if (c->d.mrso != 2 || !launch_final_t<kPT32, float>(c, proj, xi, df, mode, final_consts(c))) return hipErrorNotSupported;
-    *nblocks_out = (int)nbt;
+    size_t grid = 0;
+    if (c->d.mrso != 2 || !launch_final_t<kPT32, float>(c, proj, xi, df, mode, final_consts(c), 0, &grid))
+        return hipErrorNotSupported;
+    *nblocks_out = (int)grid;
     return hipGetLastError();
 }
 

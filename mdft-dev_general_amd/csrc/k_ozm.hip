@@ -12,19 +12,28 @@
 // The chops apply to the FP64 kernel; the FP32 kernel skips them (their effect,
 // single-precision rounding of one component, is below FP32's own rounding).
 //
-// Work layout.  One wave takes a tile of 8 pairs that share the interpolation
-// interval iq (pairs are sorted by iq on the host); its 16 (side, pair) rows
-// are the M dimension of the matrix products.  Per tile:
+// Work layout.  A workgroup of four waves takes a tile of 8 pairs that share the
+// interpolation interval iq (pairs are sorted by iq on the host, one host-built
+// record per pair: grid indices of q and -q, flags, the angles of q and alpha);
+// the tile's 16 (side, pair) rows are the M dimension of the matrix products.
+// Workgroup b takes tile (b & 7) * per + (b >> 3), so the tiles of one
+// accelerator die (workgroups go to the 8 dies round robin) are contiguous in
+// |q|, and their c(k) fragments share that die's L2.  Per tile:
+//  0. the 16 projection records X (np complex each) are copied into LDS with
+//     whole-record coalesced loads;
 //  1. rotation to the q frame on the vector ALUs, lanes over pairs: lane
 //     (pair p = l & 7, slot j = l >> 3) takes rotation units (m, khi) from a
 //     host-built round table (units of equal m/2 + 1 in a round, so the lanes
-//     run one instruction stream), forms R^m(mup, khi) = exp(-i mup phi_q)
-//     d^m(mup, khi)(theta_q) once per unit entry (d^m from its Fourier series in
-//     theta, coefficients host-built) and uses it for both sides and every mu:
+//     run one instruction stream; rounds balanced over the four waves), forms
+//     R^m(mup, khi) = exp(-i mup phi_q) d^m(mup, khi)(theta_q) once per unit
+//     entry (d^m from its Fourier series in theta, coefficients host-built and
+//     read one iteration ahead) and uses it for both sides and every mu:
 //       A_q(m, khi, mu) = sum_mup X_q(m, mup, mu) R(mup, khi)
 //       A_-q(m, -khi, mu) = (-1)^m sum_mup X_-q(m, mup, mu) R(mup, khi);
-//     X is read straight from the projection array, A goes to the wave's LDS.
-//  2. OZ contraction: for each khi the rows (m, mu) of both sides are
+//     in FP32 the phase e^{-i mup phi} is applied once per element of X as it
+//     enters LDS and once per output, so the unit loops multiply by real d^m;
+//  2. OZ contraction (OZ blocks khi spread over the four waves): for each khi
+//     the rows (m, mu) of both sides are
 //       gamma_s(m, khi, mu) = sum_{(n, nu)} c_{m mu; n nu}^{khi}(|q|) V_s(n, nu)
 //     with V_s(n, nu) = A_s(n, khi, |nu|) for nu < 0 and conj(A_-s(n, khi, nu))
 //     otherwise.  All pairs of the tile share c[iq] and c[iq+1], so with
@@ -32,11 +41,12 @@
 //     [16 rows x 2 ncol] (re/im of V) times a [2 ncol x 4 nrow] real form of
 //     (c[iq], c[iq+1]) whose fragments are precomputed per interval in the
 //     order the matrix instruction reads them (ozm_frags, at c(k) upload);
-//     alpha and 1 - alpha are applied per row after the product.
+//     alpha and 1 - alpha are applied per row after the product, the result
+//     transposed through LDS to the (side, pair) rows;
 //  3. rotation back on the vector ALUs, units (m, mup):
 //       out_q(m, mup, mu) = sum_khi gamma_q(m, khi, mu) conj(R(mup, khi))
 //       out_-q(m, mup, mu) = (-1)^m sum_khi gamma_-q(m, khi, mu) conj(R(mup, -khi))
-//     stored straight to the projection array.
+//     into LDS, then copied out record by record (coalesced).
 // R is formed from the factorisation with d^m from its Fourier series (both
 // exact in FP64 to rounding), then chopped as the reference chops its
 // recursion's result; the OZ sums are summed in the matrix unit's order (a
