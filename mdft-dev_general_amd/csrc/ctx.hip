@@ -520,7 +520,8 @@ void mdft_ctx_destroy(mdft_ctx *c) {
                     c->d_ky,  c->d_kz,   c->d_ck,   c->d_vext, c->d_proj,     c->d_xi_stage,   c->d_df_stage,
                     c->d_partials, c->d_slices, c->d_sums, c->d_vpart, c->d_mpart, c->d_oz32_ftab, c->d_oz32_lanerow,
                     c->d_ozm_frag, c->d_ozm_frag16, c->d_ozm_col, c->d_ozm_ftab, c->d_ozm_units, c->d_ozm_rlen,
-                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart, c->d_vext32};
+                    c->d_bf16_M, c->d_bf16_Q, c->d_chscale, c->d_adjscale, c->d_vscale, c->d_dpart, c->d_vext32,
+                    c->d_a2p_img[0], c->d_a2p_img[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_pairs(c->pairs);

@@ -327,6 +327,44 @@ struct Dims {
     static constexpr int NP = np_of(L);
 };
 
+// The constant tables of angl2proj_t in the layout of its LDS.  The image is built once per
+// context and precision (a2p_tab_build, the staging arithmetic run once) and each workgroup
+// copies it with 16-byte loads: the index arithmetic and FP64 -> T conversions of the
+// staging then run once instead of once per 4-point tile.
+template <int L, typename T> struct alignas(16) A2pTab {
+    using V = typename C2<T>::t;
+    using D = Dims<L>;
+    V twpsi[D::H1 * D::NPSI];  // (cos, sin)(2 pi mu2 ipsi / npsi) at [mu2][ipsi]
+    V twphi[D::W * D::NPHI];   // (cos, sin)(2 pi mup iphi / nphi) / (nphi npsi) at [mup + L][iphi]
+    T a2p[D::NP * D::NTH], fmv[D::NP];
+    int pmup[D::NP], pmu2[D::NP];
+};
+
+template <int L, typename T>
+__global__ __launch_bounds__(kThreads) void a2p_tab_build(A2pTab<L, T> *__restrict__ out, AngTables t) {
+    using D = Dims<L>;
+    using V = typename C2<T>::t;
+    constexpr int NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < H1 * NPSI; i += kThreads) {
+        const int kk = ((i / NPSI) * (i % NPSI)) % NPSI;
+        out->twpsi[i] = mk<V>((T)t.cps[kk], (T)t.sps[kk]);
+    }
+    for (int i = tid; i < W * NPHI; i += kThreads) {
+        const int mup = i / NPHI - L, iphi = i % NPHI;
+        int kk = (mup * iphi) % NPHI;
+        kk += (kk < 0) ? NPHI : 0;
+        const double nrm = 1.0 / (double)(NPHI * NPSI);
+        out->twphi[i] = mk<V>((T)(t.cph[kk] * nrm), (T)(t.sph[kk] * nrm));
+    }
+    for (int i = tid; i < NP * NTH; i += kThreads) out->a2p[i] = (T)t.a2p[i];
+    for (int i = tid; i < NP; i += kThreads) {
+        out->fmv[i] = (T)t.fm[i];
+        out->pmup[i] = t.pmup[i];
+        out->pmu2[i] = t.pmu[i] / 2;
+    }
+}
+
 // feat = 0: o = rho0 (xi^2 - 1) (the excess term); feat = 1: o = clip(-beta xi / 2, -4, 4) / 4
 // with xi the external potential -- the log of the square-root Boltzmann factor, bounded
 // (features of the learned warm start); feat = 2: o = xi^2 - 1 = rho/rho0 - 1 (the
@@ -340,17 +378,18 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
                                                        size_t N, AngTables t, double k0, int feat, double beta,
                                                        const double *__restrict__ aux,
                                                        const double *__restrict__ vsinv,
-                                                       const float *__restrict__ xi32) {
+                                                       const float *__restrict__ xi32,
+                                                       const A2pTab<L, T> *__restrict__ img) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
     __shared__ T o[PT * NO];
     __shared__ V G[PT * NTH * NPHI * H1];
     __shared__ V F[PT * NTH * W * H1];
-    __shared__ V twpsi[H1 * NPSI];  // (cos, sin)(2 pi mu2 ipsi / npsi) at [mu2][ipsi]
-    __shared__ V twphi[W * NPHI];   // (cos, sin)(2 pi mup iphi / nphi) / (nphi npsi) at [mup + L][iphi]
-    __shared__ T a2p[NP * NTH], fmv[NP];
-    __shared__ int pmup[NP], pmu2[NP];
+    __shared__ A2pTab<L, T> tb;
+    const V *twpsi = tb.twpsi, *twphi = tb.twphi;
+    const T *a2p = tb.a2p, *fmv = tb.fmv;
+    const int *pmup = tb.pmup, *pmu2 = tb.pmu2;
     const int tid = threadIdx.x;
     const size_t r0 = (size_t)blockIdx.x * PT;
     const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
@@ -365,19 +404,12 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
         xr[q] = (i < npts * NO) ? (xi32 ? hold_f32(xi32[r0 * NO + i]) : src[i]) : 0.0;  // FP32 iterate: xi32
         sr[q] = vsinv ? vsinv[i % NO] : 1.0;  // scaled variables: xi = u / s(io), loaded with the tile
     }
-    for (int i = tid; i < H1 * NPSI; i += kThreads) {
-        const int kk = ((i / NPSI) * (i % NPSI)) % NPSI;
-        twpsi[i] = mk<V>((T)t.cps[kk], (T)t.sps[kk]);
+    {
+        constexpr int n16 = (int)(sizeof(A2pTab<L, T>) / 16);
+        const uint4 *src16 = reinterpret_cast<const uint4 *>(img);
+        uint4 *dst16 = reinterpret_cast<uint4 *>(&tb);
+        for (int i = tid; i < n16; i += kThreads) dst16[i] = src16[i];
     }
-    for (int i = tid; i < W * NPHI; i += kThreads) {
-        const int mup = i / NPHI - L, iphi = i % NPHI;
-        int kk = (mup * iphi) % NPHI;
-        kk += (kk < 0) ? NPHI : 0;
-        const double nrm = 1.0 / (double)(NPHI * NPSI);
-        twphi[i] = mk<V>((T)(t.cph[kk] * nrm), (T)(t.sph[kk] * nrm));
-    }
-    for (int i = tid; i < NP * NTH; i += kThreads) a2p[i] = (T)t.a2p[i];
-    for (int i = tid; i < NP; i += kThreads) { fmv[i] = (T)t.fm[i]; pmup[i] = t.pmup[i]; pmu2[i] = t.pmu[i] / 2; }
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
         const int i = tid + q * kThreads;
@@ -760,9 +792,16 @@ bool launch_a2p_t(mdft_ctx *c, const double *xi, typename C2<T>::t *proj, int fe
     const size_t nb = (npts + PT - 1) / PT;
     const double beta = 1.0 / c->d.kT;
     const double k0 = (feat == 3) ? 0.5 * c->dv : c->rho0;
+    void *&img = c->d_a2p_img[sizeof(T) == 8];
     switch (c->L) {
 #define A2P_CASE(LL) \
-    case LL: hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, npts, c->ang, k0, feat, beta, aux, feat == 0 ? c->vsinv : nullptr, feat == 0 ? c->xi32 : nullptr); return true;
+    case LL: \
+        if (!img) { \
+            if (hipMalloc(&img, sizeof(A2pTab<LL, T>)) != hipSuccess) { img = nullptr; return false; } \
+            hipLaunchKernelGGL((a2p_tab_build<LL, T>), dim3(1), dim3(kThreads), 0, c->stream, static_cast<A2pTab<LL, T> *>(img), c->ang); \
+        } \
+        hipLaunchKernelGGL((angl2proj_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, xi, proj, npts, c->ang, k0, feat, beta, aux, feat == 0 ? c->vsinv : nullptr, feat == 0 ? c->xi32 : nullptr, static_cast<const A2pTab<LL, T> *>(img)); \
+        return true;
         A2P_CASE(0) A2P_CASE(1) A2P_CASE(2) A2P_CASE(3) A2P_CASE(4) A2P_CASE(5)
 #undef A2P_CASE
         default: return false;

@@ -121,6 +121,7 @@ struct mdft_ctx {
     const void *dot_vec[3] = {nullptr, nullptr, nullptr};
     double *d_dpart = nullptr;    // [blocks][3] their block partials
     float *d_vext32 = nullptr;    // FP32 copy of vext read by the fast minimiser's final pass
+    void *d_a2p_img[2] = {nullptr, nullptr};  // angl2proj_t's LDS table image, FP32 and FP64
     bool vext32_ok = false;       // d_vext32 holds the current vext
     size_t dpart_cap = 0;
     float *d_chscale = nullptr;   // per-channel input scale of the warm-start features [2 np]
