@@ -136,36 +136,34 @@ struct VecSet {
 };
 
 // part[blk][j] = sum_i x[i] v_j[i]   (16-byte accesses)
+template <int K>
 __global__ __launch_bounds__(kThreads) void multi_dot_partial(size_t n, const double *__restrict__ x, VecSet v,
                                                              double *__restrict__ part) {
-    double acc[MDFT_VEC_MAX];
+    double acc[K];
 #pragma unroll
-    for (int j = 0; j < MDFT_VEC_MAX; j++) acc[j] = 0.0;
+    for (int j = 0; j < K; j++) acc[j] = 0.0;
     const size_t n2 = n / 2;
     const double2 *x2 = reinterpret_cast<const double2 *>(x);
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += (size_t)gridDim.x * kThreads) {
         const double2 xv = x2[i];
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) {
-                const double2 w = ((v.self >> j) & 1) ? xv : reinterpret_cast<const double2 *>(v.p[j])[i];
+        for (int j = 0; j < K; j++) {
+                const double2 w = reinterpret_cast<const double2 *>(v.p[j])[i];  // x itself: a cache hit
                 acc[j] += xv.x * w.x + xv.y * w.y;
             }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) acc[j] += x[n - 1] * v.p[j][n - 1];
+        for (int j = 0; j < K; j++) acc[j] += x[n - 1] * v.p[j][n - 1];
     }
     __shared__ double red[MDFT_VEC_MAX][kThreads / 64];
 #pragma unroll
-    for (int j = 0; j < MDFT_VEC_MAX; j++) {
-        if (j >= v.k) break;
+    for (int j = 0; j < K; j++) {
         const double s = wave_sum(acc[j]);
         if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
     }
     __syncthreads();
-    if ((int)threadIdx.x < v.k) {
+    if ((int)threadIdx.x < K) {
         double s = 0.0;
         for (int w = 0; w < kThreads / 64; w++) s += red[threadIdx.x][w];
         part[(size_t)blockIdx.x * MDFT_VEC_MAX + threadIdx.x] = s;
@@ -191,13 +189,13 @@ __global__ __launch_bounds__(kThreads) void multi_dot_final(const double *__rest
 }
 
 // z = sum_j c_j v_j   (z may be one of the v_j: each element is read before it is written)
+template <int K>
 __global__ __launch_bounds__(kThreads) void combine(size_t n, VecSet v, double *z) {
     const size_t n2 = n / 2;
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n2; i += (size_t)gridDim.x * kThreads) {
         double sx = 0.0, sy = 0.0;
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) {
+        for (int j = 0; j < K; j++) {
                 const double2 w = reinterpret_cast<const double2 *>(v.p[j])[i];
                 sx += v.c[j] * w.x;
                 sy += v.c[j] * w.y;
@@ -207,14 +205,14 @@ __global__ __launch_bounds__(kThreads) void combine(size_t n, VecSet v, double *
     if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
         double s = 0.0;
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) s += v.c[j] * v.p[j][n - 1];
+        for (int j = 0; j < K; j++) s += v.c[j] * v.p[j][n - 1];
         z[n - 1] = s;
     }
 }
 
 // z = sum_j c_j v_j and y = x + a z in one pass (the new search direction and the
 // first trial point of the line search); y must not alias x, z or a v_j
+template <int K>
 __global__ __launch_bounds__(kThreads) void combine_step(size_t n, VecSet v, double *z, double a,
                                                         const double *__restrict__ x, double *__restrict__ y) {
     const size_t n2 = n / 2;
@@ -222,8 +220,7 @@ __global__ __launch_bounds__(kThreads) void combine_step(size_t n, VecSet v, dou
         const double2 xv = reinterpret_cast<const double2 *>(x)[i];
         double sx = 0.0, sy = 0.0;
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) {
+        for (int j = 0; j < K; j++) {
                 const double2 w = reinterpret_cast<const double2 *>(v.p[j])[i];
                 sx += v.c[j] * w.x;
                 sy += v.c[j] * w.y;
@@ -234,8 +231,7 @@ __global__ __launch_bounds__(kThreads) void combine_step(size_t n, VecSet v, dou
     if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
         double s = 0.0;
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) s += v.c[j] * v.p[j][n - 1];
+        for (int j = 0; j < K; j++) s += v.c[j] * v.p[j][n - 1];
         z[n - 1] = s;
         y[n - 1] = x[n - 1] + a * s;
     }
@@ -251,36 +247,34 @@ struct VecSet32 {
     int self;
 };
 
+template <int K>
 __global__ __launch_bounds__(kThreads) void multi_dot_partial32(size_t n, const float *__restrict__ x, VecSet32 v,
                                                                double *__restrict__ part) {
-    double acc[MDFT_VEC_MAX];
+    double acc[K];
 #pragma unroll
-    for (int j = 0; j < MDFT_VEC_MAX; j++) acc[j] = 0.0;
+    for (int j = 0; j < K; j++) acc[j] = 0.0;
     const size_t n4 = n / 4;
     const float4 *x4 = reinterpret_cast<const float4 *>(x);
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
         const float4 xv = x4[i];
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) {
-                const float4 w = ((v.self >> j) & 1) ? xv : reinterpret_cast<const float4 *>(v.p[j])[i];
+        for (int j = 0; j < K; j++) {
+                const float4 w = reinterpret_cast<const float4 *>(v.p[j])[i];  // x itself: a cache hit
                 acc[j] += (double)xv.x * w.x + (double)xv.y * w.y + (double)xv.z * w.z + (double)xv.w * w.w;
             }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         for (size_t t = 4 * n4; t < n; t++)
 #pragma unroll
-            for (int j = 0; j < MDFT_VEC_MAX; j++)
-                if (j < v.k) acc[j] += (double)x[t] * v.p[j][t];
+            for (int j = 0; j < K; j++) acc[j] += (double)x[t] * v.p[j][t];
     __shared__ double red[MDFT_VEC_MAX][kThreads / 64];
 #pragma unroll
-    for (int j = 0; j < MDFT_VEC_MAX; j++) {
-        if (j >= v.k) break;
+    for (int j = 0; j < K; j++) {
         const double s = wave_sum(acc[j]);
         if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
     }
     __syncthreads();
-    if ((int)threadIdx.x < v.k) {
+    if ((int)threadIdx.x < K) {
         double s = 0.0;
         for (int w = 0; w < kThreads / 64; w++) s += red[threadIdx.x][w];
         part[(size_t)blockIdx.x * MDFT_VEC_MAX + threadIdx.x] = s;
@@ -289,6 +283,7 @@ __global__ __launch_bounds__(kThreads) void multi_dot_partial32(size_t n, const 
 
 // z = (float) sum_j c_j v_j and y = x + a z (with z as stored, so that y - x is exactly
 // a times the stored direction); y must not alias x, z or a v_j
+template <int K>
 __global__ __launch_bounds__(kThreads) void combine_step32(size_t n, VecSet32 v, float *z, double a,
                                                           const double *__restrict__ x, double *__restrict__ y) {
     const size_t n4 = n / 4;
@@ -297,8 +292,7 @@ __global__ __launch_bounds__(kThreads) void combine_step32(size_t n, VecSet32 v,
         const double2 x1 = reinterpret_cast<const double2 *>(x)[2 * i + 1];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) {
+        for (int j = 0; j < K; j++) {
                 const float4 w = reinterpret_cast<const float4 *>(v.p[j])[i];
                 s0 += v.c[j] * w.x;
                 s1 += v.c[j] * w.y;
@@ -314,8 +308,7 @@ __global__ __launch_bounds__(kThreads) void combine_step32(size_t n, VecSet32 v,
         for (size_t t = 4 * n4; t < n; t++) {
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < MDFT_VEC_MAX; j++)
-                if (j < v.k) s += v.c[j] * v.p[j][t];
+            for (int j = 0; j < K; j++) s += v.c[j] * v.p[j][t];
             z[t] = (float)s;
             y[t] = x[t] + a * (double)z[t];
         }
@@ -338,6 +331,7 @@ __global__ __launch_bounds__(kThreads) void axpy32(size_t n, double a, const flo
 
 // The same two passes with the iterate itself in FP32 (x, y float): y = x + a z
 // formed in FP64 and rounded once.
+template <int K>
 __global__ __launch_bounds__(kThreads) void combine_step32x(size_t n, VecSet32 v, float *z, double a,
                                                            const float *__restrict__ x, float *__restrict__ y) {
     const size_t n4 = n / 4;
@@ -345,8 +339,7 @@ __global__ __launch_bounds__(kThreads) void combine_step32x(size_t n, VecSet32 v
         const float4 xv = reinterpret_cast<const float4 *>(x)[i];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-        for (int j = 0; j < MDFT_VEC_MAX; j++)
-            if (j < v.k) {
+        for (int j = 0; j < K; j++) {
                 const float4 w = reinterpret_cast<const float4 *>(v.p[j])[i];
                 s0 += v.c[j] * w.x;
                 s1 += v.c[j] * w.y;
@@ -363,8 +356,7 @@ __global__ __launch_bounds__(kThreads) void combine_step32x(size_t n, VecSet32 v
         for (size_t t = 4 * n4; t < n; t++) {
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < MDFT_VEC_MAX; j++)
-                if (j < v.k) s += v.c[j] * v.p[j][t];
+            for (int j = 0; j < K; j++) s += v.c[j] * v.p[j][t];
             z[t] = (float)s;
             y[t] = (float)((double)x[t] + a * (double)z[t]);
         }
@@ -440,6 +432,21 @@ __global__ __launch_bounds__(kThreads) void density_integrals(const double *__re
     }
 }
 
+// Launches kernel<K> for the runtime vector count k, 1 <= k <= MDFT_VEC_MAX (checked at the
+// boundary): with K known the loop over the vectors has no branches, so a thread's loads of
+// one element group are all in flight together instead of one wait per vector.
+#define VEC_K_DISPATCH(k, LAUNCH)                                                        \
+    switch (k) {                                                                          \
+        case 1: LAUNCH(1); break;   case 2: LAUNCH(2); break;   case 3: LAUNCH(3); break;   \
+        case 4: LAUNCH(4); break;   case 5: LAUNCH(5); break;   case 6: LAUNCH(6); break;   \
+        case 7: LAUNCH(7); break;   case 8: LAUNCH(8); break;   case 9: LAUNCH(9); break;   \
+        case 10: LAUNCH(10); break; case 11: LAUNCH(11); break; case 12: LAUNCH(12); break; \
+        case 13: LAUNCH(13); break; case 14: LAUNCH(14); break; case 15: LAUNCH(15); break; \
+        case 16: LAUNCH(16); break;                                                       \
+        default: return hipErrorInvalidValue;                                             \
+    }
+static_assert(MDFT_VEC_MAX == 16, "VEC_K_DISPATCH lists 1..16");
+
 unsigned blocks_for(size_t n) {
     size_t b = (n / 2 + kThreads - 1) / kThreads;
     if (b < 1) b = 1;
@@ -509,7 +516,9 @@ hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs
     }
     v.k = k;
     const unsigned nb = blocks_for(n);
-    hipLaunchKernelGGL(multi_dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart);
+#define L_(K) hipLaunchKernelGGL(multi_dot_partial<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart)
+    VEC_K_DISPATCH(k, L_)
+#undef L_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     double *dout = c->d_mpart + (size_t)kMaxBlocks * MDFT_VEC_MAX;
@@ -526,7 +535,9 @@ hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const d
         v.c[j] = coef[j];
     }
     v.k = k;
-    hipLaunchKernelGGL(combine, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z);
+#define L_(K) hipLaunchKernelGGL(combine<K>, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z)
+    VEC_K_DISPATCH(k, L_)
+#undef L_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
@@ -540,7 +551,9 @@ hipError_t vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, co
         v.c[j] = coef[j];
     }
     v.k = k;
-    hipLaunchKernelGGL(combine_step, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
+#define L_(K) hipLaunchKernelGGL(combine_step<K>, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
+    VEC_K_DISPATCH(k, L_)
+#undef L_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
@@ -593,7 +606,9 @@ hipError_t vec_multi_dot_noalloc(mdft_ctx *c, size_t n, int k, const double *con
     }
     v.k = k;
     const unsigned nb = blocks_for(n);  // <= kMaxBlocks: every partial fits the array
-    hipLaunchKernelGGL(multi_dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, s);
+#define L_(K) hipLaunchKernelGGL(multi_dot_partial<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, s)
+    VEC_K_DISPATCH(k, L_)
+#undef L_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     double *dout = s + (size_t)kMaxBlocks * MDFT_VEC_MAX;
@@ -615,7 +630,9 @@ hipError_t vec_multi_dot32(mdft_ctx *c, size_t n, int k, const float *const *vec
     }
     v.k = k;
     const unsigned nb = blocks_for(n / 2);
-    hipLaunchKernelGGL(multi_dot_partial32, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart);
+#define L_(K) hipLaunchKernelGGL(multi_dot_partial32<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart)
+    VEC_K_DISPATCH(k, L_)
+#undef L_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     double *dout = c->d_mpart + (size_t)kMaxBlocks * MDFT_VEC_MAX;
@@ -633,7 +650,9 @@ hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, 
         v.c[j] = coef[j];
     }
     v.k = k;
-    hipLaunchKernelGGL(combine_step32, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
+#define L_(K) hipLaunchKernelGGL(combine_step32<K>, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
+    VEC_K_DISPATCH(k, L_)
+#undef L_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
@@ -654,7 +673,9 @@ hipError_t vec_combine_step32x(mdft_ctx *c, size_t n, int k, const double *coef,
         v.c[j] = coef[j];
     }
     v.k = k;
-    hipLaunchKernelGGL(combine_step32x, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y);
+#define L_(K) hipLaunchKernelGGL(combine_step32x<K>, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
+    VEC_K_DISPATCH(k, L_)
+#undef L_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
