@@ -530,6 +530,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
     for (int i = tid; i < NTH; i += kThreads) sth[i] = vsinv ? vsinv[i * NPHI * NPSI] : 1.0;
     double fid = 0.0, fext = 0.0, fexc = 0.0;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;  // products of dF with the DotArgs vectors
+    double dss = 0.0;                     // DOT32: the product of dF with itself
     const double rho0 = k.rho0, kT = k.kT, dv = k.dv;
     const size_t ntiles = (N + PT - 1) / PT;
     for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -540,6 +541,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
         asm volatile("" : "+v"(tl));
         const size_t r0 = tile * PT;
         const int npts = DOT32 ? PT : (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+        // DOT32: the tile's base pointers (uniform), indexed by 32-bit element offsets
+        const float *xi32t = DOT32 ? xi32 + r0 * NO : nullptr, *vx32t = DOT32 ? vext32 + r0 * NO : nullptr;
+        float *df32t = DOT32 ? df32 + r0 * NO : nullptr;
         const V *src = proj + r0 * NP;
         for (int i = tl; i < npts * NP; i += kThreads) Gm[i] = src[i];
         __syncthreads();
@@ -601,9 +605,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
         for (int q = 0; q < NIT; q++) {
             const int i = tl + q * kThreads;
             if constexpr (DOT32) {
-                const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
-                xr32[q] = xi32[g];
-                vr32[q] = vext32[g];
+                const int ii = i < npts * NO ? i : npts * NO - 1;
+                xr32[q] = xi32t[ii];
+                vr32[q] = vx32t[ii];
                 continue;
             }
             xr[q] = 0.0;
@@ -621,12 +625,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
         float dq[2][NIT];
         const float *dp0 = (da.k > 0 && !(da.self & 1)) ? static_cast<const float *>(da.p[0]) : df32;
         const float *dp1 = (da.k > 1 && !((da.self >> 1) & 1)) ? static_cast<const float *>(da.p[1]) : df32;
+        const float *dp0t = DOT32 ? dp0 + r0 * NO : nullptr, *dp1t = DOT32 ? dp1 + r0 * NO : nullptr;
         auto load_dq = [&](int q) {
             const int i = tl + q * kThreads;
-            const size_t g = r0 * NO + (i < npts * NO ? i : npts * NO - 1);
-            dq[0][q] = dp0[g];
-            dq[1][q] = dp1[g];
+            const int ii = i < npts * NO ? i : npts * NO - 1;
+            dq[0][q] = dp0t[ii];
+            dq[1][q] = dp1t[ii];
         };
+        // FP32 path: the free-energy terms of a thread's elements in this tile, summed in FP32
+        // and added to the FP64 sums once per tile
+        float fexc_t = 0.0f, fid_t = 0.0f, fext_t = 0.0f;
         if constexpr (DOT32) {
     #pragma unroll
             for (int q = 0; q < kAhead && q < NIT; q++) load_dq(q);
@@ -662,8 +670,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
             if constexpr (sizeof(T) == 4) {
                 if (mode == 0) {
                     // FP32 path: the per-orientation terms in FP32 (the transforms feeding them
-                    // are FP32 already), summed in FP64; rho/rho0 - 1 is formed in FP64 first so
-                    // that the excess term keeps its small deviations from the bulk
+                    // are FP32 already), summed in FP32 over the thread's elements of the tile
+                    // and in FP64 over the tiles; rho/rho0 - 1 is formed in FP64 first so that
+                    // the excess term keeps its small deviations from the bulk
                     const double si = vsinv ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
                     const double x = xq * si;
                     const double xx = x * x;
@@ -671,7 +680,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                     const float vexcf = (float)k.prefactor * wf * vt;
                     const float drf = (float)(xx - 1.0);
                     const float rho0f = (float)rho0, x2 = (float)xx, xf = (float)x;
-                    fexc += (double)(rho0f * drf * vexcf);
+                    fexc_t += rho0f * drf * vexcf;
                     float gid = 0.0f;
                     if (xx * rho0 > DBL_EPSILON) {
                         // ln(rho / rho0) from the hardware base-2 logarithm: x2 > 6e-15 here (rho > DBL_EPSILON),
@@ -680,23 +689,26 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                         const float lg = __builtin_amdgcn_logf(x2) * 0.693147181f;
                         const float ve = DOT32 ? ve32 : (float)vr[q];
                         const float kTf = (float)kT, dvf = (float)dv;
-                        fid += (double)(kTf * dvf * wf * rho0f * (x2 * lg - drf));
-                        fext += (double)(dvf * wf * rho0f * x2 * ve);
+                        fid_t += kTf * dvf * wf * rho0f * (x2 * lg - drf);
+                        fext_t += dvf * wf * rho0f * x2 * ve;
                         gid = wf * 2.0f * rho0f * xf * (kTf * lg + ve);
                     } else {
-                        fid += kT * dv * wq[io] * rho0;
+                        fid_t += (float)(kT * dv * rho0) * wf;
                     }
                     const float gv = (gid + 2.0f * rho0f * xf * vexcf) * (float)si;
                     if constexpr (DOT32) {
-                        df32[g] = gv;
+                        df32t[i] = gv;
                         const double gd = (double)gv;
                         // the empty asm keeps the conversions here: hoisted next to the loads above,
                         // they would make each load wait
                         float o0 = dq[0][q], o1 = dq[1][q];
                         asm volatile("" : "+v"(o0), "+v"(o1));
-                        d0 += gd * ((da.self & 1) ? gd : (double)o0);
-                        if (da.k > 1) d1 += gd * (((da.self >> 1) & 1) ? gd : (double)o1);
-                        if (da.k > 2) d2 += gd * (((da.self >> 2) & 1) ? gd : (double)static_cast<const float *>(da.p[2])[g]);
+                        // every product is formed (an operand that is not needed was read from dF
+                        // and is discarded at the end); the gradient's own product separately
+                        dss = fma(gd, gd, dss);
+                        d0 = fma(gd, (double)o0, d0);
+                        d1 = fma(gd, (double)o1, d1);
+                        if (da.k > 2 && !((da.self >> 2) & 1)) d2 += gd * (double)static_cast<const float *>(da.p[2])[g];
                         continue;
                     }
                     if (df32) df32[g] = gv;
@@ -746,10 +758,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                 df[g] = v;
             }
         }
+        fexc += (double)fexc_t;
+        fid += (double)fid_t;
+        fext += (double)fext_t;
     }
     block_reduce3(fid, fext, fexc, partials);
     if constexpr (sizeof(T) == 4) {
         if (da.k > 0) {
+            if constexpr (DOT32) {
+                if (da.self & 1) d0 = dss;
+                if ((da.self >> 1) & 1) d1 = dss;
+                if ((da.self >> 2) & 1) d2 = dss;
+            }
             __syncthreads();
             block_reduce3(d0, d1, d2, dpart);
         }
