@@ -215,13 +215,19 @@ __device__ inline void run_stages(V *A, const V *twl, decltype(V::x) s, V *__res
         const int task = threadIdx.x + q * kThreads;
         if (NTASK % kThreads != 0 && task >= NTASK) continue;
         const int row = task % PB, j = task / PB, k = j % Ns;
+        // FIRST: element e = j + r nb of the line at rowbase + r rstep; rows past the batch's
+        // projections are transformed too but never stored: they read the last valid row, so
+        // every load is issued unconditionally (a load under a branch made the wave wait for it
+        // before issuing the next)
+        const long long rowbase = (base + (long long)j * g.es) * np + p0 + (row < pb ? row : pb - 1);
+        const long long rstep = (long long)nb * g.es * np;
         V v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int e = j + r * nb;
             V x;
             if constexpr (FIRST) {
-                x = (row < pb) ? data[(base + (long long)e * g.es) * np + p0 + row] : mk<V>(0, 0);
+                x = data[rowbase + r * rstep];
             } else {
                 x = A[row * ld + e];
             }
