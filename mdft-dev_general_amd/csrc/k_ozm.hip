@@ -553,19 +553,41 @@ __global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict
         }
     }
     __syncthreads();
-    // X of the 16 (side, pair) rows, each row one contiguous record
-    for (int i = tid; i < 16 * NP; i += 64 * kWaves) {
-        const int r = i / NP, e = i - r * NP;
-        C x = proj[(size_t)s.g[r] * NP + e];
-        if constexpr (sizeof(T) == 4) {  // X exp(-i mup phi) for the factored rotation
-            const int mup = a.pmup[e];
-            const C z = s.ph[r & 7][mup < 0 ? -mup : mup];
-            const T ex = z.x, ey = mup < 0 ? z.y : -z.y;
-            const C u = x;
-            x.x = u.x * ex - u.y * ey;
-            x.y = u.x * ey + u.y * ex;
+    // X of the 16 (side, pair) rows, each row one contiguous record; in FP32 times
+    // exp(-i mup phi) for the factored rotation, two elements per 16-byte access when the
+    // records are a whole number of 16-byte words
+    if constexpr (sizeof(T) == 4 && NP % 2 == 0) {
+        constexpr int NP2 = NP / 2;
+        for (int i = tid; i < 16 * NP2; i += 64 * kWaves) {
+            const int r = i / NP2, e = 2 * (i - r * NP2);
+            const float4 x2 = reinterpret_cast<const float4 *>(proj + (size_t)s.g[r] * NP)[e / 2];
+            C out[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int mup = a.pmup[e + h];
+                const C z = s.ph[r & 7][mup < 0 ? -mup : mup];
+                const T ex = z.x, ey = mup < 0 ? z.y : -z.y;
+                const T ux = h ? x2.z : x2.x, uy = h ? x2.w : x2.y;
+                out[h].x = ux * ex - uy * ey;
+                out[h].y = ux * ey + uy * ex;
+            }
+            s.xs[r][e] = out[0];
+            s.xs[r][e + 1] = out[1];
         }
-        s.xs[r][e] = x;
+    } else {
+        for (int i = tid; i < 16 * NP; i += 64 * kWaves) {
+            const int r = i / NP, e = i - r * NP;
+            C x = proj[(size_t)s.g[r] * NP + e];
+            if constexpr (sizeof(T) == 4) {
+                const int mup = a.pmup[e];
+                const C z = s.ph[r & 7][mup < 0 ? -mup : mup];
+                const T ex = z.x, ey = mup < 0 ? z.y : -z.y;
+                const C u = x;
+                x.x = u.x * ex - u.y * ey;
+                x.y = u.x * ey + u.y * ex;
+            }
+            s.xs[r][e] = x;
+        }
     }
     __syncthreads();
     rot_all<T, L, P::nclass, false>(a, s, w, p, j);
@@ -579,11 +601,22 @@ __global__ __launch_bounds__(64 * kWaves) void ozm(typename Ty<T>::c *__restrict
     rot_all<T, L, P::nclass, true>(a, s, w, p, j);
     __syncthreads();
     // store rule (:648-656): both sides of real pairs; for q == -q only the -q side
-    for (int i = tid; i < 16 * NP; i += 64 * kWaves) {
-        const int r = i / NP, e = i - r * NP;
-        const int f = s.flag[r & 7];
-        if ((f & 4) || (r < 8 && (f & 1))) continue;
-        proj[(size_t)s.g[r] * NP + e] = s.xs[r][e];
+    if constexpr (sizeof(T) == 4 && NP % 2 == 0) {
+        constexpr int NP2 = NP / 2;
+        for (int i = tid; i < 16 * NP2; i += 64 * kWaves) {
+            const int r = i / NP2, e = 2 * (i - r * NP2);
+            const int f = s.flag[r & 7];
+            if ((f & 4) || (r < 8 && (f & 1))) continue;
+            const C u0 = s.xs[r][e], u1 = s.xs[r][e + 1];
+            reinterpret_cast<float4 *>(proj + (size_t)s.g[r] * NP)[e / 2] = make_float4(u0.x, u0.y, u1.x, u1.y);
+        }
+    } else {
+        for (int i = tid; i < 16 * NP; i += 64 * kWaves) {
+            const int r = i / NP, e = i - r * NP;
+            const int f = s.flag[r & 7];
+            if ((f & 4) || (r < 8 && (f & 1))) continue;
+            proj[(size_t)s.g[r] * NP + e] = s.xs[r][e];
+        }
     }
 }
 
