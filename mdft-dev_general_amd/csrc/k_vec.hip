@@ -334,23 +334,41 @@ __global__ __launch_bounds__(kThreads) void axpy32(size_t n, double a, const flo
 template <int K>
 __global__ __launch_bounds__(kThreads) void combine_step32x(size_t n, VecSet32 v, float *z, double a,
                                                            const float *__restrict__ x, float *__restrict__ y) {
-    const size_t n4 = n / 4;
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
-        const float4 xv = reinterpret_cast<const float4 *>(x)[i];
+    const size_t n4 = n / 4, stride = (size_t)gridDim.x * kThreads;
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+    float4 *z4 = reinterpret_cast<float4 *>(z), *y4 = reinterpret_cast<float4 *>(y);
+    auto finish = [&](size_t i, const float4 &xv, const float4 *w) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
         for (int j = 0; j < K; j++) {
-                const float4 w = reinterpret_cast<const float4 *>(v.p[j])[i];
-                s0 += v.c[j] * w.x;
-                s1 += v.c[j] * w.y;
-                s2 += v.c[j] * w.z;
-                s3 += v.c[j] * w.w;
-            }
+            s0 += v.c[j] * w[j].x;
+            s1 += v.c[j] * w[j].y;
+            s2 += v.c[j] * w[j].z;
+            s3 += v.c[j] * w[j].w;
+        }
         const float4 zs = make_float4((float)s0, (float)s1, (float)s2, (float)s3);
-        reinterpret_cast<float4 *>(z)[i] = zs;
-        reinterpret_cast<float4 *>(y)[i] =
-            make_float4((float)((double)xv.x + a * zs.x), (float)((double)xv.y + a * zs.y),
-                        (float)((double)xv.z + a * zs.z), (float)((double)xv.w + a * zs.w));
+        z4[i] = zs;
+        y4[i] = make_float4((float)((double)xv.x + a * zs.x), (float)((double)xv.y + a * zs.y),
+                            (float)((double)xv.z + a * zs.z), (float)((double)xv.w + a * zs.w));
+    };
+    // two element groups per iteration, all their loads issued before either is used
+    size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    for (; i + stride < n4; i += 2 * stride) {
+        const float4 xa = x4[i], xb = x4[i + stride];
+        float4 wa[K], wb[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            wa[j] = reinterpret_cast<const float4 *>(v.p[j])[i];
+            wb[j] = reinterpret_cast<const float4 *>(v.p[j])[i + stride];
+        }
+        finish(i, xa, wa);
+        finish(i + stride, xb, wb);
+    }
+    if (i < n4) {
+        float4 wa[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) wa[j] = reinterpret_cast<const float4 *>(v.p[j])[i];
+        finish(i, x4[i], wa);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         for (size_t t = 4 * n4; t < n; t++) {
