@@ -464,8 +464,7 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     }
     __syncthreads();
     V *dst = proj + r0 * NP;
-    for (int i = tid; i < npts * NP; i += kThreads) {
-        const int pt = i / NP, ip = i - pt * NP;
+    auto theta = [&](int pt, int ip) {
         const V *f = F + (pt * NTH * W + (pmup[ip] + L)) * H1 + pmu2[ip];
         const T *a = a2p + ip * NTH;
         T re = 0, im = 0;
@@ -475,7 +474,20 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
             re += v.x * a[it];
             im += v.y * a[it];
         }
-        dst[i] = mk<V>(re * fmv[ip], im * fmv[ip]);
+        return mk<V>(re * fmv[ip], im * fmv[ip]);
+    };
+    if constexpr (sizeof(T) == 4 && NP % 2 == 0) {
+        // FP32: two projections of a point per thread, one 16-byte store
+        for (int i = tid; i < npts * (NP / 2); i += kThreads) {
+            const int pt = i / (NP / 2), ip = 2 * (i - pt * (NP / 2));
+            const V u0 = theta(pt, ip), u1 = theta(pt, ip + 1);
+            reinterpret_cast<float4 *>(dst)[i] = make_float4(u0.x, u0.y, u1.x, u1.y);
+        }
+    } else {
+        for (int i = tid; i < npts * NP; i += kThreads) {
+            const int pt = i / NP, ip = i - pt * NP;
+            dst[i] = theta(pt, ip);
+        }
     }
 }
 
@@ -504,7 +516,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
     if constexpr (DOT32) mode = 0;
-    __shared__ V Gm[PT * NP];
+    __shared__ __align__(16) V Gm[PT * NP];
     __shared__ V f[PT * NTH * W * H1];
     __shared__ V h[PT * NTH * NPHI * H1];
     __shared__ V twphi[NPHI * W];   // (cos, sin)(2 pi mup iphi / nphi) at [iphi][mup + L]
@@ -545,7 +557,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
         const float *xi32t = DOT32 ? xi32 + r0 * NO : nullptr, *vx32t = DOT32 ? vext32 + r0 * NO : nullptr;
         float *df32t = DOT32 ? df32 + r0 * NO : nullptr;
         const V *src = proj + r0 * NP;
-        for (int i = tl; i < npts * NP; i += kThreads) Gm[i] = src[i];
+        if constexpr (sizeof(T) == 4 && NP % 2 == 0) {  // FP32: 16-byte loads, two projections each
+            for (int i = tl; i < npts * (NP / 2); i += kThreads)
+                reinterpret_cast<float4 *>(Gm)[i] = reinterpret_cast<const float4 *>(src)[i];
+        } else {
+            for (int i = tl; i < npts * NP; i += kThreads) Gm[i] = src[i];
+        }
         __syncthreads();
         // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m): one thread per
         // (point, mup, mu2), all theta at once (each projection read once)
