@@ -588,28 +588,43 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
             for (int it = 0; it < NTH; it++) f[((pt * NTH + it) * W + (mup + L)) * H1 + mu2] = mk<V>(re[it], im[it]);
         }
         __syncthreads();
-        for (int i = tl; i < npts * NTH * NPHI * H1; i += kThreads) {
-            const int mu2 = i % H1;
-            int rest = i / H1;
-            const int iphi = rest % NPHI;
-            rest /= NPHI;
+        // h(theta, phi, mu2): the sum over mup of f(theta, mu2, mup) with the phi twiddles; items
+        // mu2-major, so that whole waves take mu2 = 0, whose imaginary part the psi synthesis
+        // below never reads
+        const int nline = npts * NTH * NPHI;  // (point, theta, phi) lines
+        for (int i = tl; i < nline * H1; i += kThreads) {
+            const int mu2 = i / nline, line = i - mu2 * nline;
+            const int iphi = line % NPHI, rest = line / NPHI;
             const V *fr = f + rest * W * H1 + mu2;
             const V *tw = twphi + iphi * W;
             T re = 0, im = 0;
+            if (mu2 == 0) {
     #pragma unroll
-            for (int j = 0; j < W; j++) {
-                const V v = fr[j * H1], e = tw[j];
-                if constexpr (sizeof(T) == 4) {  // four multiply-adds (FP32 only, as in angl2proj_t)
-                    re = fma(v.x, e.x, re);
-                    re = fma(v.y, e.y, re);
-                    im = fma(v.x, e.y, im);
-                    im = fma(-v.y, e.x, im);
-                } else {
-                    re += v.x * e.x + v.y * e.y;
-                    im += v.x * e.y - v.y * e.x;
+                for (int j = 0; j < W; j++) {
+                    const V v = fr[j * H1], e = tw[j];
+                    if constexpr (sizeof(T) == 4) {
+                        re = fma(v.x, e.x, re);
+                        re = fma(v.y, e.y, re);
+                    } else {
+                        re += v.x * e.x + v.y * e.y;
+                    }
+                }
+            } else {
+    #pragma unroll
+                for (int j = 0; j < W; j++) {
+                    const V v = fr[j * H1], e = tw[j];
+                    if constexpr (sizeof(T) == 4) {  // four multiply-adds (FP32 only, as in angl2proj_t)
+                        re = fma(v.x, e.x, re);
+                        re = fma(v.y, e.y, re);
+                        im = fma(v.x, e.y, im);
+                        im = fma(-v.y, e.x, im);
+                    } else {
+                        re += v.x * e.x + v.y * e.y;
+                        im += v.x * e.y - v.y * e.x;
+                    }
                 }
             }
-            h[i] = mk<V>(re, im);
+            h[line * H1 + mu2] = mk<V>(re, im);
         }
         __syncthreads();
         // the tile's xi and vext, all of a thread's loads issued together for the loop below;
