@@ -327,6 +327,38 @@ struct Dims {
     static constexpr int NP = np_of(L);
 };
 
+// x / D for 0 <= x < NMAX as (x * M) >> S with a 24-bit multiply (one full-rate instruction;
+// the 32-bit multiply-high of the compiler's constant division issues at a quarter rate).  M
+// and S are chosen at compile time and checked against x / D for every x in the range.
+template <unsigned D, unsigned NMAX>
+struct DivC {
+    static constexpr bool exact(unsigned long long m, unsigned s) {
+        for (unsigned x = 0; x < NMAX; x++)
+            if ((unsigned)((x * m) >> s) != x / D) return false;
+        return true;
+    }
+    static constexpr unsigned pick() {
+        for (unsigned s = 0; s < 32; s++) {
+            const unsigned long long m = ((1ull << s) + D - 1) / D;
+            if (m < (1ull << 24) && (NMAX - 1) * m < (1ull << 32) && exact(m, s)) return s;
+        }
+        return 32;
+    }
+    static constexpr unsigned S = pick();
+    static_assert(S < 32 && NMAX <= (1u << 24), "no 24-bit reciprocal for this divisor and range");
+    static constexpr unsigned M = (unsigned)(((1ull << S) + D - 1) / D);
+    __device__ static unsigned q(unsigned x) { return __umul24(x, M) >> S; }
+};
+
+// a * b for a, b < 2^24 and a product below 2^32, kept a 24-bit multiply: the empty asm stops
+// the compiler from folding the product into a subtraction or a 64-bit address, where it would
+// become a quarter-rate 32-bit multiply again
+__device__ inline unsigned mul24(unsigned a, unsigned b) {
+    unsigned r = __umul24(a, b);
+    asm("" : "+v"(r));
+    return r;
+}
+
 // The constant tables of angl2proj_t in the layout of its LDS.  The image is built once per
 // context and precision (a2p_tab_build, the staging arithmetic run once) and each workgroup
 // copies it with 16-byte loads: the index arithmetic and FP64 -> T conversions of the
@@ -503,9 +535,10 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
 // each use counts exactly the memory operations issued after it, so the product operands
 // loaded ahead stay in flight.
 template <int L, int PT, typename T, bool DOT32 = false>
-// DOT32: 7 waves per SIMD (the LDS limit at mmax = 5, FP32), at most 72 registers; the general
-// variant, with every mode's branches, may take more registers rather than spill
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 ? 7 : 4, 7))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
+// DOT32: at least 6 waves per SIMD (at most 80 registers: the tile's iterate and vext, loaded
+// before the synthesis phases, stay in registers through them; at 7 waves and 72 registers they
+// spill); the general variant, with every mode's branches, may take more registers rather than spill
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 ? 6 : 4, 7))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode, float *__restrict__ df32,
@@ -524,9 +557,31 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
     __shared__ T p2a[NP * NTH];
     __shared__ double wq[NO];
     __shared__ double sth[NTH];  // 1 / s of each theta (the weights, hence s, depend on theta only)
-    __shared__ int pidx[(L + 1) * W * H1];
+    // projection index of (m, mup, mu2), as bytes: the loaded index is then known to be small and
+    // its products stay 24-bit multiplies (entries with m below the item's lowest m are never read)
+    static_assert(NP <= 255, "projection indices are stored as bytes");
+    __shared__ unsigned char pidx[(L + 1) * W * H1];
+    __shared__ unsigned char ford[W * H1];  // the theta synthesis' (mup, mu2) items, longest m loop first
     const int tid = threadIdx.x;
-    for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = t.pidx[i];
+    for (int i = tid; i < (L + 1) * W * H1; i += kThreads) pidx[i] = (unsigned char)t.pidx[i];
+    {
+        // the item (mup + L) * H1 + mu2 sums over m = max(|mup|, 2 mu2) .. L; ranked by that
+        // length (ties by index), the lanes of a wave take loops of nearly one length
+        auto len = [](int it) {
+            const int a = it / H1 - L, b = 2 * (it % H1);
+            const int aa = a < 0 ? -a : a;
+            return L + 1 - (aa > b ? aa : b);
+        };
+        for (int i = tid; i < W * H1; i += kThreads) {
+            const int li = len(i);
+            int r = 0;
+            for (int j = 0; j < W * H1; j++) {
+                const int lj = len(j);
+                r += (lj > li || (lj == li && j < i)) ? 1 : 0;
+            }
+            ford[r] = (unsigned char)i;
+        }
+    }
     for (int i = tid; i < NPHI * W; i += kThreads) {
         const int iphi = i / W, mup = i % W - L;
         int kk = (mup * iphi) % NPHI;
@@ -563,20 +618,39 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
         } else {
             for (int i = tl; i < npts * NP; i += kThreads) Gm[i] = src[i];
         }
+        // DOT32: the tile's FP32 iterate and FP32 vext, loaded here so that their latency
+        // passes during the synthesis phases
+        constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
+        float xr32[NIT], vr32[NIT];
+        if constexpr (DOT32) {
+    #pragma unroll
+            for (int q = 0; q < NIT; q++) {
+                const int i = tl + q * kThreads;
+                const int ii = i < npts * NO ? i : npts * NO - 1;
+                xr32[q] = xi32t[ii];
+                vr32[q] = vx32t[ii];
+            }
+        }
         __syncthreads();
         // f(theta, mu2, mup) = sum_m p(m, mup, mu2) d(theta) fm(m): one thread per
-        // (point, mup, mu2), all theta at once (each projection read once)
-        for (int i = tl; i < npts * W * H1; i += kThreads) {
-            const int mu2 = i % H1;
-            const int mup = (i / H1) % W - L, pt = i / (H1 * W);
+        // (point, mup, mu2), all theta at once (each projection read once); items by rank of
+        // their loop length, the tile's points innermost
+        for (int i = tl; i < PT * W * H1; i += kThreads) {
+            const int rk = i / PT, pt = i - rk * PT;
+            if (pt >= npts) continue;
+            const int item = ford[rk];
+            const int jm = (int)DivC<H1, W * H1>::q(item);
+            const int mu2 = item - jm * H1, mup = jm - L;
             const int a = mup < 0 ? -mup : mup, b = 2 * mu2;
             const int mlo = a > b ? a : b;
             T re[NTH], im[NTH];
     #pragma unroll
             for (int it = 0; it < NTH; it++) re[it] = im[it] = (T)0;
+            const unsigned char *pp = pidx + (mul24(jm, H1) + mu2);  // the item's entry of each m row
+            const V *gp = Gm + mul24(pt, NP);
             for (int m = mlo; m <= L; m++) {
-                const int p = pidx[(m * W + mup + L) * H1 + mu2];
-                const V v = Gm[pt * NP + p];
+                const unsigned p = pp[m * (W * H1)];
+                const V v = gp[p];
                 const T *c = p2a + p * NTH;
     #pragma unroll
                 for (int it = 0; it < NTH; it++) {
@@ -584,19 +658,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                     im[it] += v.y * c[it];
                 }
             }
+            V *fo = f + (mul24(mul24(pt, NTH * W) + jm, H1) + mu2);
     #pragma unroll
-            for (int it = 0; it < NTH; it++) f[((pt * NTH + it) * W + (mup + L)) * H1 + mu2] = mk<V>(re[it], im[it]);
+            for (int it = 0; it < NTH; it++) fo[it * (W * H1)] = mk<V>(re[it], im[it]);
         }
         __syncthreads();
         // h(theta, phi, mu2): the sum over mup of f(theta, mu2, mup) with the phi twiddles; items
         // mu2-major, so that whole waves take mu2 = 0, whose imaginary part the psi synthesis
         // below never reads
-        const int nline = npts * NTH * NPHI;  // (point, theta, phi) lines
-        for (int i = tl; i < nline * H1; i += kThreads) {
-            const int mu2 = i / nline, line = i - mu2 * nline;
-            const int iphi = line % NPHI, rest = line / NPHI;
-            const V *fr = f + rest * W * H1 + mu2;
-            const V *tw = twphi + iphi * W;
+        constexpr int NL = PT * NTH * NPHI;  // (point, theta, phi) lines of a full tile
+        for (int i = tl; i < NL * H1; i += kThreads) {
+            const int mu2 = (int)DivC<NL, NL * H1>::q(i), line = i - mu2 * NL;
+            if (line >= npts * NTH * NPHI) continue;
+            const int rest = (int)DivC<NPHI, NL>::q(line), iphi = line - rest * NPHI;
+            const V *fr = f + (mul24(rest, W * H1) + mu2);
+            const V *tw = twphi + mul24(iphi, W);
             T re = 0, im = 0;
             if (mu2 == 0) {
     #pragma unroll
@@ -624,24 +700,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                     }
                 }
             }
-            h[line * H1 + mu2] = mk<V>(re, im);
+            h[mul24(line, H1) + mu2] = mk<V>(re, im);
         }
         __syncthreads();
-        // the tile's xi and vext, all of a thread's loads issued together for the loop below;
-        // the other workgroups on the compute unit cover their latency (loaded before the
-        // synthesis phases, they would hold registers through them)
-        constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
+        // the general variant: the tile's xi and vext, all of a thread's loads issued together
+        // for the loop below; the other workgroups on the compute unit cover their latency
+        // (loaded before the synthesis phases, these FP64 values would hold twice the registers)
         double xr[NIT], vr[NIT];
-        float xr32[NIT], vr32[NIT];  // DOT32: the FP32 iterate and the FP32 copy of vext
     #pragma unroll
         for (int q = 0; q < NIT; q++) {
+            if constexpr (DOT32) continue;
             const int i = tl + q * kThreads;
-            if constexpr (DOT32) {
-                const int ii = i < npts * NO ? i : npts * NO - 1;
-                xr32[q] = xi32t[ii];
-                vr32[q] = vx32t[ii];
-                continue;
-            }
             xr[q] = 0.0;
             vr[q] = 0.0;
             if (i < npts * NO) {
@@ -688,10 +757,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
             } else {
                 xq = xi32 ? held_f32(xr[q]) : xr[q];
             }
-            const int pt = i / NO, io = i - pt * NO;
-            const int ipsi = io % NPSI, line = io / NPSI;  // line = it*NPHI + iphi
-            const V *hr = h + (pt * NTH * NPHI + line) * H1;
-            const V *tp = twpsi + ipsi * H1;
+            // LDS offsets by 24-bit products: every operand here is small and non-negative
+            const unsigned pt = DivC<NO, PT * NO>::q(i);
+            const int io = i - (int)mul24(pt, NO);
+            const unsigned line = DivC<NPSI, NO>::q(io), ipsi = io - mul24(line, NPSI);  // line = it*NPHI + iphi
+            const V *hr = h + mul24(mul24(pt, NTH * NPHI) + line, H1);
+            const V *tp = twpsi + mul24(ipsi, H1);
             T vt = hr[0].x;
     #pragma unroll
             for (int mu2 = 1; mu2 < H1; mu2++) {
@@ -705,7 +776,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                     // are FP32 already), summed in FP32 over the thread's elements of the tile
                     // and in FP64 over the tiles; rho/rho0 - 1 is formed in FP64 first so that
                     // the excess term keeps its small deviations from the bulk
-                    const double si = vsinv ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
+                    const double si = vsinv ? sth[DivC<NPHI * NPSI, NO>::q(io)] : 1.0;  // scaled variables: xi = u / s, dF/du = dF/dxi / s
                     const double x = xq * si;
                     const double xx = x * x;
                     const float wf = (float)wq[io];
@@ -757,7 +828,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
             const double v = (double)vt;
             const double wio = wq[io];
             const double vexc = k.prefactor * wio * v;
-            const double si = (mode == 0 && vsinv) ? sth[io / (NPHI * NPSI)] : 1.0;  // scaled variables (mode 0)
+            const double si = (mode == 0 && vsinv) ? sth[DivC<NPHI * NPSI, NO>::q(io)] : 1.0;  // scaled variables (mode 0)
             const double x = xq * si;
             const double rho = x * x * rho0;
             fexc += (x * x * rho0 - rho0) * vexc;
