@@ -535,10 +535,11 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
 // each use counts exactly the memory operations issued after it, so the product operands
 // loaded ahead stay in flight.
 template <int L, int PT, typename T, bool DOT32 = false>
-// DOT32: at least 6 waves per SIMD (at most 80 registers: the tile's iterate and vext, loaded
-// before the synthesis phases, stay in registers through them; at 7 waves and 72 registers they
-// spill); the general variant, with every mode's branches, may take more registers rather than spill
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 ? 6 : 4, 7))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
+// DOT32: at least 5 waves per SIMD (at most 96 registers): the tile's iterate, vext and product
+// operands, loaded before the synthesis phases, stay in registers through them (measured faster
+// than 6 or 7 waves with fewer of them in flight); the general variant, with every mode's
+// branches, may take more registers rather than spill
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 ? 5 : 4, 7))) void final_t(const typename C2<T>::t *__restrict__ proj, const double *__restrict__ xi,
                                                    const double *__restrict__ vext, double *__restrict__ df,
                                                    double *__restrict__ partials, size_t N, AngTables t,
                                                    FinalConsts k, int mode, float *__restrict__ df32,
@@ -618,10 +619,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
         } else {
             for (int i = tl; i < npts * NP; i += kThreads) Gm[i] = src[i];
         }
-        // DOT32: the tile's FP32 iterate and FP32 vext, loaded here so that their latency
-        // passes during the synthesis phases
+        // DOT32: the tile's FP32 iterate, FP32 vext and the FP32 operands of the first two
+        // gradient products, loaded here so that their latency passes during the synthesis
+        // phases; an operand that is not needed (the gradient itself, or beyond da.k) is read
+        // from dF, which exists here
         constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
         float xr32[NIT], vr32[NIT];
+        float dq[2][NIT];
+        const float *dp0 = (da.k > 0 && !(da.self & 1)) ? static_cast<const float *>(da.p[0]) : df32;
+        const float *dp1 = (da.k > 1 && !((da.self >> 1) & 1)) ? static_cast<const float *>(da.p[1]) : df32;
+        const float *dp0t = DOT32 ? dp0 + r0 * NO : nullptr, *dp1t = DOT32 ? dp1 + r0 * NO : nullptr;
         if constexpr (DOT32) {
     #pragma unroll
             for (int q = 0; q < NIT; q++) {
@@ -629,6 +636,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                 const int ii = i < npts * NO ? i : npts * NO - 1;
                 xr32[q] = xi32t[ii];
                 vr32[q] = vx32t[ii];
+                dq[0][q] = dp0t[ii];
+                dq[1][q] = dp1t[ii];
             }
         }
         __syncthreads();
@@ -719,32 +728,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
                 if (mode == 0 || mode == 3 || mode == 4) vr[q] = vext[g];
             }
         }
-        // DOT32: FP32 operands of the first two gradient products, loaded two elements ahead of
-        // their use (a load next to its use would make every element wait for HBM); an operand that
-        // is not needed (the gradient itself, or beyond da.k) is read from dF, which exists here
-        constexpr int kAhead = 2;
-        float dq[2][NIT];
-        const float *dp0 = (da.k > 0 && !(da.self & 1)) ? static_cast<const float *>(da.p[0]) : df32;
-        const float *dp1 = (da.k > 1 && !((da.self >> 1) & 1)) ? static_cast<const float *>(da.p[1]) : df32;
-        const float *dp0t = DOT32 ? dp0 + r0 * NO : nullptr, *dp1t = DOT32 ? dp1 + r0 * NO : nullptr;
-        auto load_dq = [&](int q) {
-            const int i = tl + q * kThreads;
-            const int ii = i < npts * NO ? i : npts * NO - 1;
-            dq[0][q] = dp0t[ii];
-            dq[1][q] = dp1t[ii];
-        };
         // FP32 path: the free-energy terms of a thread's elements in this tile, summed in FP32
         // and added to the FP64 sums once per tile
         float fexc_t = 0.0f, fid_t = 0.0f, fext_t = 0.0f;
-        if constexpr (DOT32) {
-    #pragma unroll
-            for (int q = 0; q < kAhead && q < NIT; q++) load_dq(q);
-        }
     #pragma unroll
         for (int q = 0; q < NIT; q++) {
-            if constexpr (DOT32) {
-                if (q + kAhead < NIT) load_dq(q + kAhead);
-            }
             const int i = tl + q * kThreads;
             if (i >= npts * NO) break;
             double xq;
