@@ -397,65 +397,17 @@ __global__ __launch_bounds__(kThreads) void a2p_tab_build(A2pTab<L, T> *__restri
     }
 }
 
-// feat = 0: o = rho0 (xi^2 - 1) (the excess term); feat = 1: o = clip(-beta xi / 2, -4, 4) / 4
-// with xi the external potential -- the log of the square-root Boltzmann factor, bounded
-// (features of the learned warm start); feat = 2: o = xi^2 - 1 = rho/rho0 - 1 (the
-// warm start's density target; 0 in the bulk, -1 where the solvent is excluded); feat = 3:
-// o = k0 aux / (xi w(io)) where xi^2 > MDFT_WS_DENSITY_FLOOR, else 0, with aux = dF at the
-// density guess xi and k0 = dv / 2: dF/dr / w for r = rho/rho0, whose angl2proj is the
-// adjoint of proj2angl up to a per-projection factor (the warm start's training gradient);
-// feat = 4: o = xi (the plain transform). k0 is rho0 for feat 0.
+// The psi, phi and theta steps of angl2proj over one tile (o in LDS and visible to every
+// thread): G (psi DFT), F (phi DFT), then the theta contraction into the tile's projections.
 template <int L, int PT, typename T>
-__global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
-                                                       size_t N, AngTables t, double k0, int feat, double beta,
-                                                       const double *__restrict__ aux,
-                                                       const double *__restrict__ vsinv,
-                                                       const float *__restrict__ xi32,
-                                                       const A2pTab<L, T> *__restrict__ img) {
+__device__ inline void a2p_phases(const T *o, typename C2<T>::t *G, typename C2<T>::t *F, const A2pTab<L, T> &tb,
+                                  typename C2<T>::t *__restrict__ dst, int npts, int tid) {
     using D = Dims<L>;
     using V = typename C2<T>::t;
-    constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
-    __shared__ T o[PT * NO];
-    __shared__ V G[PT * NTH * NPHI * H1];
-    __shared__ V F[PT * NTH * W * H1];
-    __shared__ A2pTab<L, T> tb;
+    constexpr int NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, NPSI = D::NPSI, H1 = D::H1, W = D::W;
     const V *twpsi = tb.twpsi, *twphi = tb.twphi;
     const T *a2p = tb.a2p, *fmv = tb.fmv;
     const int *pmup = tb.pmup, *pmu2 = tb.pmu2;
-    const int tid = threadIdx.x;
-    const size_t r0 = (size_t)blockIdx.x * PT;
-    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
-    // the tile's FP64 input is loaded into registers before the tables are staged,
-    // so its HBM latency overlaps the staging
-    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
-    const double *src = xi + r0 * NO;
-    double xr[NIT], sr[NIT];
-#pragma unroll
-    for (int q = 0; q < NIT; q++) {
-        const int i = tid + q * kThreads;
-        xr[q] = (i < npts * NO) ? (xi32 ? hold_f32(xi32[r0 * NO + i]) : src[i]) : 0.0;  // FP32 iterate: xi32
-        sr[q] = vsinv ? vsinv[i % NO] : 1.0;  // scaled variables: xi = u / s(io), loaded with the tile
-    }
-    {
-        constexpr int n16 = (int)(sizeof(A2pTab<L, T>) / 16);
-        const uint4 *src16 = reinterpret_cast<const uint4 *>(img);
-        uint4 *dst16 = reinterpret_cast<uint4 *>(&tb);
-        for (int i = tid; i < n16; i += kThreads) dst16[i] = src16[i];
-    }
-#pragma unroll
-    for (int q = 0; q < NIT; q++) {
-        const int i = tid + q * kThreads;
-        if (i >= npts * NO) break;
-        const double xv = xi32 ? held_f32(xr[q]) : xr[q];
-        const double x = vsinv ? xv * sr[q] : xv;
-        if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
-        else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
-        else if (feat == 2) o[i] = (T)(x * x - 1.0);
-        else if (feat == 4) o[i] = (T)x;
-        else o[i] = (x * x > MDFT_WS_DENSITY_FLOOR * (1.0 + 1e-12)) ? (T)(k0 * aux[r0 * NO + i] / (x * t.w[i % NO]))
-                                                                   : (T)0;
-    }
-    __syncthreads();
     for (int i = tid; i < npts * NTH * NPHI * H1; i += kThreads) {
         const int mu2 = i % H1;
         const T *line = o + (i / H1) * NPSI;
@@ -495,7 +447,6 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
         F[i] = mk<V>(re, im);
     }
     __syncthreads();
-    V *dst = proj + r0 * NP;
     auto theta = [&](int pt, int ip) {
         const V *f = F + (pt * NTH * W + (pmup[ip] + L)) * H1 + pmu2[ip];
         const T *a = a2p + ip * NTH;
@@ -521,6 +472,65 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
             dst[i] = theta(pt, ip);
         }
     }
+}
+
+// feat = 0: o = rho0 (xi^2 - 1) (the excess term); feat = 1: o = clip(-beta xi / 2, -4, 4) / 4
+// with xi the external potential -- the log of the square-root Boltzmann factor, bounded
+// (features of the learned warm start); feat = 2: o = xi^2 - 1 = rho/rho0 - 1 (the
+// warm start's density target; 0 in the bulk, -1 where the solvent is excluded); feat = 3:
+// o = k0 aux / (xi w(io)) where xi^2 > MDFT_WS_DENSITY_FLOOR, else 0, with aux = dF at the
+// density guess xi and k0 = dv / 2: dF/dr / w for r = rho/rho0, whose angl2proj is the
+// adjoint of proj2angl up to a per-projection factor (the warm start's training gradient);
+// feat = 4: o = xi (the plain transform). k0 is rho0 for feat 0.
+template <int L, int PT, typename T>
+__global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict__ xi, typename C2<T>::t *__restrict__ proj,
+                                                       size_t N, AngTables t, double k0, int feat, double beta,
+                                                       const double *__restrict__ aux,
+                                                       const double *__restrict__ vsinv,
+                                                       const float *__restrict__ xi32,
+                                                       const A2pTab<L, T> *__restrict__ img) {
+    using D = Dims<L>;
+    using V = typename C2<T>::t;
+    constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, H1 = D::H1, W = D::W;
+    __shared__ T o[PT * NO];
+    __shared__ V G[PT * NTH * NPHI * H1];
+    __shared__ V F[PT * NTH * W * H1];
+    __shared__ A2pTab<L, T> tb;
+    const int tid = threadIdx.x;
+    const size_t r0 = (size_t)blockIdx.x * PT;
+    const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
+    // the tile's FP64 input is loaded into registers before the tables are staged,
+    // so its HBM latency overlaps the staging
+    constexpr int NIT = (PT * NO + kThreads - 1) / kThreads;
+    const double *src = xi + r0 * NO;
+    double xr[NIT], sr[NIT];
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+        const int i = tid + q * kThreads;
+        xr[q] = (i < npts * NO) ? (xi32 ? hold_f32(xi32[r0 * NO + i]) : src[i]) : 0.0;  // FP32 iterate: xi32
+        sr[q] = vsinv ? vsinv[i % NO] : 1.0;  // scaled variables: xi = u / s(io), loaded with the tile
+    }
+    {
+        constexpr int n16 = (int)(sizeof(A2pTab<L, T>) / 16);
+        const uint4 *src16 = reinterpret_cast<const uint4 *>(img);
+        uint4 *dst16 = reinterpret_cast<uint4 *>(&tb);
+        for (int i = tid; i < n16; i += kThreads) dst16[i] = src16[i];
+    }
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+        const int i = tid + q * kThreads;
+        if (i >= npts * NO) break;
+        const double xv = xi32 ? held_f32(xr[q]) : xr[q];
+        const double x = vsinv ? xv * sr[q] : xv;
+        if (feat == 0) o[i] = (T)(k0 * (x * x - 1.0));
+        else if (feat == 1) o[i] = (T)(0.25 * fmin(fmax(-0.5 * beta * x, -4.0), 4.0));
+        else if (feat == 2) o[i] = (T)(x * x - 1.0);
+        else if (feat == 4) o[i] = (T)x;
+        else o[i] = (x * x > MDFT_WS_DENSITY_FLOOR * (1.0 + 1e-12)) ? (T)(k0 * aux[r0 * NO + i] / (x * t.w[i % NO]))
+                                                                   : (T)0;
+    }
+    __syncthreads();
+    a2p_phases<L, PT, T>(o, G, F, tb, proj + r0 * NP, npts, tid);
 }
 
 // Persistent: each workgroup stages the tables once and then takes tiles blockIdx.x,
@@ -869,9 +879,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DOT32 
 
 constexpr int kPT = 4;  // points per workgroup tile (keeps the LDS tile under 64 KB at mmax = 5)
 
-// Workgroups of a persistent transform: 8 per compute unit (32 waves, the occupancy the
-// LDS of the mmax = 5 kernels allows), never more than the tiles.
-size_t persistent_grid(const mdft_ctx *c, size_t ntiles) {
+// Workgroups of a persistent kernel, never more than the tiles.  resident_grid: as many as
+// the compute units hold at once (registers and LDS of that kernel); a larger grid leaves
+// workgroups waiting for a slot until others have finished all their tiles.  The fast final
+// pass runs faster so (3.85 -> 3.77 ms at 128^3, mmax 5); the general final pass, at 3 (FP64,
+// the LDS limit) or 4 (FP32) resident workgroups per compute unit, runs faster with 8 per
+// compute unit (FP64 6.4 against 6.9 ms), the later ones taking the slots of the first to finish.
+template <typename K>
+size_t resident_grid(const mdft_ctx *c, K kernel, size_t ntiles) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const size_t g = (size_t)per_cu * (size_t)(c->num_cus > 0 ? c->num_cus : 256);
+    return ntiles < g ? ntiles : g;
+}
+size_t oversubscribed_grid(const mdft_ctx *c, size_t ntiles) {
     const size_t g = 8 * (size_t)(c->num_cus > 0 ? c->num_cus : 256);
     return ntiles < g ? ntiles : g;
 }
@@ -923,8 +944,7 @@ template <int PT, typename T>
 bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi, double *df, int mode, FinalConsts k,
                     size_t npts = 0, size_t *grid_out = nullptr) {
     if (!npts) npts = c->N;
-    const size_t nb = persistent_grid(c, (npts + PT - 1) / PT);
-    if (grid_out) *grid_out = nb;
+    const size_t ntiles = (npts + PT - 1) / PT;
     DotArgs da{};
     if (mode == 0 && c->dot_k > 0) {
         da.k = c->dot_k;
@@ -937,8 +957,15 @@ bool launch_final_t(mdft_ctx *c, const typename C2<T>::t *proj, const double *xi
     switch (c->L) {
 #define FIN_CASE(LL) \
     case LL: \
-        if (dot32) hipLaunchKernelGGL((final_t<LL, PT, T, sizeof(T) == 4>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, c->df32, c->vsinv, da, c->d_dpart, c->xi32, c->d_vext32); \
-        else hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr, da, c->d_dpart, mode == 0 ? c->xi32 : nullptr, nullptr); \
+        if (dot32) { \
+            const size_t nb = resident_grid(c, final_t<LL, PT, T, sizeof(T) == 4>, ntiles); \
+            if (grid_out) *grid_out = nb; \
+            hipLaunchKernelGGL((final_t<LL, PT, T, sizeof(T) == 4>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, c->df32, c->vsinv, da, c->d_dpart, c->xi32, c->d_vext32); \
+        } else { \
+            const size_t nb = oversubscribed_grid(c, ntiles); \
+            if (grid_out) *grid_out = nb; \
+            hipLaunchKernelGGL((final_t<LL, PT, T>), dim3((unsigned)nb), dim3(kThreads), 0, c->stream, proj, xi, c->d_vext, df, c->d_partials, npts, c->ang, k, mode, mode == 0 ? c->df32 : nullptr, mode == 0 ? c->vsinv : nullptr, da, c->d_dpart, mode == 0 ? c->xi32 : nullptr, nullptr); \
+        } \
         return true;
         FIN_CASE(0) FIN_CASE(1) FIN_CASE(2) FIN_CASE(3) FIN_CASE(4) FIN_CASE(5)
 #undef FIN_CASE
