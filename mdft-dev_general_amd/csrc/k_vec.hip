@@ -472,6 +472,18 @@ unsigned blocks_for(size_t n) {
     return (unsigned)b;
 }
 
+// blocks_for capped at the workgroups the compute units hold at once for this kernel: a
+// grid-stride kernel given more ends on a partial round of workgroups, at a fraction of the
+// memory parallelism
+template <typename K>
+unsigned resident_blocks(const mdft_ctx *c, K kernel, size_t n) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const size_t g = (size_t)per_cu * (size_t)(c->num_cus > 0 ? c->num_cus : 256);
+    const unsigned b = blocks_for(n);
+    return g < b ? (unsigned)g : b;
+}
+
 hipError_t ensure_vec_partials(mdft_ctx *c) {
     if (c->d_vpart) return hipSuccess;
     return hipMalloc(&c->d_vpart, sizeof(double) * (kMaxBlocks + 8));
@@ -493,7 +505,7 @@ hipError_t finish_sum(mdft_ctx *c, unsigned nb, double *out) {
 hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, double *out) {
     hipError_t e = ensure_vec_partials(c);
     if (e != hipSuccess) return e;
-    const unsigned nb = blocks_for(n);
+    const unsigned nb = resident_blocks(c, dot_partial, n);
     hipLaunchKernelGGL(dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, x, y, c->d_vpart);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return finish_sum(c, nb, out);
@@ -502,7 +514,7 @@ hipError_t vec_dot(mdft_ctx *c, size_t n, const double *x, const double *y, doub
 hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double *y, const double *z, double *out) {
     hipError_t e = ensure_vec_partials(c);
     if (e != hipSuccess) return e;
-    const unsigned nb = blocks_for(n);
+    const unsigned nb = resident_blocks(c, axpy_dot_partial, n);
     hipLaunchKernelGGL(axpy_dot_partial, dim3(nb), dim3(kThreads), 0, c->stream, n, a, x, y, z, c->d_vpart);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (!z || !out) return hipStreamSynchronize(c->stream);
@@ -510,7 +522,7 @@ hipError_t vec_axpy_dot(mdft_ctx *c, size_t n, double a, const double *x, double
 }
 
 hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double b, const double *y, double *z) {
-    const unsigned nb = blocks_for(2 * n);
+    const unsigned nb = resident_blocks(c, lincomb, 2 * n);
     hipLaunchKernelGGL(lincomb, dim3(nb), dim3(kThreads), 0, c->stream, n, a, x, b, y, z);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -518,7 +530,7 @@ hipError_t vec_lincomb(mdft_ctx *c, size_t n, double a, const double *x, double 
 }
 
 hipError_t launch_widen_f32(mdft_ctx *c, const float2 *in, double2 *out, size_t n) {
-    hipLaunchKernelGGL(widen, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
+    hipLaunchKernelGGL(widen, dim3(resident_blocks(c, widen, 2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
     return hipGetLastError();
 }
 
@@ -533,8 +545,8 @@ hipError_t vec_multi_dot(mdft_ctx *c, size_t n, int k, const double *const *vecs
         if (vecs[j] == x) v.self |= 1 << j;
     }
     v.k = k;
-    const unsigned nb = blocks_for(n);
-#define L_(K) hipLaunchKernelGGL(multi_dot_partial<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart)
+    unsigned nb = 1;
+#define L_(K) nb = resident_blocks(c, multi_dot_partial<K>, n); hipLaunchKernelGGL(multi_dot_partial<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart)
     VEC_K_DISPATCH(k, L_)
 #undef L_
     hipError_t e = hipGetLastError();
@@ -553,7 +565,7 @@ hipError_t vec_combine(mdft_ctx *c, size_t n, int k, const double *coef, const d
         v.c[j] = coef[j];
     }
     v.k = k;
-#define L_(K) hipLaunchKernelGGL(combine<K>, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z)
+#define L_(K) hipLaunchKernelGGL(combine<K>, dim3(resident_blocks(c, combine<K>, 2 * n)), dim3(kThreads), 0, c->stream, n, v, z)
     VEC_K_DISPATCH(k, L_)
 #undef L_
     hipError_t e = hipGetLastError();
@@ -569,7 +581,7 @@ hipError_t vec_combine_step(mdft_ctx *c, size_t n, int k, const double *coef, co
         v.c[j] = coef[j];
     }
     v.k = k;
-#define L_(K) hipLaunchKernelGGL(combine_step<K>, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
+#define L_(K) hipLaunchKernelGGL(combine_step<K>, dim3(resident_blocks(c, combine_step<K>, 2 * n)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
     VEC_K_DISPATCH(k, L_)
 #undef L_
     hipError_t e = hipGetLastError();
@@ -623,8 +635,8 @@ hipError_t vec_multi_dot_noalloc(mdft_ctx *c, size_t n, int k, const double *con
         if (vecs[j] == x) v.self |= 1 << j;
     }
     v.k = k;
-    const unsigned nb = blocks_for(n);  // <= kMaxBlocks: every partial fits the array
-#define L_(K) hipLaunchKernelGGL(multi_dot_partial<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, s)
+    unsigned nb = 1;  // <= kMaxBlocks: every partial fits the array
+#define L_(K) nb = resident_blocks(c, multi_dot_partial<K>, n); hipLaunchKernelGGL(multi_dot_partial<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, s)
     VEC_K_DISPATCH(k, L_)
 #undef L_
     hipError_t e = hipGetLastError();
@@ -647,8 +659,8 @@ hipError_t vec_multi_dot32(mdft_ctx *c, size_t n, int k, const float *const *vec
         if (vecs[j] == x) v.self |= 1 << j;
     }
     v.k = k;
-    const unsigned nb = blocks_for(n / 2);
-#define L_(K) hipLaunchKernelGGL(multi_dot_partial32<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart)
+    unsigned nb = 1;
+#define L_(K) nb = resident_blocks(c, multi_dot_partial32<K>, n / 2); hipLaunchKernelGGL(multi_dot_partial32<K>, dim3(nb), dim3(kThreads), 0, c->stream, n, x, v, c->d_mpart)
     VEC_K_DISPATCH(k, L_)
 #undef L_
     hipError_t e = hipGetLastError();
@@ -668,7 +680,7 @@ hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, 
         v.c[j] = coef[j];
     }
     v.k = k;
-#define L_(K) hipLaunchKernelGGL(combine_step32<K>, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
+#define L_(K) hipLaunchKernelGGL(combine_step32<K>, dim3(resident_blocks(c, combine_step32<K>, n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
     VEC_K_DISPATCH(k, L_)
 #undef L_
     hipError_t e = hipGetLastError();
@@ -677,7 +689,7 @@ hipError_t vec_combine_step32(mdft_ctx *c, size_t n, int k, const double *coef, 
 }
 
 hipError_t vec_axpy32(mdft_ctx *c, size_t n, double a, const float *z, const double *x, double *y) {
-    hipLaunchKernelGGL(axpy32, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, a, z, x, y);
+    hipLaunchKernelGGL(axpy32, dim3(resident_blocks(c, axpy32, n / 2)), dim3(kThreads), 0, c->stream, n, a, z, x, y);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
@@ -691,7 +703,7 @@ hipError_t vec_combine_step32x(mdft_ctx *c, size_t n, int k, const double *coef,
         v.c[j] = coef[j];
     }
     v.k = k;
-#define L_(K) hipLaunchKernelGGL(combine_step32x<K>, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
+#define L_(K) hipLaunchKernelGGL(combine_step32x<K>, dim3(resident_blocks(c, combine_step32x<K>, n / 2)), dim3(kThreads), 0, c->stream, n, v, z, a, x, y)
     VEC_K_DISPATCH(k, L_)
 #undef L_
     hipError_t e = hipGetLastError();
@@ -700,28 +712,28 @@ hipError_t vec_combine_step32x(mdft_ctx *c, size_t n, int k, const double *coef,
 }
 
 hipError_t vec_axpy32x(mdft_ctx *c, size_t n, double a, const float *z, const float *x, float *y) {
-    hipLaunchKernelGGL(axpy32x, dim3(blocks_for(n / 2)), dim3(kThreads), 0, c->stream, n, a, z, x, y);
+    hipLaunchKernelGGL(axpy32x, dim3(resident_blocks(c, axpy32x, n / 2)), dim3(kThreads), 0, c->stream, n, a, z, x, y);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
 }
 
 hipError_t vec_narrow(mdft_ctx *c, size_t n, const double *in, float *out) {
-    hipLaunchKernelGGL(narrow1, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
+    hipLaunchKernelGGL(narrow1, dim3(resident_blocks(c, narrow1, 2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
 }
 
 hipError_t vec_widen(mdft_ctx *c, size_t n, const float *in, double *out) {
-    hipLaunchKernelGGL(widen1, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
+    hipLaunchKernelGGL(widen1, dim3(resident_blocks(c, widen1, 2 * n)), dim3(kThreads), 0, c->stream, n, in, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(c->stream);
 }
 
 hipError_t vec_scale_orient(mdft_ctx *c, size_t n, int no, const double *fac, const void *in, bool in_f32, double *out) {
-    hipLaunchKernelGGL(scale_orient, dim3(blocks_for(2 * n)), dim3(kThreads), 0, c->stream, n, no, fac,
+    hipLaunchKernelGGL(scale_orient, dim3(resident_blocks(c, scale_orient, 2 * n)), dim3(kThreads), 0, c->stream, n, no, fac,
                        in_f32 ? nullptr : static_cast<const double *>(in), in_f32 ? static_cast<const float *>(in) : nullptr,
                        out);
     hipError_t e = hipGetLastError();
