@@ -299,7 +299,12 @@ contains
         stats(9) = g0
 900     continue
         ! the caller's xi/g buffers must hold the result
-        if (x32) then
+        if (x32 .and. scaled_in .and. rc == 0) then
+            ! xi = u / s straight from the FP32 iterate: one pass instead of a widening and
+            ! an unscaling of xi
+            rc = mdft_scale_variables(ctx, n, -1_c_int, 1_c_int, x, xi)
+            if (rc == 0) scaled_in = .false.
+        else if (x32) then
             if (rc == 0) rc = mdft_vec_widen_f32(ctx, n, x, xi)
         else if (rc == 0 .and. .not. c_associated(x, xi)) then
             rc = mdft_vec_lincomb(ctx, n, 1.0_dp, x, 0.0_dp, c_null_ptr, xi)
