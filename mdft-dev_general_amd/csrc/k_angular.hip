@@ -492,10 +492,14 @@ __global__ __launch_bounds__(kThreads) void angl2proj_t(const double *__restrict
     using D = Dims<L>;
     using V = typename C2<T>::t;
     constexpr int NO = D::NO, NP = D::NP, NTH = D::NTH, NPHI = D::NPHI, H1 = D::H1, W = D::W;
-    __shared__ T o[PT * NO];
+    // o (read by the psi step) and F (written by the phi step, after the barrier that ends
+    // the psi step) share their LDS: the smaller footprint lets more workgroups share a CU
+    constexpr size_t kOB = sizeof(T) * PT * NO, kFB = sizeof(V) * PT * NTH * W * H1;
+    __shared__ __align__(16) unsigned char oF[kOB > kFB ? kOB : kFB];
     __shared__ V G[PT * NTH * NPHI * H1];
-    __shared__ V F[PT * NTH * W * H1];
     __shared__ A2pTab<L, T> tb;
+    T *o = reinterpret_cast<T *>(oF);
+    V *F = reinterpret_cast<V *>(oF);
     const int tid = threadIdx.x;
     const size_t r0 = (size_t)blockIdx.x * PT;
     const int npts = (int)((N - r0) < (size_t)PT ? (N - r0) : (size_t)PT);
